@@ -1,0 +1,340 @@
+"""ctypes binding of libbghip.so (include/bghip.h) with numpy conveniences.
+
+This is plumbing: every computation happens in the HIP kernels behind the C ABI. Importing this
+module never falls back to anything else: if the shared library is missing, `load()` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libbghip.so")
+
+NEURON_LIF, NEURON_IZH = 0, 1
+SYN_SPIKING, SYN_DS, SYN_STDP, SYN_DYNSTDP = 0, 1, 2, 3
+ERR_RING_COLLISION, ERR_SPIKE_OVERFLOW, ERR_SLOT_OVERFLOW = 1, 2, 4
+
+_F4 = ctypes.c_float * 4
+_I4 = ctypes.c_int32 * 4
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_up = ctypes.POINTER(ctypes.c_uint32)
+_qp = ctypes.POINTER(ctypes.c_uint64)
+_bp = ctypes.POINTER(ctypes.c_uint8)
+
+
+class SynapseTypes(ctypes.Structure):
+    _fields_ = [("tau", _F4), ("decay", _F4), ("total_delay", _I4), ("U", _F4), ("D", _F4), ("F", _F4),
+                ("u0", ctypes.c_float), ("r0", ctypes.c_float),
+                ("Apos", ctypes.c_float), ("Aneg", ctypes.c_float), ("STDPgap", ctypes.c_float),
+                ("tauspost", ctypes.c_float), ("tauspre", ctypes.c_float), ("taupos", ctypes.c_float),
+                ("tauneg", ctypes.c_float), ("Wex", ctypes.c_float), ("mupos", ctypes.c_float),
+                ("muneg", ctypes.c_float), ("useFroemkeDanSTDP", ctypes.c_int32)]
+
+
+class NetDesc(ctypes.Structure):
+    _fields_ = [("num_neurons", ctypes.c_int32), ("max_synapses_per_neuron", ctypes.c_int32),
+                ("neuron_model", ctypes.c_int32), ("synapse_model", ctypes.c_int32),
+                ("ring_bits", ctypes.c_int32), ("max_spikes", ctypes.c_int32), ("dt", ctypes.c_float),
+                ("noise_seed", ctypes.c_uint64), ("block_steps", ctypes.c_int32), ("types", SynapseTypes)]
+
+
+class Neurons(ctypes.Structure):
+    _fields_ = [(n, _fp) for n in ("Vm", "Vthresh", "Vreset", "I0", "Inoise", "C1", "C2", "Trefract",
+                                   "summation_map")] + \
+               [("nStepsInRefr", _ip)] + \
+               [(n, _fp) for n in ("Aconst", "Bconst", "Cconst", "Dconst", "u", "C3")] + \
+               [("spikeCount", _ip), ("spikeCountOffset", _ip), ("spike_history", _qp)]
+
+
+class Synapses(ctypes.Structure):
+    _fields_ = [("in_use", _bp), ("sourceNeuronIndex", _ip), ("destNeuronIndex", _ip), ("type", _ip),
+                ("W", _fp), ("psr", _fp), ("decay", _fp), ("total_delay", _ip), ("delayIdx", _ip),
+                ("delayQueue", _up), ("r", _fp), ("u", _fp), ("U", _fp), ("D", _fp), ("F", _fp),
+                ("lastSpike", _qp), ("delayQueuePost", _up), ("delayIdxPost", _ip)] + \
+               [(n, _fp) for n in ("Apos", "Aneg", "STDPgap", "tauspost", "tauspre", "taupos", "tauneg", "Wex",
+                                   "mupos", "muneg")] + \
+               [("useFroemkeDanSTDP", _bp), ("synapse_counts", _up)]
+
+
+class IndexMap(ctypes.Structure):
+    _fields_ = [(n, _up) for n in ("incomingSynapseIndexMap", "incomingSynapseBegin", "incomingSynapseCount",
+                                   "outgoingSynapseIndexMap", "outgoingSynapseBegin", "outgoingSynapseCount")]
+
+
+class Growth(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("epsilon", "beta", "rho", "targetRate", "maxRate", "minRadius",
+                                              "startRadius", "epochDuration")]
+
+
+class GrowthStats(ctypes.Structure):
+    _fields_ = [("total_synapses", ctypes.c_int64), ("created", ctypes.c_int64), ("erased", ctypes.c_int64)]
+
+
+NEURON_FIELDS_F = ("Vm", "Vthresh", "Vreset", "I0", "Inoise", "C1", "C2", "Trefract", "summation_map")
+IZH_FIELDS = ("Aconst", "Bconst", "Cconst", "Dconst", "u", "C3")
+SYN_DTYPES = {"in_use": np.uint8, "sourceNeuronIndex": np.int32, "destNeuronIndex": np.int32, "type": np.int32,
+              "W": np.float32, "psr": np.float32, "decay": np.float32, "total_delay": np.int32,
+              "delayIdx": np.int32, "delayQueue": np.uint32, "r": np.float32, "u": np.float32,
+              "U": np.float32, "D": np.float32, "F": np.float32, "lastSpike": np.uint64,
+              "delayQueuePost": np.uint32, "delayIdxPost": np.int32, "Apos": np.float32, "Aneg": np.float32,
+              "STDPgap": np.float32, "tauspost": np.float32, "tauspre": np.float32, "taupos": np.float32,
+              "tauneg": np.float32, "Wex": np.float32, "mupos": np.float32, "muneg": np.float32,
+              "useFroemkeDanSTDP": np.uint8}
+DS_FIELDS = ("r", "u", "U", "D", "F", "lastSpike")
+STDP_FIELDS = ("delayQueuePost", "delayIdxPost", "Apos", "Aneg", "STDPgap", "tauspost", "tauspre", "taupos",
+               "tauneg", "Wex", "mupos", "muneg", "useFroemkeDanSTDP")
+
+_lib = None
+
+
+def load():
+    """Load libbghip.so; raise if it has not been built (there is no other implementation)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    sig = {
+        "bghip_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+        "bghip_ctx_destroy": (None, [vp]),
+        "bghip_last_error": (ctypes.c_char_p, [vp]),
+        "bghip_ctx_mem_info": (ctypes.c_int, [vp, _qp, _qp]),
+        "bghip_net_create": (ctypes.c_int, [vp, ctypes.POINTER(NetDesc), ctypes.POINTER(vp)]),
+        "bghip_net_destroy": (None, [vp]),
+        "bghip_net_footprint": (ctypes.c_int64, [ctypes.POINTER(NetDesc)]),
+        "bghip_upload_neurons": (ctypes.c_int, [vp, ctypes.POINTER(Neurons)]),
+        "bghip_upload_synapses": (ctypes.c_int, [vp, ctypes.POINTER(Synapses), ctypes.c_uint64]),
+        "bghip_upload_layout": (ctypes.c_int, [vp, _fp, _fp, _ip]),
+        "bghip_upload_growth": (ctypes.c_int, [vp, ctypes.POINTER(Growth), _fp, _fp]),
+        "bghip_advance": (ctypes.c_int, [vp, ctypes.c_uint64]),
+        "bghip_update_connections": (ctypes.c_int, [vp, ctypes.POINTER(GrowthStats)]),
+        "bghip_update_history": (ctypes.c_int, [vp]),
+        "bghip_sync": (ctypes.c_int, [vp]),
+        "bghip_net_error_bits": (ctypes.c_uint32, [vp]),
+        "bghip_net_step": (ctypes.c_uint64, [vp]),
+        "bghip_download_neurons": (ctypes.c_int, [vp, ctypes.POINTER(Neurons)]),
+        "bghip_download_synapses": (ctypes.c_int, [vp, ctypes.POINTER(Synapses)]),
+        "bghip_total_synapses": (ctypes.c_int64, [vp]),
+        "bghip_download_index_map": (ctypes.c_int, [vp, ctypes.POINTER(IndexMap)]),
+        "bghip_download_growth": (ctypes.c_int, [vp, _fp, _fp]),
+        "bghip_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(ctypes.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_ctx_mem_info", "bghip_net_create",
+            "bghip_net_destroy", "bghip_net_footprint", "bghip_upload_neurons", "bghip_upload_synapses",
+            "bghip_upload_layout", "bghip_upload_growth", "bghip_advance", "bghip_update_connections",
+            "bghip_update_history", "bghip_sync", "bghip_net_error_bits", "bghip_net_step",
+            "bghip_download_neurons", "bghip_download_synapses", "bghip_total_synapses",
+            "bghip_download_index_map", "bghip_download_growth", "bghip_kernel_timing")
+
+
+def _ptr(a: np.ndarray | None, ct):
+    if a is None:
+        return ctypes.cast(None, ct)
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ct)
+
+
+_CT = {np.float32: _fp, np.int32: _ip, np.uint32: _up, np.uint64: _qp, np.uint8: _bp}
+
+
+class BgHipError(RuntimeError):
+    pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.bghip_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise BgHipError(f"bghip_ctx_create({device}) = {rc}")
+        self.h = h
+
+    def error(self) -> str:
+        return load().bghip_last_error(self.h).decode()
+
+    def mem_info(self):
+        f, t = ctypes.c_uint64(), ctypes.c_uint64()
+        self.check(load().bghip_ctx_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)), "mem_info")
+        return f.value, t.value
+
+    def check(self, rc: int, what: str):
+        if rc != 0:
+            raise BgHipError(f"{what} -> {rc}: {self.error()}")
+
+    def close(self):
+        if self.h:
+            load().bghip_ctx_destroy(self.h)
+            self.h = None
+
+
+class Net:
+    """One device network: the GPU-side replacement of GPUSpikingModel's device structs."""
+
+    def __init__(self, ctx: Context, desc: NetDesc):
+        self.ctx = ctx
+        self.desc = desc
+        h = ctypes.c_void_p()
+        ctx.check(load().bghip_net_create(ctx.h, ctypes.byref(desc), ctypes.byref(h)), "net_create")
+        self.h = h
+        self.N = desc.num_neurons
+        self.M = desc.max_synapses_per_neuron
+        self.W32 = desc.ring_bits // 32
+
+    def close(self):
+        if self.h:
+            load().bghip_net_destroy(self.h)
+            self.h = None
+
+    def _c(self, rc, what):
+        self.ctx.check(rc, what)
+
+    # ---- uploads
+    def upload_neurons(self, arrs: dict):
+        nb = Neurons()
+        keep = []
+        for f in NEURON_FIELDS_F + IZH_FIELDS:
+            if f in arrs and arrs[f] is not None:
+                a = np.ascontiguousarray(arrs[f], dtype=np.float32)
+                keep.append(a)
+                setattr(nb, f, _ptr(a, _fp))
+        for f in ("nStepsInRefr", "spikeCount", "spikeCountOffset"):
+            if f in arrs:
+                a = np.ascontiguousarray(arrs[f], dtype=np.int32)
+                keep.append(a)
+                setattr(nb, f, _ptr(a, _ip))
+        if "spike_history" in arrs:
+            a = np.ascontiguousarray(arrs["spike_history"], dtype=np.uint64)
+            keep.append(a)
+            nb.spike_history = _ptr(a, _qp)
+        self._c(load().bghip_upload_neurons(self.h, ctypes.byref(nb)), "upload_neurons")
+
+    def upload_synapses(self, arrs: dict, step: int):
+        sb = Synapses()
+        keep = []
+        for f, dt in SYN_DTYPES.items():
+            if f in arrs and arrs[f] is not None:
+                a = np.ascontiguousarray(arrs[f], dtype=dt)
+                keep.append(a)
+                setattr(sb, f, _ptr(a, _CT[dt]))
+        a = np.ascontiguousarray(arrs["synapse_counts"], dtype=np.uint32)
+        keep.append(a)
+        sb.synapse_counts = _ptr(a, _up)
+        self._c(load().bghip_upload_synapses(self.h, ctypes.byref(sb), step), "upload_synapses")
+
+    def upload_layout(self, xloc, yloc, ntype):
+        x = np.ascontiguousarray(xloc, np.float32)
+        y = np.ascontiguousarray(yloc, np.float32)
+        t = np.ascontiguousarray(ntype, np.int32)
+        self._c(load().bghip_upload_layout(self.h, _ptr(x, _fp), _ptr(y, _fp), _ptr(t, _ip)), "upload_layout")
+
+    def upload_growth(self, g: Growth, radii, rates=None):
+        r = np.ascontiguousarray(radii, np.float32)
+        q = np.ascontiguousarray(rates, np.float32) if rates is not None else None
+        self._c(load().bghip_upload_growth(self.h, ctypes.byref(g), _ptr(r, _fp), _ptr(q, _fp)), "upload_growth")
+
+    # ---- simulation
+    def advance(self, n_steps: int):
+        self._c(load().bghip_advance(self.h, n_steps), "advance")
+
+    def update_connections(self) -> GrowthStats:
+        st = GrowthStats()
+        self._c(load().bghip_update_connections(self.h, ctypes.byref(st)), "update_connections")
+        return st
+
+    def update_history(self):
+        self._c(load().bghip_update_history(self.h), "update_history")
+
+    def sync(self):
+        self._c(load().bghip_sync(self.h), "sync")
+
+    def error_bits(self) -> int:
+        return load().bghip_net_error_bits(self.h)
+
+    @property
+    def step(self) -> int:
+        return load().bghip_net_step(self.h)
+
+    @property
+    def total_synapses(self) -> int:
+        return load().bghip_total_synapses(self.h)
+
+    # ---- downloads
+    def download_neurons(self, max_spikes: int) -> dict:
+        N = self.N
+        izh = self.desc.neuron_model == NEURON_IZH
+        out = {f: np.zeros(N, np.float32) for f in NEURON_FIELDS_F}
+        if izh:
+            out.update({f: np.zeros(N, np.float32) for f in IZH_FIELDS})
+        for f in ("nStepsInRefr", "spikeCount", "spikeCountOffset"):
+            out[f] = np.zeros(N, np.int32)
+        out["spike_history"] = np.zeros(N * max_spikes, np.uint64)
+        nb = Neurons()
+        for f, a in out.items():
+            setattr(nb, f, _ptr(a, _CT[a.dtype.type]))
+        self._c(load().bghip_download_neurons(self.h, ctypes.byref(nb)), "download_neurons")
+        return out
+
+    def download_synapses(self) -> dict:
+        S = self.N * self.M
+        sk = self.desc.synapse_model
+        fields = ["in_use", "sourceNeuronIndex", "destNeuronIndex", "type", "W", "psr", "decay", "total_delay",
+                  "delayIdx", "delayQueue"]
+        if sk in (SYN_DS, SYN_DYNSTDP):
+            fields += list(DS_FIELDS)
+        if sk in (SYN_STDP, SYN_DYNSTDP):
+            fields += list(STDP_FIELDS)
+        out = {}
+        sb = Synapses()
+        for f in fields:
+            n = S * self.W32 if f == "delayQueue" else S
+            out[f] = np.zeros(n, SYN_DTYPES[f])
+            setattr(sb, f, _ptr(out[f], _CT[SYN_DTYPES[f]]))
+        out["synapse_counts"] = np.zeros(self.N, np.uint32)
+        sb.synapse_counts = _ptr(out["synapse_counts"], _up)
+        self._c(load().bghip_download_synapses(self.h, ctypes.byref(sb)), "download_synapses")
+        return out
+
+    def download_index_map(self) -> dict:
+        T = self.total_synapses
+        N = self.N
+        out = {"incomingSynapseIndexMap": np.zeros(T, np.uint32), "incomingSynapseBegin": np.zeros(N, np.uint32),
+               "incomingSynapseCount": np.zeros(N, np.uint32), "outgoingSynapseIndexMap": np.zeros(T, np.uint32),
+               "outgoingSynapseBegin": np.zeros(N, np.uint32), "outgoingSynapseCount": np.zeros(N, np.uint32)}
+        im = IndexMap()
+        for f, a in out.items():
+            setattr(im, f, _ptr(a, _up))
+        self._c(load().bghip_download_index_map(self.h, ctypes.byref(im)), "download_index_map")
+        return out
+
+    def download_growth(self):
+        r = np.zeros(self.N, np.float32)
+        q = np.zeros(self.N, np.float32)
+        self._c(load().bghip_download_growth(self.h, _ptr(r, _fp), _ptr(q, _fp)), "download_growth")
+        return r, q
+
+    def timing(self, enable: bool):
+        avg = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        self._c(load().bghip_kernel_timing(self.h, 1 if enable else 0, ctypes.byref(avg), ctypes.byref(cnt)),
+                "kernel_timing")
+        return avg.value, cnt.value
+
+
+def footprint(desc: NetDesc) -> int:
+    return load().bghip_net_footprint(ctypes.byref(desc))

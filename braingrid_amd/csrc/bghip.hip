@@ -1,0 +1,657 @@
+// libbghip.so: the C ABI of include/bghip.h on top of the kernels in bghip_kernels.h.
+//
+// One network = one HIP stream + one device arena (a single hipMalloc carved into the SoA
+// arrays). The host side converts between the reference's array conventions (relative delay-ring
+// orientation, per-synapse delayIdx, separate type/source/delay arrays) and the device's packed
+// slot layout only at upload and download time.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <new>
+#include <vector>
+#include "../../include/bghip.h"
+#include "bghip_kernels_epoch.h"
+#include "bghip_launch.h"
+
+struct bghip_ctx {
+    int device;
+    char err[512];
+};
+
+struct bghip_net {
+    bghip_ctx *ctx;
+    bghip_net_desc desc;
+    DevNet dn;
+    hipStream_t stream;
+    char *arena;
+    size_t arena_bytes;
+    uint64_t step;
+    int kmax;              // steps per fused launch
+    int sel;               // mask buffer holding the most recent block's fire masks
+    uint64_t last_t0;
+    bool pending;          // most recent block's spikes not yet inserted into the rings
+    int min_delay_uploaded;
+    bool has_layout, has_growth;
+    bghip_growth growth;
+    uint32_t *d_out_map, *d_out_begin, *d_out_count, *d_cursor, *d_stats;
+    uint64_t *d_total;
+    uint64_t total;
+    bool out_valid;
+    hipEvent_t ev0, ev1;
+    bool timing;
+    int64_t timed_launches;
+};
+
+static int set_err(bghip_ctx *ctx, int code, const char *fmt, ...)
+{
+    if (ctx) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(ctx->err, sizeof ctx->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                       \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return set_err((ctx), BGHIP_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------------
+extern "C" int bghip_ctx_create(int device, bghip_ctx **out)
+{
+    if (!out) return BGHIP_EINVAL;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return BGHIP_EDEVICE;
+    if (hipSetDevice(device) != hipSuccess) return BGHIP_EDEVICE;
+    bghip_ctx *c = new (std::nothrow) bghip_ctx();
+    if (!c) return BGHIP_ENOMEM;
+    c->device = device;
+    c->err[0] = 0;
+    *out = c;
+    return BGHIP_OK;
+}
+
+extern "C" void bghip_ctx_destroy(bghip_ctx *ctx) { delete ctx; }
+extern "C" const char *bghip_last_error(const bghip_ctx *ctx) { return ctx ? ctx->err : "no context"; }
+
+extern "C" int bghip_ctx_mem_info(bghip_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes)
+{
+    size_t f = 0, t = 0;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return BGHIP_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Arena layout. Every array is 256-byte aligned.
+struct ArenaPlan {
+    size_t off = 0;
+    size_t take(size_t bytes)
+    {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    }
+};
+
+static bool desc_ok(const bghip_net_desc *d)
+{
+    if (!d || d->num_neurons <= 0 || d->num_neurons > BG_MAX_NEURONS) return false;
+    if (d->max_synapses_per_neuron < 0 || d->max_spikes <= 0) return false;
+    int R = d->ring_bits;
+    if (R != 32 && R != 64 && R != 128 && R != 256) return false;
+    if (d->neuron_model < 0 || d->neuron_model > 1 || d->synapse_model < 0 || d->synapse_model > 3) return false;
+    if (!(d->dt > 0)) return false;
+    for (int t = 0; t < 4; t++)
+        if (d->types.total_delay[t] < 1 || d->types.total_delay[t] >= R || d->types.total_delay[t] > 255) return false;
+    return true;
+}
+
+struct Carve {
+    ArenaPlan plan;
+    std::vector<std::pair<void **, size_t>> items;
+    template <typename T>
+    void add(T **p, size_t count)
+    {
+        items.push_back({(void **)p, plan.take(sizeof(T) * (count ? count : 1))});
+    }
+};
+
+static void plan_net(bghip_net *n, Carve &c)
+{
+    const bghip_net_desc &d = n->desc;
+    DevNet &dn = n->dn;
+    size_t N = d.num_neurons, S = (size_t)N * d.max_synapses_per_neuron, W32 = d.ring_bits / 32;
+    bool izh = d.neuron_model == BGHIP_NEURON_IZH;
+    bool ds = d.synapse_model == BGHIP_SYN_DS || d.synapse_model == BGHIP_SYN_DYNSTDP;
+    bool stdp = d.synapse_model == BGHIP_SYN_STDP || d.synapse_model == BGHIP_SYN_DYNSTDP;
+    c.add(&dn.Vm, N); c.add(&dn.Vth, N); c.add(&dn.Vreset, N); c.add(&dn.I0, N); c.add(&dn.Inoise, N);
+    c.add(&dn.C1, N); c.add(&dn.C2, N); c.add(&dn.Trefract, N); c.add(&dn.sp, N); c.add(&dn.refr, N);
+    c.add(&dn.refr_steps, N); c.add(&dn.cnt, N); c.add(&dn.off, N);
+    c.add(&dn.hist, N * (size_t)d.max_spikes); c.add(&dn.fired_step, N);
+    c.add(&dn.mask[0], N); c.add(&dn.mask[1], N);
+    if (izh) { c.add(&dn.A, N); c.add(&dn.B, N); c.add(&dn.C, N); c.add(&dn.D, N); c.add(&dn.u, N); c.add(&dn.C3, N); }
+    c.add(&dn.meta, S); c.add(&dn.phase, S); c.add(&dn.psr, S); c.add(&dn.W, S); c.add(&dn.ring, S * W32);
+    if (ds) { c.add(&dn.r, S); c.add(&dn.us, S); c.add(&dn.U, S); c.add(&dn.Dd, S); c.add(&dn.F, S); c.add(&dn.last, S); }
+    if (stdp) {
+        c.add(&dn.Apos, S); c.add(&dn.Aneg, S); c.add(&dn.gap, S); c.add(&dn.tauspost, S); c.add(&dn.tauspre, S);
+        c.add(&dn.taupos, S); c.add(&dn.tauneg, S); c.add(&dn.Wex, S); c.add(&dn.mupos, S); c.add(&dn.muneg, S);
+        c.add(&dn.useFD, S);
+    }
+    c.add(&dn.syn_count, N); c.add(&dn.slot_hi, N); c.add(&dn.in_map, S); c.add(&dn.in_begin, N);
+    c.add(&dn.xloc, N); c.add(&dn.yloc, N); c.add(&dn.radii, N); c.add(&dn.rates, N); c.add(&dn.ntype, N);
+    c.add(&dn.err, 1);
+    c.add(&n->d_out_map, S); c.add(&n->d_out_begin, N); c.add(&n->d_out_count, N); c.add(&n->d_cursor, N);
+    c.add(&n->d_stats, 4); c.add(&n->d_total, 1);
+}
+
+extern "C" int64_t bghip_net_footprint(const bghip_net_desc *desc)
+{
+    if (!desc_ok(desc)) return -1;
+    bghip_net tmp;
+    memset(&tmp, 0, sizeof tmp);
+    tmp.desc = *desc;
+    Carve c;
+    plan_net(&tmp, c);
+    return (int64_t)c.plan.off;
+}
+
+static int block_kmax(const bghip_net *n)
+{
+    int k = BG_KMAX;
+    for (int t = 0; t < 4; t++) k = std::min(k, (int)n->desc.types.total_delay[t]);
+    if (n->min_delay_uploaded > 0) k = std::min(k, n->min_delay_uploaded);
+    if (n->desc.block_steps > 0) k = std::min(k, (int)n->desc.block_steps);
+    if (n->desc.synapse_model == BGHIP_SYN_STDP || n->desc.synapse_model == BGHIP_SYN_DYNSTDP) k = 1;
+    return std::max(k, 1);
+}
+
+extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghip_net **out)
+{
+    if (!ctx || !out) return BGHIP_EINVAL;
+    if (!desc_ok(desc)) return set_err(ctx, BGHIP_EINVAL, "invalid network description");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    bghip_net *n = new (std::nothrow) bghip_net();
+    if (!n) return BGHIP_ENOMEM;
+    memset((void *)n, 0, sizeof *n);
+    n->ctx = ctx;
+    n->desc = *desc;
+    Carve c;
+    plan_net(n, c);
+    n->arena_bytes = c.plan.off;
+    if (hipMalloc((void **)&n->arena, n->arena_bytes) != hipSuccess) {
+        delete n;
+        return set_err(ctx, BGHIP_ENOMEM, "hipMalloc of %zu bytes failed", c.plan.off);
+    }
+    for (auto &it : c.items) *it.first = n->arena + it.second;
+    DevNet &dn = n->dn;
+    dn.N = desc->num_neurons;
+    dn.M = desc->max_synapses_per_neuron;
+    dn.R = desc->ring_bits;
+    dn.W32 = desc->ring_bits / 32;
+    dn.max_spikes = desc->max_spikes;
+    dn.dt = desc->dt;
+    dn.seed = desc->noise_seed;
+    dn.S = (uint64_t)dn.N * dn.M;
+    for (int t = 0; t < 4; t++) dn.decay[t] = desc->types.decay[t];
+    if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipFree(n->arena);
+        delete n;
+        return set_err(ctx, BGHIP_EDEVICE, "hipStreamCreate failed");
+    }
+    HIP_TRY(ctx, hipEventCreate(&n->ev0));
+    HIP_TRY(ctx, hipEventCreate(&n->ev1));
+    // zero everything (empty synapse table, zero rings/counters), then the sentinels
+    HIP_TRY(ctx, hipMemsetAsync(n->arena, 0, n->arena_bytes, n->stream));
+    HIP_TRY(ctx, hipMemsetAsync(dn.fired_step, 0xFF, sizeof(uint64_t) * dn.N, n->stream));
+    HIP_TRY(ctx, hipMemsetAsync(dn.hist, 0xFF, sizeof(uint64_t) * (size_t)dn.N * dn.max_spikes, n->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(n->stream));
+    n->kmax = block_kmax(n);
+    *out = n;
+    return BGHIP_OK;
+}
+
+extern "C" void bghip_net_destroy(bghip_net *n)
+{
+    if (!n) return;
+    (void)hipSetDevice(n->ctx->device);
+    (void)hipStreamSynchronize(n->stream);
+    (void)hipEventDestroy(n->ev0);
+    (void)hipEventDestroy(n->ev1);
+    (void)hipStreamDestroy(n->stream);
+    (void)hipFree(n->arena);
+    delete n;
+}
+
+extern "C" uint64_t bghip_net_step(const bghip_net *n) { return n ? n->step : 0; }
+
+template <typename T>
+static int h2d(bghip_net *n, T *dst, const T *src, size_t count)
+{
+    if (!src || !count) return BGHIP_OK;
+    HIP_TRY(n->ctx, hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyHostToDevice, n->stream));
+    return BGHIP_OK;
+}
+template <typename T>
+static int d2h(bghip_net *n, T *dst, const T *src, size_t count)
+{
+    if (!dst || !count) return BGHIP_OK;
+    HIP_TRY(n->ctx, hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, n->stream));
+    return BGHIP_OK;
+}
+#define CHK(x) do { int rc_ = (x); if (rc_ != BGHIP_OK) return rc_; } while (0)
+
+// ------------------------------------------------------------------------------------------------
+extern "C" int bghip_upload_neurons(bghip_net *n, const bghip_neurons *h)
+{
+    if (!n || !h) return BGHIP_EINVAL;
+    DevNet &dn = n->dn;
+    size_t N = dn.N;
+    if (!h->Vm || !h->Vthresh || !h->Vreset || !h->I0 || !h->Inoise || !h->C1 || !h->C2 || !h->Trefract)
+        return set_err(n->ctx, BGHIP_EINVAL, "upload_neurons: missing LIF field");
+    bool izh = n->desc.neuron_model == BGHIP_NEURON_IZH;
+    if (izh && (!h->Aconst || !h->Bconst || !h->Cconst || !h->Dconst || !h->u || !h->C3))
+        return set_err(n->ctx, BGHIP_EINVAL, "upload_neurons: missing Izhikevich field");
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    // refractory length, AllLIFNeurons::fire (Neurons/AllLIFNeurons.cpp:73), computed on the host
+    std::vector<int32_t> rs(N);
+    for (size_t i = 0; i < N; i++) rs[i] = (int32_t)((double)(h->Trefract[i] / n->desc.dt) + 0.5);
+    CHK(h2d(n, dn.Vm, h->Vm, N)); CHK(h2d(n, dn.Vth, h->Vthresh, N)); CHK(h2d(n, dn.Vreset, h->Vreset, N));
+    CHK(h2d(n, dn.I0, h->I0, N)); CHK(h2d(n, dn.Inoise, h->Inoise, N)); CHK(h2d(n, dn.C1, h->C1, N));
+    CHK(h2d(n, dn.C2, h->C2, N)); CHK(h2d(n, dn.Trefract, h->Trefract, N)); CHK(h2d(n, dn.refr_steps, rs.data(), N));
+    CHK(h2d(n, dn.sp, h->summation_map, N)); CHK(h2d(n, dn.refr, h->nStepsInRefr, N));
+    CHK(h2d(n, dn.cnt, h->spikeCount, N)); CHK(h2d(n, dn.off, h->spikeCountOffset, N));
+    CHK(h2d(n, dn.hist, h->spike_history, N * (size_t)dn.max_spikes));
+    if (izh) {
+        CHK(h2d(n, dn.A, h->Aconst, N)); CHK(h2d(n, dn.B, h->Bconst, N)); CHK(h2d(n, dn.C, h->Cconst, N));
+        CHK(h2d(n, dn.D, h->Dconst, N)); CHK(h2d(n, dn.u, h->u, N)); CHK(h2d(n, dn.C3, h->C3, N));
+    }
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    return BGHIP_OK;
+}
+
+static int rebuild_incoming(bghip_net *n)
+{
+    DevNet &dn = n->dn;
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, n->stream, dn.syn_count, dn.in_begin, dn.N, n->d_total);
+    unsigned blocks = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
+    hipLaunchKernelGGL(k_fill_incoming, dim3(blocks), dim3(BG_BLOCK), 0, n->stream, dn);
+    HIP_TRY(n->ctx, hipGetLastError());
+    HIP_TRY(n->ctx, hipMemcpyAsync(&n->total, n->d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, n->stream));
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    n->out_valid = false;
+    return BGHIP_OK;
+}
+
+static inline uint32_t ring_get(const uint32_t *w, int b) { return (w[b >> 5] >> (b & 31)) & 1u; }
+
+extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint64_t step)
+{
+    if (!n || !h) return BGHIP_EINVAL;
+    DevNet &dn = n->dn;
+    const int N = dn.N, M = dn.M, R = dn.R, W32 = dn.W32;
+    const size_t S = dn.S;
+    const int sk = n->desc.synapse_model;
+    const bool ds = sk == BGHIP_SYN_DS || sk == BGHIP_SYN_DYNSTDP;
+    const bool stdp = sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
+    if (!h->in_use || !h->sourceNeuronIndex || !h->type || !h->W || !h->psr || !h->total_delay || !h->delayQueue ||
+        !h->delayIdx || !h->synapse_counts)
+        return set_err(n->ctx, BGHIP_EINVAL, "upload_synapses: missing field");
+    if (ds && (!h->r || !h->u || !h->U || !h->D || !h->F || !h->lastSpike))
+        return set_err(n->ctx, BGHIP_EINVAL, "upload_synapses: missing DS field");
+    if (stdp && (!h->Apos || !h->Aneg || !h->STDPgap || !h->tauspost || !h->tauspre || !h->taupos || !h->tauneg ||
+                 !h->Wex || !h->mupos || !h->muneg || !h->useFroemkeDanSTDP))
+        return set_err(n->ctx, BGHIP_EINVAL, "upload_synapses: missing STDP field");
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    std::vector<uint32_t> meta(S, 0u), ring((size_t)W32 * S, 0u), cnt(N, 0u), hi(N, 0u);
+    std::vector<uint8_t> phase(S, 0);
+    int min_delay = 255;
+    for (int d = 0; d < N; d++) {
+        for (int k = 0; k < M; k++) {
+            size_t s = (size_t)d * M + k;
+            if (!h->in_use[s]) continue;
+            int src = h->sourceNeuronIndex[s], t = h->type[s], del = h->total_delay[s], di = h->delayIdx[s];
+            if (src < 0 || src >= N || t < 0 || t > 3 || del < 1 || del >= R || del > 255 || di < 0 || di >= R)
+                return set_err(n->ctx, BGHIP_EINVAL, "upload_synapses: slot %zu out of range", s);
+            if (h->decay && memcmp(&h->decay[s], &dn.decay[t], 4) != 0)
+                return set_err(n->ctx, BGHIP_EUNSUPPORTED, "upload_synapses: slot %zu decay differs from its type's", s);
+            if (stdp && h->delayQueuePost && h->delayQueuePost[s] != 0)
+                return set_err(n->ctx, BGHIP_EUNSUPPORTED, "upload_synapses: pending post-spike at a step boundary");
+            if (stdp && h->delayIdxPost && (h->delayIdxPost[s] & 31) != (di & 31))
+                return set_err(n->ctx, BGHIP_EUNSUPPORTED, "upload_synapses: pre/post ring phases differ");
+            min_delay = std::min(min_delay, del);
+            meta[s] = (uint32_t)src | ((uint32_t)t << BG_META_TYPE_SHIFT) | ((uint32_t)del << BG_META_DELAY_SHIFT) |
+                      BG_META_IN_USE;
+            // reference bit b (relative to delayIdx) -> absolute bit (b + phase) mod R,
+            // phase = (step - delayIdx) mod R
+            uint32_t ph = (uint32_t)((step - (uint64_t)di) & (uint64_t)(R - 1));
+            phase[s] = (uint8_t)ph;
+            const uint32_t *src_ring = &h->delayQueue[s * W32];
+            for (int b = 0; b < R; b++)
+                if (ring_get(src_ring, b)) {
+                    int a = (b + (int)ph) & (R - 1);
+                    ring[(size_t)(a >> 5) * S + s] |= 1u << (a & 31);
+                }
+            cnt[d]++;
+            hi[d] = k + 1;
+        }
+        if (cnt[d] != h->synapse_counts[d])
+            return set_err(n->ctx, BGHIP_EINVAL, "upload_synapses: synapse_counts[%d] disagrees with in_use", d);
+    }
+    n->min_delay_uploaded = cnt.empty() ? 0 : min_delay;
+    CHK(h2d(n, dn.meta, meta.data(), S)); CHK(h2d(n, dn.phase, phase.data(), S));
+    CHK(h2d(n, dn.ring, ring.data(), (size_t)W32 * S)); CHK(h2d(n, dn.psr, h->psr, S)); CHK(h2d(n, dn.W, h->W, S));
+    CHK(h2d(n, dn.syn_count, cnt.data(), N)); CHK(h2d(n, dn.slot_hi, hi.data(), N));
+    if (ds) {
+        CHK(h2d(n, dn.r, h->r, S)); CHK(h2d(n, dn.us, h->u, S)); CHK(h2d(n, dn.U, h->U, S));
+        CHK(h2d(n, dn.Dd, h->D, S)); CHK(h2d(n, dn.F, h->F, S)); CHK(h2d(n, dn.last, h->lastSpike, S));
+    }
+    if (stdp) {
+        CHK(h2d(n, dn.Apos, h->Apos, S)); CHK(h2d(n, dn.Aneg, h->Aneg, S)); CHK(h2d(n, dn.gap, h->STDPgap, S));
+        CHK(h2d(n, dn.tauspost, h->tauspost, S)); CHK(h2d(n, dn.tauspre, h->tauspre, S));
+        CHK(h2d(n, dn.taupos, h->taupos, S)); CHK(h2d(n, dn.tauneg, h->tauneg, S)); CHK(h2d(n, dn.Wex, h->Wex, S));
+        CHK(h2d(n, dn.mupos, h->mupos, S)); CHK(h2d(n, dn.muneg, h->muneg, S));
+        CHK(h2d(n, dn.useFD, h->useFroemkeDanSTDP, S));
+    }
+    n->step = step;
+    n->pending = false;
+    n->kmax = block_kmax(n);
+    return rebuild_incoming(n);
+}
+
+extern "C" int bghip_upload_layout(bghip_net *n, const float *xloc, const float *yloc, const int32_t *ntype)
+{
+    if (!n || !xloc || !yloc || !ntype) return BGHIP_EINVAL;
+    size_t N = n->dn.N;
+    for (size_t i = 0; i < N; i++)
+        if (ntype[i] != 1 && ntype[i] != 2) return set_err(n->ctx, BGHIP_EINVAL, "neuron type must be INH(1)/EXC(2)");
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    CHK(h2d(n, n->dn.xloc, xloc, N)); CHK(h2d(n, n->dn.yloc, yloc, N)); CHK(h2d(n, n->dn.ntype, ntype, N));
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    n->has_layout = true;
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_upload_growth(bghip_net *n, const bghip_growth *g, const float *radii, const float *rates)
+{
+    if (!n || !g || !radii) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    n->growth = *g;
+    size_t N = n->dn.N;
+    CHK(h2d(n, n->dn.radii, radii, N));
+    if (rates) CHK(h2d(n, n->dn.rates, rates, N));
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    n->has_growth = true;
+    return BGHIP_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+static void flush_pending(bghip_net *n)
+{
+    if (!n->pending) return;
+    DevNet &dn = n->dn;
+    unsigned blocks = (unsigned)((dn.S + BG_BLOCK - 1) / BG_BLOCK);
+    if (blocks) hipLaunchKernelGGL(k_flush_block, dim3(blocks), dim3(BG_BLOCK), 0, n->stream, dn, n->last_t0, n->sel);
+    n->pending = false;
+}
+
+extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
+{
+    if (!n) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    DevNet &dn = n->dn;
+    const int nk = n->desc.neuron_model, sk = n->desc.synapse_model;
+    const bool per_step = n->kmax <= 1 || sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
+    if (per_step) {
+        flush_pending(n);
+        for (uint64_t i = 0; i < n_steps; i++) {
+            uint64_t t = n->step + i;
+            launch_neuron_step(dn, n->stream, nk, t);
+            launch_synapse_step(dn, n->stream, sk, t);
+            n->timed_launches += 2;
+        }
+        n->step += n_steps;
+    } else {
+        uint64_t done = 0;
+        while (done < n_steps) {
+            int K = (int)std::min<uint64_t>((uint64_t)n->kmax, n_steps - done);
+            uint64_t t0 = n->step + done;
+            int pull = n->pending ? 1 : 0;
+            if (nk == BGHIP_NEURON_LIF)
+                launch_fused_lif(dn, n->stream, sk, n->kmax, t0, K, pull, n->last_t0, n->sel);
+            else
+                launch_fused_izh(dn, n->stream, sk, n->kmax, t0, K, pull, n->last_t0, n->sel);
+            n->sel ^= 1;
+            n->last_t0 = t0;
+            n->pending = true;
+            done += (uint64_t)K;
+            n->timed_launches += 1;
+        }
+        n->step += n_steps;
+        flush_pending(n);
+    }
+    HIP_TRY(n->ctx, hipGetLastError());
+    return BGHIP_OK;
+}
+
+extern "C" uint32_t bghip_net_error_bits(bghip_net *n)
+{
+    uint32_t e = 0;
+    if (!n) return 0;
+    (void)hipSetDevice(n->ctx->device);
+    (void)hipMemcpyAsync(&e, n->dn.err, 4, hipMemcpyDeviceToHost, n->stream);
+    (void)hipStreamSynchronize(n->stream);
+    return e;
+}
+
+extern "C" int bghip_sync(bghip_net *n)
+{
+    if (!n) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    uint32_t e = bghip_net_error_bits(n);
+    if (e) return set_err(n->ctx, BGHIP_ESTATE, "device error bits 0x%x", e);
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
+{
+    if (!n) return BGHIP_EINVAL;
+    if (!n->has_layout || !n->has_growth) return set_err(n->ctx, BGHIP_EINVAL, "layout/growth state not uploaded");
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    DevNet &dn = n->dn;
+    flush_pending(n);
+    const bghip_growth &g = n->growth;
+    unsigned nb = (unsigned)((dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    float c = g.epochDuration * g.rho;
+    hipLaunchKernelGGL(k_growth_radii, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, g.epochDuration, g.epsilon,
+                       g.maxRate, g.beta, c);
+    CreateCtx cc;
+    memset(&cc, 0, sizeof cc);
+    const bghip_synapse_types &ty = n->desc.types;
+    cc.W_adj = (float)1.0e-8;
+    for (int t = 0; t < 4; t++) {
+        cc.total_delay[t] = ty.total_delay[t]; cc.U[t] = ty.U[t]; cc.D[t] = ty.D[t]; cc.F[t] = ty.F[t];
+    }
+    cc.u0 = ty.u0; cc.r0 = ty.r0;
+    cc.Apos = ty.Apos; cc.Aneg = ty.Aneg; cc.gap = ty.STDPgap; cc.tauspost = ty.tauspost; cc.tauspre = ty.tauspre;
+    cc.taupos = ty.taupos; cc.tauneg = ty.tauneg; cc.Wex = ty.Wex; cc.mupos = ty.mupos; cc.muneg = ty.muneg;
+    cc.useFD = ty.useFroemkeDanSTDP;
+    cc.syn_kind = n->desc.synapse_model;
+    cc.step = n->step;
+    HIP_TRY(n->ctx, hipMemsetAsync(n->d_stats, 0, 16, n->stream));
+    if (dn.M > 0) {
+        unsigned grid = (unsigned)std::min(dn.N, 4096);
+        hipLaunchKernelGGL(k_growth_dest, dim3(grid), dim3(BG_BLOCK), 0, n->stream, dn, cc, n->d_stats);
+    }
+    HIP_TRY(n->ctx, hipGetLastError());
+    CHK(rebuild_incoming(n));
+    if (stats) {
+        uint32_t st[4];
+        HIP_TRY(n->ctx, hipMemcpy(st, n->d_stats, 16, hipMemcpyDeviceToHost));
+        stats->total_synapses = (int64_t)n->total;
+        stats->created = st[0];
+        stats->erased = st[1];
+    }
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_update_history(bghip_net *n)
+{
+    if (!n) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    unsigned nb = (unsigned)((n->dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    hipLaunchKernelGGL(k_update_history, dim3(nb), dim3(BG_BLOCK), 0, n->stream, n->dn, n->growth.minRadius,
+                       n->has_growth ? 1 : 0);
+    HIP_TRY(n->ctx, hipGetLastError());
+    return BGHIP_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+extern "C" int bghip_download_neurons(bghip_net *n, bghip_neurons *h)
+{
+    if (!n || !h) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    DevNet &dn = n->dn;
+    size_t N = dn.N;
+    CHK(d2h(n, h->Vm, dn.Vm, N)); CHK(d2h(n, h->Vthresh, dn.Vth, N)); CHK(d2h(n, h->Vreset, dn.Vreset, N));
+    CHK(d2h(n, h->I0, dn.I0, N)); CHK(d2h(n, h->Inoise, dn.Inoise, N)); CHK(d2h(n, h->C1, dn.C1, N));
+    CHK(d2h(n, h->C2, dn.C2, N)); CHK(d2h(n, h->Trefract, dn.Trefract, N)); CHK(d2h(n, h->summation_map, dn.sp, N));
+    CHK(d2h(n, h->nStepsInRefr, dn.refr, N)); CHK(d2h(n, h->spikeCount, dn.cnt, N));
+    CHK(d2h(n, h->spikeCountOffset, dn.off, N)); CHK(d2h(n, h->spike_history, dn.hist, N * (size_t)dn.max_spikes));
+    if (n->desc.neuron_model == BGHIP_NEURON_IZH) {
+        CHK(d2h(n, h->Aconst, dn.A, N)); CHK(d2h(n, h->Bconst, dn.B, N)); CHK(d2h(n, h->Cconst, dn.C, N));
+        CHK(d2h(n, h->Dconst, dn.D, N)); CHK(d2h(n, h->u, dn.u, N)); CHK(d2h(n, h->C3, dn.C3, N));
+    }
+    return bghip_sync(n);
+}
+
+extern "C" int64_t bghip_total_synapses(bghip_net *n) { return n ? (int64_t)n->total : -1; }
+
+extern "C" int bghip_download_synapses(bghip_net *n, bghip_synapses *h)
+{
+    if (!n || !h) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    DevNet &dn = n->dn;
+    const int N = dn.N, M = dn.M, R = dn.R, W32 = dn.W32;
+    const size_t S = dn.S;
+    const int sk = n->desc.synapse_model;
+    const bool ds = sk == BGHIP_SYN_DS || sk == BGHIP_SYN_DYNSTDP;
+    const bool stdp = sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
+    std::vector<uint32_t> meta(S), ring((size_t)W32 * S);
+    std::vector<uint8_t> phase(S);
+    CHK(d2h(n, meta.data(), dn.meta, S)); CHK(d2h(n, phase.data(), dn.phase, S));
+    CHK(d2h(n, ring.data(), dn.ring, (size_t)W32 * S));
+    CHK(d2h(n, h->psr, dn.psr, S)); CHK(d2h(n, h->W, dn.W, S)); CHK(d2h(n, h->synapse_counts, dn.syn_count, N));
+    if (ds) {
+        CHK(d2h(n, h->r, dn.r, S)); CHK(d2h(n, h->u, dn.us, S)); CHK(d2h(n, h->U, dn.U, S));
+        CHK(d2h(n, h->D, dn.Dd, S)); CHK(d2h(n, h->F, dn.F, S)); CHK(d2h(n, h->lastSpike, dn.last, S));
+    }
+    if (stdp) {
+        CHK(d2h(n, h->Apos, dn.Apos, S)); CHK(d2h(n, h->Aneg, dn.Aneg, S)); CHK(d2h(n, h->STDPgap, dn.gap, S));
+        CHK(d2h(n, h->tauspost, dn.tauspost, S)); CHK(d2h(n, h->tauspre, dn.tauspre, S));
+        CHK(d2h(n, h->taupos, dn.taupos, S)); CHK(d2h(n, h->tauneg, dn.tauneg, S)); CHK(d2h(n, h->Wex, dn.Wex, S));
+        CHK(d2h(n, h->mupos, dn.mupos, S)); CHK(d2h(n, h->muneg, dn.muneg, S));
+        CHK(d2h(n, h->useFroemkeDanSTDP, dn.useFD, S));
+    }
+    CHK(bghip_sync(n));
+    const uint64_t step = n->step;
+    for (size_t s = 0; s < S; s++) {
+        uint32_t m = meta[s];
+        bool use = (m & BG_META_IN_USE) != 0;
+        int type = (m >> BG_META_TYPE_SHIFT) & 3;
+        uint32_t ph = phase[s];
+        if (h->in_use) h->in_use[s] = use ? 1 : 0;
+        if (h->sourceNeuronIndex) h->sourceNeuronIndex[s] = use ? (int32_t)(m & BG_META_SRC_MASK) : 0;
+        if (h->destNeuronIndex) h->destNeuronIndex[s] = use ? (int32_t)(s / (size_t)M) : 0;
+        if (h->type) h->type[s] = use ? type : -1;
+        if (h->total_delay) h->total_delay[s] = use ? (int32_t)((m >> BG_META_DELAY_SHIFT) & 0xFF) : 0;
+        if (h->decay) h->decay[s] = use ? dn.decay[type] : 0.0f;
+        int di = (int)((step - (uint64_t)ph) & (uint64_t)(R - 1));
+        if (h->delayIdx) h->delayIdx[s] = use ? di : 0;
+        if (h->delayQueue) {
+            uint32_t *dst = &h->delayQueue[s * W32];
+            for (int w = 0; w < W32; w++) dst[w] = 0u;
+            if (use)
+                for (int a = 0; a < R; a++)
+                    if ((ring[(size_t)(a >> 5) * S + s] >> (a & 31)) & 1u) {
+                        int b = (a - (int)ph) & (R - 1);
+                        dst[b >> 5] |= 1u << (b & 31);
+                    }
+        }
+        if (stdp) {
+            if (h->delayQueuePost) h->delayQueuePost[s] = 0u;
+            if (h->delayIdxPost) h->delayIdxPost[s] = use ? (int32_t)((step - (uint64_t)ph) & 31ull) : 0;
+        }
+    }
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_download_index_map(bghip_net *n, bghip_index_map *h)
+{
+    if (!n || !h) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    DevNet &dn = n->dn;
+    size_t N = dn.N, T = n->total;
+    if (!n->out_valid && T > 0) {
+        HIP_TRY(n->ctx, hipMemsetAsync(n->d_out_count, 0, 4 * N, n->stream));
+        HIP_TRY(n->ctx, hipMemsetAsync(n->d_cursor, 0, 4 * N, n->stream));
+        unsigned nb = (unsigned)((T + BG_BLOCK - 1) / BG_BLOCK);
+        hipLaunchKernelGGL(k_count_outgoing, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, dn.in_map, (uint64_t)T,
+                           n->d_out_count);
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, n->stream, n->d_out_count, n->d_out_begin, dn.N,
+                           n->d_total);
+        hipLaunchKernelGGL(k_place_outgoing, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, dn.in_map, (uint64_t)T,
+                           n->d_out_begin, n->d_cursor, n->d_out_map);
+        hipLaunchKernelGGL(k_sort_outgoing, dim3(dn.N), dim3(BG_BLOCK), 0, n->stream, n->d_out_begin,
+                           n->d_out_count, n->d_out_map);
+        HIP_TRY(n->ctx, hipGetLastError());
+        n->out_valid = true;
+    }
+    std::vector<uint32_t> cnt(N);
+    CHK(d2h(n, h->incomingSynapseIndexMap, dn.in_map, T)); CHK(d2h(n, h->incomingSynapseBegin, dn.in_begin, N));
+    CHK(d2h(n, cnt.data(), dn.syn_count, N));
+    CHK(d2h(n, h->outgoingSynapseIndexMap, n->d_out_map, T)); CHK(d2h(n, h->outgoingSynapseBegin, n->d_out_begin, N));
+    CHK(d2h(n, h->outgoingSynapseCount, n->d_out_count, N));
+    CHK(bghip_sync(n));
+    if (h->incomingSynapseCount) memcpy(h->incomingSynapseCount, cnt.data(), 4 * N);
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_download_growth(bghip_net *n, float *radii, float *rates)
+{
+    if (!n) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    CHK(d2h(n, radii, n->dn.radii, n->dn.N));
+    CHK(d2h(n, rates, n->dn.rates, n->dn.N));
+    return bghip_sync(n);
+}
+
+extern "C" int bghip_kernel_timing(bghip_net *n, int enable, double *avg_ms, int64_t *count)
+{
+    if (!n) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    if (enable) {
+        n->timed_launches = 0;
+        HIP_TRY(n->ctx, hipEventRecord(n->ev0, n->stream));
+        n->timing = true;
+        return BGHIP_OK;
+    }
+    if (!n->timing) return BGHIP_EINVAL;
+    HIP_TRY(n->ctx, hipEventRecord(n->ev1, n->stream));
+    HIP_TRY(n->ctx, hipEventSynchronize(n->ev1));
+    float ms = 0.0f;
+    HIP_TRY(n->ctx, hipEventElapsedTime(&ms, n->ev0, n->ev1));
+    n->timing = false;
+    if (count) *count = n->timed_launches;
+    if (avg_ms) *avg_ms = n->timed_launches ? (double)ms / (double)n->timed_launches : 0.0;
+    return BGHIP_OK;
+}

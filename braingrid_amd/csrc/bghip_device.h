@@ -1,0 +1,272 @@
+// Device-side network arena and the per-element model arithmetic shared by all kernels.
+//
+// Layout in HBM (one arena per network):
+//   neurons   SoA float/int arrays of length N (Vm, Vthresh, ... exactly the reference's fields)
+//   synapses  slot arrays of length S = N*M, destination-major (iSyn = M*dest + k), the layout of
+//             Synapses/AllSynapses.cpp:366-371; `meta` packs source | type | total_delay | in_use
+//   rings     ring_words x S, word-major (ring[w*S + slot]) so a step touches one coalesced word
+//   history   uint64 N x max_spikes (Neurons/AllSpikingNeurons.h:185-204)
+// Delay rings are kept in absolute-time orientation: the spike delivered at step T sits at bit
+// T mod R. The reference's per-synapse delayIdx equals (T - phase) mod R, where phase is the
+// creation step mod R; conversion happens at upload/download time only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/bg_detmath.h"
+
+#define BG_META_SRC_MASK 0x3FFFFu
+#define BG_META_TYPE_SHIFT 18
+#define BG_META_DELAY_SHIFT 20
+#define BG_META_IN_USE 0x80000000u
+#define BG_MAX_NEURONS 262144
+#define BG_NO_SPIKE 0xFFFFFFFFFFFFFFFFull
+
+enum { NK_LIF = 0, NK_IZH = 1 };
+enum { SK_SPIKING = 0, SK_DS = 1, SK_STDP = 2, SK_DYNSTDP = 3 };
+
+struct DevNet {
+    int N, M, R, W32, max_spikes;
+    float dt;
+    uint64_t seed;
+    uint64_t S;  // N*M
+    // neurons
+    float *Vm, *Vth, *Vreset, *I0, *Inoise, *C1, *C2, *Trefract, *sp;
+    int *refr, *refr_steps;
+    float *A, *B, *C, *D, *u, *C3;
+    int *cnt, *off;
+    uint64_t *hist;
+    uint64_t *fired_step;  // last firing step (per-step path), BG_NO_SPIKE if none
+    uint32_t *mask[2];     // per-block fire masks (fused path)
+    // synapses
+    uint32_t *meta;
+    uint8_t *phase;
+    float *psr, *W;
+    uint32_t *ring;
+    float *r, *us, *U, *Dd, *F;
+    uint64_t *last;
+    float *Apos, *Aneg, *gap, *tauspost, *tauspre, *taupos, *tauneg, *Wex, *mupos, *muneg;
+    uint8_t *useFD;
+    uint32_t *syn_count, *slot_hi, *in_map, *in_begin;
+    // growth
+    float *xloc, *yloc, *radii, *rates;
+    int32_t *ntype;
+    float decay[4];
+    uint32_t *err;
+};
+
+__device__ __forceinline__ void dev_error(const DevNet &n, uint32_t bit) { atomicOr(n.err, bit); }
+
+// ---------------------------------------------------------------------------------------------
+// Neuron advance. LIF: Neurons/AllLIFNeurons.cpp:20-77; IZH: Neurons/AllIZHNeurons.cpp:570-655;
+// spike recording: Neurons/AllSpikingNeurons.cpp:155-173. The summation point value `sp` is the
+// synaptic input accumulated in the previous step; the caller zeroes it afterwards.
+// Returns true when the neuron fires at step t.
+struct NeuronRegs {
+    float Vm, Vth, Vreset, I0, Inoise, C1, C2, u, A, B, C, D, C3;
+    int refr, refr_steps, cnt, off;
+};
+
+template <int NK>
+__device__ __forceinline__ bool neuron_advance(const DevNet &n, int i, uint64_t t, float sp, NeuronRegs &s)
+{
+    if (s.refr > 0) {
+        --s.refr;
+        return false;
+    }
+    if (s.Vm >= s.Vth) {
+        int ms = n.max_spikes;
+        int idx = (s.cnt + s.off) % ms;
+        n.hist[(uint64_t)i * ms + idx] = t;
+        s.cnt++;
+        if (!(s.cnt < ms)) dev_error(n, 2u);
+        s.refr = s.refr_steps;
+        if (NK == NK_LIF) {
+            s.Vm = s.Vreset;
+        } else {
+            s.Vm = (float)((double)s.C * 0.001);
+            s.u = s.u + s.D;
+        }
+        return true;
+    }
+    float x = sp + s.I0;
+    float noise = bg_noise_normal(n.seed, (uint32_t)i, t);
+    x = x + noise * s.Inoise;
+    if (NK == NK_LIF) {
+        s.Vm = s.C1 * s.Vm + s.C2 * x;
+    } else {
+        float Vint = s.Vm * 1000.0f;
+        double q = 0.04 * (double)Vint * (double)Vint + (double)(5.0f * Vint) + 140.0 - (double)s.u;
+        float Vb = (float)((double)Vint + (double)s.C3 * q);
+        s.u = s.u + s.C3 * s.A * (s.B * Vint - s.u);
+        s.Vm = (float)((double)Vb * 0.001 + (double)(s.C2 * x));
+    }
+    return false;
+}
+
+template <int NK>
+__device__ __forceinline__ void neuron_load(const DevNet &n, int i, NeuronRegs &s)
+{
+    s.Vm = n.Vm[i]; s.Vth = n.Vth[i]; s.Vreset = n.Vreset[i]; s.I0 = n.I0[i]; s.Inoise = n.Inoise[i];
+    s.C1 = n.C1[i]; s.C2 = n.C2[i]; s.refr = n.refr[i]; s.refr_steps = n.refr_steps[i];
+    s.cnt = n.cnt[i]; s.off = n.off[i];
+    if (NK == NK_IZH) { s.u = n.u[i]; s.A = n.A[i]; s.B = n.B[i]; s.C = n.C[i]; s.D = n.D[i]; s.C3 = n.C3[i]; }
+}
+
+template <int NK>
+__device__ __forceinline__ void neuron_store(const DevNet &n, int i, const NeuronRegs &s)
+{
+    n.Vm[i] = s.Vm; n.refr[i] = s.refr; n.cnt[i] = s.cnt;
+    if (NK == NK_IZH) n.u[i] = s.u;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Dynamic-synapse PSR update on a delivered spike: AllDSSynapses::changePSR
+// (Synapses/AllDSSynapses.cpp:227-247); plain spiking: AllSpikingSynapses::changePSR (:327-334).
+template <int SK>
+__device__ __forceinline__ float change_psr(const DevNet &n, uint64_t slot, uint64_t t, float psr, float decay)
+{
+    if (SK == SK_DS || SK == SK_DYNSTDP) {
+        float r = n.r[slot], u = n.us[slot];
+        uint64_t last = n.last[slot];
+        if (last != BG_NO_SPIKE) {
+            float isi = (float)(t - last) * n.dt;
+            r = 1.0f + (r * (1.0f - u) - 1.0f) * bg_expf(-isi / n.Dd[slot]);
+            float U = n.U[slot];
+            u = U + u * (1.0f - U) * bg_expf(-isi / n.F[slot]);
+            n.r[slot] = r;
+            n.us[slot] = u;
+        }
+        psr += ((n.W[slot] / decay) * u) * r;
+        n.last[slot] = t;
+        return psr;
+    } else {
+        return psr + (n.W[slot] / decay);
+    }
+}
+
+// getSpikeHistory (Neurons/AllSpikingNeurons.cpp:182-188)
+__device__ __forceinline__ uint64_t spike_hist(const DevNet &n, int idx, int offIndex)
+{
+    int ms = n.max_spikes;
+    int k = (n.cnt[idx] + n.off[idx] + ms + offIndex) % ms;
+    return n.hist[(uint64_t)idx * ms + k];
+}
+
+// AllSTDPSynapses::stdpLearning (Synapses/AllSTDPSynapses.cpp:432-472)
+__device__ __forceinline__ void stdp_learning(const DevNet &n, uint64_t slot, double delta, double epost, double epre)
+{
+    float STDPgap = n.gap[slot], muneg = n.muneg[slot], mupos = n.mupos[slot], tauneg = n.tauneg[slot];
+    float taupos = n.taupos[slot], Aneg = n.Aneg[slot], Apos = n.Apos[slot], Wex = n.Wex[slot];
+    float W = n.W[slot];
+    float dw;
+    if (delta < -(double)STDPgap) {
+        dw = (float)((double)(bg_powf(W, muneg) * Aneg) * bg_exp(delta / (double)tauneg));
+    } else if (delta > (double)STDPgap) {
+        dw = (float)((double)(bg_powf(Wex - W, mupos) * Apos) * bg_exp(-delta / (double)taupos));
+    } else {
+        return;
+    }
+    W = (float)((double)W + epost * epre * (double)dw);
+    if ((Wex < 0 && W > 0) || (Wex > 0 && W < 0)) W = 0;
+    if (fabsf(W) > fabsf(Wex)) W = Wex;
+    n.W[slot] = W;
+}
+
+// The learning part of AllSTDPSynapses::advanceSynapse (Synapses/AllSTDPSynapses.cpp:289-407).
+// The pre-spike walk over the post neuron's history computes (spikeHistory - (int64_t)step) in
+// unsigned 64-bit arithmetic, exactly as the reference does; both walks are bounded by the ring
+// length max_spikes, beyond which the reference would index outside the history ring.
+template <int SK>
+__device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, uint64_t t, bool fPre, bool fPost,
+                                         int idxPre, int idxPost, int total_delay, float psr, float decay)
+{
+    const float dt = n.dt;
+    float tauspre = n.tauspre[slot], tauspost = n.tauspost[slot], taupos = n.taupos[slot], tauneg = n.tauneg[slot];
+    bool useFD = n.useFD[slot] != 0;
+    const int walk = n.max_spikes;
+    uint64_t sh, sh2;
+    float delta, epre, epost;
+    if (fPre) {
+        sh = spike_hist(n, idxPre, -2);
+        if (sh != BG_NO_SPIKE && useFD) {
+            delta = (float)(uint64_t)(t - sh) * dt;
+            epre = (float)(1.0 - (double)bg_expf(-delta / tauspre));
+        } else {
+            epre = 1.0f;
+        }
+        for (int o = -1; o >= -walk; --o) {
+            sh = spike_hist(n, idxPost, o);
+            if (sh == BG_NO_SPIKE) break;
+            delta = (float)(uint64_t)(sh - t) * dt;
+            if ((double)delta <= -3.0 * (double)tauneg) break;
+            if (useFD) {
+                sh2 = spike_hist(n, idxPost, o - 1);
+                if (sh2 == BG_NO_SPIKE) break;
+                epost = (float)(1.0 - (double)bg_expf(-((float)(uint64_t)(sh - sh2) * dt) / tauspost));
+            } else {
+                epost = 1.0f;
+            }
+            stdp_learning(n, slot, (double)delta, (double)epost, (double)epre);
+        }
+        psr = change_psr<SK == SK_DYNSTDP ? SK_DS : SK_SPIKING>(n, slot, t, psr, decay);
+    }
+    if (fPost) {
+        sh = spike_hist(n, idxPost, -2);
+        if (sh != BG_NO_SPIKE && useFD) {
+            delta = (float)(uint64_t)(t - sh) * dt;
+            epost = (float)(1.0 - (double)bg_expf(-delta / tauspost));
+        } else {
+            epost = 1.0f;
+        }
+        for (int o = -1; o >= -walk; --o) {
+            sh = spike_hist(n, idxPre, o);
+            if (sh == BG_NO_SPIKE) break;
+            delta = (float)(uint64_t)(t - sh - (uint64_t)(int64_t)total_delay) * dt;
+            if (delta <= 0 || (double)delta >= 3.0 * (double)taupos) break;
+            if (useFD) {
+                sh2 = spike_hist(n, idxPre, o - 1);
+                if (sh2 == BG_NO_SPIKE) break;
+                epre = (float)(1.0 - (double)bg_expf(-((float)(uint64_t)(sh - sh2) * dt) / tauspre));
+            } else {
+                epre = 1.0f;
+            }
+            stdp_learning(n, slot, (double)delta, (double)epost, (double)epre);
+        }
+    }
+    return psr;
+}
+
+// Wavefront sum in the engine's canonical order: lane l holds the lane-strided sequential partial
+// sum; butterfly over 32, 16, ..., 1 leaves the same value in every lane (for lane 0 this is the
+// tree acc[l] += acc[l + off] of the oracle's wave64 mode).
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ConnGrowth::updateFrontiers + updateOverlap for one (i = src, j = dest) pair
+// (Connections/ConnGrowth.cpp:319-376, Layouts/Layout.cpp:54-69).
+__device__ __forceinline__ float overlap_area(float xi, float yi, float r1, float xj, float yj, float r2, bool same)
+{
+    if (same) return 0.0f;
+    float dx = xi - xj, dy = yi - yj;
+    float lenAB2 = dx * dx + dy * dy;
+    float lenAB = sqrtf(lenAB2);
+    float delta = lenAB - (r1 + r2);
+    if (!(delta < 0)) return 0.0f;
+    float mn = r1 < r2 ? r1 : r2, mx = r1 < r2 ? r2 : r1;
+    if (lenAB + mn <= mx) {
+        const float pi = 3.1415926536f;
+        return pi * mn * mn;
+    }
+    float r12 = r1 * r1, r22 = r2 * r2;
+    float cosCBA = (float)((double)(r22 + lenAB2 - r12) / (2.0 * (double)r2 * (double)lenAB));
+    float angCBA = bg_acosf(cosCBA);
+    float angCBD = (float)(2.0 * (double)angCBA);
+    float cosCAB = (float)((double)(r12 + lenAB2 - r22) / (2.0 * (double)r1 * (double)lenAB));
+    float angCAB = bg_acosf(cosCAB);
+    float angCAD = (float)(2.0 * (double)angCAB);
+    return (float)(0.5 * (double)(r22 * (angCBD - bg_sinf(angCBD)) + r12 * (angCAD - bg_sinf(angCAD))));
+}

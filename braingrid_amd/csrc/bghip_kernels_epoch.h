@@ -1,0 +1,341 @@
+// Non-templated kernels: delay-ring flush, growth, history update and index-map construction.
+// Included by bghip.hip only.
+#pragma once
+#include "bghip_kernels.h"
+
+// Inserts the spikes of the last fused block into the delay rings so that between API calls the
+// rings hold every pending spike, as the reference's rings do after each step.
+__global__ __launch_bounds__(BG_BLOCK) void k_flush_block(DevNet n, uint64_t tprev0, int sel)
+{
+    uint64_t slot = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (slot >= n.S) return;
+    const uint32_t meta = n.meta[slot];
+    if (!(meta & BG_META_IN_USE)) return;
+    uint32_t fm = n.mask[sel][meta & BG_META_SRC_MASK];
+    if (!fm) return;
+    const uint32_t delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
+    const uint32_t Rm = (uint32_t)n.R - 1;
+    while (fm) {
+        const int b0 = __ffs(fm) - 1;
+        fm &= fm - 1;
+        const uint32_t b = (uint32_t)((tprev0 + (uint64_t)b0 + delay) & Rm);
+        uint32_t *w = &n.ring[(uint64_t)(b >> 5) * n.S + slot];
+        const uint32_t m = 1u << (b & 31);
+        if (*w & m) dev_error(n, 1u);
+        *w |= m;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Growth. ConnGrowth::updateConns (Connections/ConnGrowth.cpp:295-311): rates, outgrowth, radii.
+__global__ void k_growth_radii(DevNet n, float epochDuration, float epsilon, float maxRate, float beta, float c)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n.N) return;
+    if (!(n.cnt[i] < n.max_spikes)) dev_error(n, 2u);
+    float rate = (float)n.cnt[i] / epochDuration;
+    n.rates[i] = rate;
+    float x = epsilon - rate / maxRate;
+    x = x / beta;
+    x = bg_expf(x);
+    x = x + 1.0f;
+    x = 2.0f / x;
+    float outgrowth = 1.0f - x;
+    n.radii[i] = n.radii[i] + outgrowth * c;
+}
+
+struct CreateCtx {
+    float W_adj;            // SYNAPSE_STRENGTH_ADJUSTMENT (AllSynapses.h:190)
+    int32_t total_delay[4];
+    float U[4], D[4], F[4];
+    float u0, r0;
+    float Apos, Aneg, gap, tauspost, tauspre, taupos, tauneg, Wex, mupos, muneg;
+    int32_t useFD;
+    int syn_kind;
+    uint64_t step;          // creation step (delay-ring phase)
+};
+
+__device__ __forceinline__ int syn_type_of(int ts, int td)
+{   // Layouts/Layout.cpp:129-141 (INH = 1, EXC = 2)
+    if (ts == 1) return td == 1 ? 0 : 1;
+    return td == 1 ? 2 : 3;
+}
+__device__ __forceinline__ int syn_sign_of(int type) { return type <= 1 ? -1 : 1; }
+
+// createSynapse (Synapses/AllSpikingSynapses.cpp:191-233 and subclasses) for slot `slot`.
+__device__ void create_synapse(const DevNet &n, const CreateCtx &c, uint64_t slot, int src, int type, float w)
+{
+    n.meta[slot] = (uint32_t)src | ((uint32_t)type << BG_META_TYPE_SHIFT) |
+                   ((uint32_t)c.total_delay[type] << BG_META_DELAY_SHIFT) | BG_META_IN_USE;
+    n.phase[slot] = (uint8_t)(c.step & (uint64_t)(n.R - 1));
+    n.psr[slot] = 0.0f;
+    n.W[slot] = w;
+    for (int q = 0; q < n.W32; q++) n.ring[(uint64_t)q * n.S + slot] = 0u;
+    if (c.syn_kind == SK_DS || c.syn_kind == SK_DYNSTDP) {
+        n.r[slot] = c.r0;
+        n.us[slot] = c.u0;
+        n.last[slot] = BG_NO_SPIKE;
+        n.U[slot] = c.U[type];
+        n.Dd[slot] = c.D[type];
+        n.F[slot] = c.F[type];
+    }
+    if (c.syn_kind == SK_STDP || c.syn_kind == SK_DYNSTDP) {
+        n.Apos[slot] = c.Apos; n.Aneg[slot] = c.Aneg; n.gap[slot] = c.gap; n.tauspost[slot] = c.tauspost;
+        n.tauspre[slot] = c.tauspre; n.taupos[slot] = c.taupos; n.tauneg[slot] = c.tauneg; n.Wex[slot] = c.Wex;
+        n.mupos[slot] = c.mupos; n.muneg[slot] = c.muneg; n.useFD[slot] = (uint8_t)c.useFD;
+    }
+}
+
+#define BG_GROWTH_TILE 1024
+#define BG_CONN_WORDS (BG_MAX_NEURONS / 32)
+// ConnGrowth::updateSynapsesWeights (Connections/ConnGrowth.cpp:388-463), one workgroup per
+// destination neuron (grid-strided). Existing synapses get W = area * sign * 1e-8 or are erased
+// when area < 0; every unconnected source with area > 0 gets a synapse in dest's first free slot,
+// in ascending source order. Source coordinates and radii stream through LDS in tiles; the
+// connected-source set is an LDS bitmap; create decisions are compacted to slots with a
+// wavefront-ballot + workgroup prefix scan.
+__device__ void growth_serial(const DevNet &n, const CreateCtx &c, int d, const uint32_t *conn, uint32_t *stats)
+{
+    // Replays the reference's per-source order one source at a time. Used for rows that contain
+    // holes or need erasures (the reference's growth rule itself never produces area < 0).
+    const int N = n.N, M = n.M;
+    const uint64_t base = (uint64_t)d * M;
+    const float xd = n.xloc[d], yd = n.yloc[d], rd = n.radii[d];
+    const int td = n.ntype[d];
+    uint32_t count = n.syn_count[d], hi = n.slot_hi[d], created = 0, erased = 0;
+    for (int s = 0; s < N; s++) {
+        float area = overlap_area(n.xloc[s], n.yloc[s], n.radii[s], xd, yd, rd, s == d);
+        int type = syn_type_of(n.ntype[s], td);
+        bool connected = (conn[s >> 5] >> (s & 31)) & 1u;
+        if (connected) {
+            if (area < 0) {
+                for (uint32_t k = 0; k < hi; k++) {
+                    uint32_t meta = n.meta[base + k];
+                    if ((meta & BG_META_IN_USE) && (int)(meta & BG_META_SRC_MASK) == s) {
+                        n.meta[base + k] = 0u;
+                        count--;
+                        erased++;
+                    }
+                }
+            }
+        } else if (area > 0) {
+            if (count >= (uint32_t)M) { dev_error(n, 4u); continue; }
+            uint32_t k = 0;
+            while (n.meta[base + k] & BG_META_IN_USE) k++;
+            create_synapse(n, c, base + k, s, type, area * (float)syn_sign_of(type) * c.W_adj);
+            if (k + 1 > hi) hi = k + 1;
+            count++;
+            created++;
+        }
+    }
+    while (hi > 0 && !(n.meta[base + hi - 1] & BG_META_IN_USE)) hi--;
+    n.syn_count[d] = count;
+    n.slot_hi[d] = hi;
+    if (!(count < (uint32_t)M)) dev_error(n, 4u);
+    atomicAdd(&stats[0], created);
+    atomicAdd(&stats[1], erased);
+}
+
+__global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c, uint32_t *stats)
+{
+    __shared__ float sx[BG_GROWTH_TILE], sy[BG_GROWTH_TILE], sr[BG_GROWTH_TILE];
+    __shared__ uint32_t conn[BG_CONN_WORDS];
+    __shared__ uint32_t s_wave[BG_BLOCK / BG_WAVE];
+    __shared__ uint32_t s_base, s_erase;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int N = n.N, M = n.M;
+    const int nwords = (N + 31) >> 5;
+    for (int d = blockIdx.x; d < N; d += gridDim.x) {
+        const float xd = n.xloc[d], yd = n.yloc[d], rd = n.radii[d];
+        const int td = n.ntype[d];
+        const uint64_t base = (uint64_t)d * M;
+        const uint32_t cnt0 = n.syn_count[d], hi0 = n.slot_hi[d];
+        __syncthreads();
+        for (int w = tid; w < nwords; w += BG_BLOCK) conn[w] = 0u;
+        if (tid == 0) { s_erase = 0; s_base = 0; }
+        __syncthreads();
+        // existing synapses: adjust W or flag an erase
+        for (uint32_t k = tid; k < hi0; k += BG_BLOCK) {
+            uint32_t meta = n.meta[base + k];
+            if (!(meta & BG_META_IN_USE)) continue;
+            int s = (int)(meta & BG_META_SRC_MASK);
+            atomicOr(&conn[s >> 5], 1u << (s & 31));
+            float area = overlap_area(n.xloc[s], n.yloc[s], n.radii[s], xd, yd, rd, s == d);
+            int type = syn_type_of(n.ntype[s], td);
+            if (area < 0) atomicAdd(&s_erase, 1u);
+            else n.W[base + k] = area * (float)syn_sign_of(type) * c.W_adj;
+        }
+        __syncthreads();
+        if (s_erase || cnt0 != hi0) {
+            if (tid == 0) growth_serial(n, c, d, conn, stats);
+            continue;
+        }
+        // rows without holes: the j-th creation (ascending source) takes slot hi0 + j
+        for (int s0 = 0; s0 < N; s0 += BG_GROWTH_TILE) {
+            const int len = min(BG_GROWTH_TILE, N - s0);
+            __syncthreads();
+            for (int q = tid; q < len; q += BG_BLOCK) {
+                sx[q] = n.xloc[s0 + q];
+                sy[q] = n.yloc[s0 + q];
+                sr[q] = n.radii[s0 + q];
+            }
+            __syncthreads();
+            for (int q0 = 0; q0 < len; q0 += BG_BLOCK) {
+                const int q = q0 + tid;
+                const int s = s0 + q;
+                bool create = false;
+                float area = 0.0f;
+                if (q < len && !((conn[s >> 5] >> (s & 31)) & 1u)) {
+                    area = overlap_area(sx[q], sy[q], sr[q], xd, yd, rd, s == d);
+                    create = area > 0;
+                }
+                const uint64_t ball = __ballot(create);
+                const uint32_t lane_rank = (uint32_t)__popcll(ball & ((1ull << lane) - 1ull));
+                if (lane == 0) s_wave[wv] = (uint32_t)__popcll(ball);
+                __syncthreads();
+                uint32_t before = s_base, total = 0;
+                for (int w = 0; w < BG_BLOCK / BG_WAVE; w++) {
+                    if (w < wv) before += s_wave[w];
+                    total += s_wave[w];
+                }
+                __syncthreads();
+                if (create) {
+                    const uint32_t rank = before + lane_rank;
+                    if (cnt0 + rank >= (uint32_t)M) {
+                        dev_error(n, 4u);
+                    } else {
+                        const int type = syn_type_of(n.ntype[s], td);
+                        create_synapse(n, c, base + hi0 + rank, s, type, area * (float)syn_sign_of(type) * c.W_adj);
+                    }
+                }
+                if (tid == 0) s_base += total;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t created = s_base;
+            uint32_t count = cnt0 + created;
+            if (count > (uint32_t)M) count = M;
+            n.syn_count[d] = count;
+            n.slot_hi[d] = count;
+            if (!(count < (uint32_t)M)) dev_error(n, 4u);
+            atomicAdd(&stats[0], created);
+        }
+    }
+}
+
+// Recorder clamp + clearSpikeCounts (Recorders/XmlGrowthRecorder.cpp:89-90,
+// Neurons/AllSpikingNeurons.cpp:79-87).
+__global__ void k_update_history(DevNet n, float minRadius, int clamp)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n.N) return;
+    if (clamp && n.radii[i] < minRadius) n.radii[i] = minRadius;
+    n.off[i] = (n.cnt[i] + n.off[i]) % n.max_spikes;
+    n.cnt[i] = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Index maps (AllSynapses::createSynapseImap, Synapses/AllSynapses.cpp:223-293).
+// Exclusive scan of syn_count into in_begin, single workgroup (N <= 262144); total into *total.
+__global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t *counts, uint32_t *begin, int N, uint64_t *total)
+{
+    __shared__ uint32_t s_part[1024];
+    __shared__ uint32_t s_carry;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < N; c0 += 1024) {
+        const int i = c0 + tid;
+        const uint32_t v = i < N ? counts[i] : 0u;
+        s_part[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint32_t x = tid >= off ? s_part[tid - off] : 0u;
+            __syncthreads();
+            s_part[tid] += x;
+            __syncthreads();
+        }
+        if (i < N) begin[i] = s_carry + s_part[tid] - v;
+        __syncthreads();
+        if (tid == 1023) s_carry += s_part[1023];
+        __syncthreads();
+    }
+    if (tid == 0) *total = s_carry;
+}
+
+// incoming map: in-use slots of each destination in ascending slot order (wave per dest).
+__global__ __launch_bounds__(BG_BLOCK) void k_fill_incoming(DevNet n)
+{
+    const int lane = threadIdx.x & 63;
+    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
+    if (d >= n.N) return;
+    const uint32_t hi = n.slot_hi[d];
+    uint32_t out = n.in_begin[d];
+    for (uint32_t k0 = 0; k0 < hi; k0 += BG_WAVE) {
+        const uint32_t k = k0 + lane;
+        const uint64_t slot = (uint64_t)d * n.M + k;
+        const bool use = k < hi && (n.meta[slot] & BG_META_IN_USE);
+        const uint64_t ball = __ballot(use);
+        if (use) n.in_map[out + __popcll(ball & ((1ull << lane) - 1ull))] = (uint32_t)slot;
+        out += (uint32_t)__popcll(ball);
+    }
+}
+
+__global__ void k_count_outgoing(DevNet n, const uint32_t *in_map, uint64_t total, uint32_t *out_count)
+{
+    uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    atomicAdd(&out_count[n.meta[in_map[q]] & BG_META_SRC_MASK], 1u);
+}
+
+__global__ void k_place_outgoing(DevNet n, const uint32_t *in_map, uint64_t total, const uint32_t *out_begin,
+                                 uint32_t *cursor, uint32_t *out_map)
+{
+    uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    const uint32_t slot = in_map[q];
+    const uint32_t s = n.meta[slot] & BG_META_SRC_MASK;
+    out_map[out_begin[s] + atomicAdd(&cursor[s], 1u)] = slot;
+}
+
+// Sorts each source's outgoing segment ascending (the reference's map is stable in iSyn order).
+// Workgroup per source: bitonic sort in LDS for segments up to 4096, serial insertion sort above.
+#define BG_SORT_LDS 4096
+__global__ __launch_bounds__(BG_BLOCK) void k_sort_outgoing(const uint32_t *out_begin, const uint32_t *out_count,
+                                                            uint32_t *out_map)
+{
+    __shared__ uint32_t v[BG_SORT_LDS];
+    const int s = blockIdx.x;
+    const uint32_t b = out_begin[s], c = out_count[s];
+    if (c < 2) return;
+    if (c > BG_SORT_LDS) {
+        if (threadIdx.x == 0) {
+            for (uint32_t i = 1; i < c; i++) {
+                uint32_t x = out_map[b + i];
+                uint32_t j = i;
+                while (j > 0 && out_map[b + j - 1] > x) { out_map[b + j] = out_map[b + j - 1]; j--; }
+                out_map[b + j] = x;
+            }
+        }
+        return;
+    }
+    uint32_t P = 1;
+    while (P < c) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) v[i] = i < c ? out_map[b + i] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+                uint32_t l = i ^ j;
+                if (l > i) {
+                    bool up = (i & k) == 0;
+                    uint32_t a = v[i], bb = v[l];
+                    if ((a > bb) == up) { v[i] = bb; v[l] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) out_map[b + i] = v[i];
+}

@@ -1,0 +1,12 @@
+// Host launchers of the templated kernels; each family is compiled in its own translation unit so
+// the library builds in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "bghip_device.h"
+
+void launch_fused_lif(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_t t0, int K, int pull,
+                      uint64_t tprev0, int sel);
+void launch_fused_izh(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_t t0, int K, int pull,
+                      uint64_t tprev0, int sel);
+void launch_neuron_step(const DevNet &dn, hipStream_t s, int nk, uint64_t t);
+void launch_synapse_step(const DevNet &dn, hipStream_t s, int sk, uint64_t t);

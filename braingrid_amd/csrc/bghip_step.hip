@@ -1,0 +1,37 @@
+// Per-step neuron and synapse kernels (see k_neuron_step / k_synapse_step in bghip_kernels.h).
+#include "bghip_kernels.h"
+#include "bghip_launch.h"
+
+void launch_neuron_step(const DevNet &dn, hipStream_t s, int nk, uint64_t t)
+{
+    unsigned nb = (unsigned)((dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    if (nk == NK_LIF) hipLaunchKernelGGL(k_neuron_step<NK_LIF>, dim3(nb), dim3(BG_BLOCK), 0, s, dn, t);
+    else hipLaunchKernelGGL(k_neuron_step<NK_IZH>, dim3(nb), dim3(BG_BLOCK), 0, s, dn, t);
+}
+
+template <int SK, int W32>
+static void go(const DevNet &dn, hipStream_t s, uint64_t t)
+{
+    unsigned blocks = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
+    hipLaunchKernelGGL((k_synapse_step<SK, W32>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t);
+}
+template <int SK>
+static void go_w(const DevNet &dn, hipStream_t s, uint64_t t)
+{
+    switch (dn.W32) {
+    case 1: go<SK, 1>(dn, s, t); break;
+    case 2: go<SK, 2>(dn, s, t); break;
+    case 4: go<SK, 4>(dn, s, t); break;
+    default: go<SK, 8>(dn, s, t); break;
+    }
+}
+
+void launch_synapse_step(const DevNet &dn, hipStream_t s, int sk, uint64_t t)
+{
+    switch (sk) {
+    case SK_SPIKING: go_w<SK_SPIKING>(dn, s, t); break;
+    case SK_DS: go_w<SK_DS>(dn, s, t); break;
+    case SK_STDP: go_w<SK_STDP>(dn, s, t); break;
+    default: go_w<SK_DYNSTDP>(dn, s, t); break;
+    }
+}

@@ -1,0 +1,187 @@
+/*
+ * bghip.h -- C ABI of the MI355X spiking-network engine (libbghip.so).
+ *
+ * The library replaces BrainGrid's GPUSpikingModel (Core/GPUSpikingModel.{h,cu}) behind the
+ * reference's model interfaces. A host-side model (the C++ HipSpikingModel in
+ * braingrid_amd/host/, or any other caller) keeps BrainGrid's host SoA arrays, uploads them once,
+ * advances on the GPU, runs neurite growth on the GPU and downloads state for recorders.
+ *
+ * Conventions
+ *   - every function returns BGHIP_OK (0) or a negative bghip_status;
+ *   - handles are opaque; the library owns all device memory, the caller owns host buffers and
+ *     only lends them for the duration of a call;
+ *   - a bghip_net is externally synchronised (one host thread at a time); distinct nets may be
+ *     driven from distinct threads; every net has its own HIP stream;
+ *   - advance / update_connections are stream-ordered and return before the GPU finishes; device
+ *     errors (delay-ring collision, spike-history overflow, synapse-slot overflow) are reported by
+ *     the next bghip_sync or download as BGHIP_ESTATE;
+ *   - arrays use the reference's layouts: neuron arrays of length N, synapse "slot" arrays of
+ *     length N * max_synapses_per_neuron in destination-major order (iSyn = M * dest + k,
+ *     Synapses/AllSynapses.cpp:366-371), spike history N * max_spikes
+ *     (Neurons/AllSpikingNeurons.h:185-204), index maps as in Core/SynapseIndexMap.h:38-104.
+ *   - noise is the deterministic Philox/Box-Muller stream of include/bg_detmath.h.
+ */
+#ifndef BGHIP_H
+#define BGHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum bghip_status {
+    BGHIP_OK = 0,
+    BGHIP_EINVAL = -1,       /* bad argument or size */
+    BGHIP_ENOMEM = -2,       /* device allocation failed */
+    BGHIP_EDEVICE = -3,      /* HIP runtime error */
+    BGHIP_ESTATE = -4,       /* model state error raised on the device (see bghip_net_error_bits) */
+    BGHIP_EUNSUPPORTED = -5  /* configuration outside what the kernels implement */
+} bghip_status;
+
+/* device error bits (bghip_net_error_bits) */
+#define BGHIP_ERR_RING_COLLISION 1u   /* assert(!(delay_queue & (0x1 << idx))), Synapses/AllSpikingSynapses.cpp:277 */
+#define BGHIP_ERR_SPIKE_OVERFLOW 2u   /* assert(spikeCount < max_spikes), Neurons/AllSpikingNeurons.cpp:112 */
+#define BGHIP_ERR_SLOT_OVERFLOW 4u    /* no free slot for a new synapse, Synapses/AllSynapses.cpp:359-362 */
+
+enum { BGHIP_NEURON_LIF = 0, BGHIP_NEURON_IZH = 1 };
+enum { BGHIP_SYN_SPIKING = 0, BGHIP_SYN_DS = 1, BGHIP_SYN_STDP = 2, BGHIP_SYN_DYNSTDP = 3 };
+
+typedef struct bghip_ctx bghip_ctx;
+typedef struct bghip_net bghip_net;
+
+/* Per-synapse-type creation table, index II=0, IE=1, EI=2, EE=3 (Global.h:100). The host fills it
+ * from createSynapse (Synapses/AllSpikingSynapses.cpp:191-233, AllDSSynapses.cpp:180-218) so that
+ * synapses the GPU creates during growth carry exactly the values the reference would give them.
+ * decay must be the host's exp(-dt/tau) (AllSpikingSynapses.cpp:344-354). */
+typedef struct bghip_synapse_types {
+    float tau[4];
+    float decay[4];
+    int32_t total_delay[4];
+    float U[4], D[4], F[4];
+    float u0, r0;                 /* DEFAULT_U, 1.0 (AllDSSynapses.cpp:160-167) */
+    /* STDP creation values (AllSTDPSynapses.cpp:252-273) */
+    float Apos, Aneg, STDPgap, tauspost, tauspre, taupos, tauneg, Wex, mupos, muneg;
+    int32_t useFroemkeDanSTDP;
+} bghip_synapse_types;
+
+typedef struct bghip_net_desc {
+    int32_t num_neurons;              /* N */
+    int32_t max_synapses_per_neuron;  /* M (SimConfig maxSynapsesPerNeuron) */
+    int32_t neuron_model;             /* BGHIP_NEURON_* */
+    int32_t synapse_model;            /* BGHIP_SYN_* */
+    int32_t ring_bits;                /* delay ring length: 32 (reference) .. 256, multiple of 32 */
+    int32_t max_spikes;               /* int(epochDuration * maxFiringRate) */
+    float dt;                         /* deltaT */
+    uint64_t noise_seed;              /* Philox key of the noise stream */
+    int32_t block_steps;              /* steps per fused launch: 0 = automatic, 1 = per-step kernels */
+    bghip_synapse_types types;
+} bghip_net_desc;
+
+/* Host views of AllIFNeurons / AllIZHNeurons / AllSpikingNeurons arrays (length N unless noted).
+ * Izhikevich fields may be NULL for LIF networks. */
+typedef struct bghip_neurons {
+    float *Vm, *Vthresh, *Vreset, *I0, *Inoise, *C1, *C2, *Trefract, *summation_map;
+    int32_t *nStepsInRefr;
+    float *Aconst, *Bconst, *Cconst, *Dconst, *u, *C3;
+    int32_t *spikeCount, *spikeCountOffset;
+    uint64_t *spike_history;          /* N * max_spikes */
+} bghip_neurons;
+
+/* Host views of the synapse slot arrays (length N*M unless noted). delayQueue holds ring_bits/32
+ * words per slot, slot-major; with ring_bits == 32 it is exactly AllSpikingSynapses::delayQueue.
+ * Fields of synapse models not in use may be NULL. */
+typedef struct bghip_synapses {
+    uint8_t *in_use;
+    int32_t *sourceNeuronIndex, *destNeuronIndex, *type;
+    float *W, *psr, *decay;
+    int32_t *total_delay, *delayIdx;
+    uint32_t *delayQueue;
+    float *r, *u, *U, *D, *F;         /* DS / DynamicSTDP */
+    uint64_t *lastSpike;
+    uint32_t *delayQueuePost;         /* STDP */
+    int32_t *delayIdxPost;
+    float *Apos, *Aneg, *STDPgap, *tauspost, *tauspre, *taupos, *tauneg, *Wex, *mupos, *muneg;
+    uint8_t *useFroemkeDanSTDP;
+    uint32_t *synapse_counts;         /* N */
+} bghip_synapses;
+
+/* SynapseIndexMap (Core/SynapseIndexMap.h:38-104); maps have total_synapse_counts entries. */
+typedef struct bghip_index_map {
+    uint32_t *incomingSynapseIndexMap, *incomingSynapseBegin, *incomingSynapseCount;
+    uint32_t *outgoingSynapseIndexMap, *outgoingSynapseBegin, *outgoingSynapseCount;
+} bghip_index_map;
+
+/* ConnGrowth parameters (Connections/ConnGrowth.h m_growth) and layout. */
+typedef struct bghip_growth {
+    float epsilon, beta, rho, targetRate, maxRate, minRadius, startRadius;
+    float epochDuration;
+} bghip_growth;
+
+typedef struct bghip_growth_stats {
+    int64_t total_synapses;   /* after the update */
+    int64_t created;
+    int64_t erased;
+} bghip_growth_stats;
+
+/* ---- context ---------------------------------------------------------------------------- */
+/* Replaces GPUSpikingModel::setupSim's cudaSetDevice (Core/GPUSpikingModel.cu:109). */
+int bghip_ctx_create(int device, bghip_ctx **out);
+void bghip_ctx_destroy(bghip_ctx *ctx);
+const char *bghip_last_error(const bghip_ctx *ctx);
+/* device properties for schedulers: HBM bytes free/total */
+int bghip_ctx_mem_info(bghip_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* ---- network lifetime ------------------------------------------------------------------- */
+/* allocNeuronDeviceStruct + allocSynapseDeviceStruct (Neurons/IAllNeurons.h:89-142,
+ * Synapses/IAllSynapses.h:125-221): one SoA arena per network. */
+int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghip_net **out);
+/* deleteNeuronDeviceStruct + deleteSynapseDeviceStruct */
+void bghip_net_destroy(bghip_net *net);
+/* bytes of device memory a network with this description occupies (admission control) */
+int64_t bghip_net_footprint(const bghip_net_desc *desc);
+
+/* copyNeuronHostToDevice (Neurons/AllIFNeurons_d.cu, AllIZHNeurons_d.cu) */
+int bghip_upload_neurons(bghip_net *net, const bghip_neurons *h);
+/* copySynapseHostToDevice + copySynapseIndexMapHostToDevice (Core/GPUSpikingModel.cu:326-358);
+ * the index maps are rebuilt on the device from the slot arrays. `step` is g_simulationStep at
+ * the time of the upload (fixes the delay-ring phase). */
+int bghip_upload_synapses(bghip_net *net, const bghip_synapses *h, uint64_t step);
+/* Layout coordinates and neuron types (Layouts/Layout.h xloc, yloc, neuron_type_map: INH=1,
+ * EXC=2) plus growth state; needed by bghip_update_connections. */
+int bghip_upload_layout(bghip_net *net, const float *xloc, const float *yloc, const int32_t *neuron_type);
+int bghip_upload_growth(bghip_net *net, const bghip_growth *g, const float *radii, const float *rates);
+
+/* ---- simulation ------------------------------------------------------------------------- */
+/* n_steps calls of GPUSpikingModel::advance (Core/GPUSpikingModel.cu:188-225), starting at the
+ * network's current step; stream-ordered. */
+int bghip_advance(bghip_net *net, uint64_t n_steps);
+/* GPUSpikingModel::updateConnections (Core/GPUSpikingModel.cu:246-259): growth rates, radii,
+ * overlap, weight update, synapse create/erase, index-map rebuild, all on the device.
+ * stats may be NULL; when given the call synchronises. */
+int bghip_update_connections(bghip_net *net, bghip_growth_stats *stats);
+/* The model part of GPUSpikingModel::updateHistory (Core/GPUSpikingModel.cu:266-272): applies the
+ * recorder's minRadius clamp (Recorders/XmlGrowthRecorder.cpp:89-90) and clears spike counts
+ * (Neurons/AllSpikingNeurons.cpp:79-87). Download spikes and rates/radii before calling it. */
+int bghip_update_history(bghip_net *net);
+/* waits for the network's stream; returns BGHIP_ESTATE if a device error bit is set */
+int bghip_sync(bghip_net *net);
+uint32_t bghip_net_error_bits(bghip_net *net);
+uint64_t bghip_net_step(const bghip_net *net);
+
+/* ---- downloads (blocking) --------------------------------------------------------------- */
+int bghip_download_neurons(bghip_net *net, bghip_neurons *h);
+int bghip_download_synapses(bghip_net *net, bghip_synapses *h);
+int64_t bghip_total_synapses(bghip_net *net);
+int bghip_download_index_map(bghip_net *net, bghip_index_map *h);
+int bghip_download_growth(bghip_net *net, float *radii, float *rates);
+
+/* ---- timing (measurement support) -------------------------------------------------------- */
+/* Average duration in milliseconds of the engine's fused step launches recorded with HIP events
+ * on the network's stream since the last reset; count receives the number of launches timed. */
+int bghip_kernel_timing(bghip_net *net, int enable, double *avg_ms, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BGHIP_H */

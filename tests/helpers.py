@@ -1,0 +1,123 @@
+"""Shared test helpers: oracle <-> GPU state transfer and bit-exact comparisons."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from braingrid_amd import bghip as B
+from braingrid_amd import model as MD
+from oracle import oracle_py as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CONFIG_DIR = os.path.join(ROOT, "tests", "golden", "configs")
+
+GROWTH_CONFIGS = ["conn-lif-ds", "conn-lif-ds-dense", "conn-lif-spiking", "conn-lif-stdp", "conn-lif-dynstdp",
+                  "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"]
+
+NEURON_UPLOAD = ("Vm", "Vthresh", "Vreset", "I0", "Inoise", "C1", "C2", "Trefract", "summation_map", "nStepsInRefr",
+                 "spikeCount", "spikeCountOffset", "spike_history")
+IZH_UPLOAD = ("Aconst", "Bconst", "Cconst", "Dconst", "u", "C3")
+SYN_UPLOAD = ("in_use", "sourceNeuronIndex", "type", "W", "psr", "decay", "total_delay", "delayIdx", "delayQueue",
+              "r", "U", "D", "F", "lastSpike", "delayQueuePost", "delayIdxPost", "Apos", "Aneg", "STDPgap",
+              "tauspost", "tauspre", "taupos", "tauneg", "Wex", "mupos", "muneg", "useFroemkeDanSTDP")
+
+
+def load_config(name: str, deterministic: bool = True) -> dict:
+    cfg = O.parse_config_xml(os.path.join(CONFIG_DIR, name + ".xml"), CONFIG_DIR)
+    if "dynamic" in name:
+        cfg.update(layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1)
+    if deterministic:
+        cfg.update(noise_mode=1, math_mode=1, sum_mode=1)
+    return cfg
+
+
+def max_spikes(cfg) -> int:
+    return int(np.float32(cfg["epoch_duration"]) * np.float32(cfg["max_firing_rate"]))
+
+
+def steps_per_epoch(cfg) -> int:
+    return int(np.float32(np.float32(cfg["epoch_duration"]) / np.float32(cfg["dt"])))
+
+
+def gpu_net_from_oracle(ctx: B.Context, ora: O.OracleNet, cfg: dict, block_steps: int = 0) -> B.Net:
+    """Create a device network holding exactly the oracle's current state."""
+    N = cfg["width"] * cfg["height"]
+    M = cfg["max_synapses_per_neuron"]
+    nk = O.NEURON_CLASSES[cfg["neuron"]]
+    sk = O.SYN_CLASSES[cfg["synapse"]]
+    desc = MD.net_desc(N, M, nk, sk, max_spikes(cfg), cfg["dt"], cfg["seed"], cfg.get("ring_bits", 32),
+                       block_steps, cfg.get("stdp"), cfg.get("delay_override"))
+    net = B.Net(ctx, desc)
+    neu = {f: ora.array(f) for f in NEURON_UPLOAD}
+    if nk == O.IZH:
+        neu.update({f: ora.array(f) for f in IZH_UPLOAD})
+    net.upload_neurons(neu)
+    syn = {}
+    for f in SYN_UPLOAD:
+        src = "u_syn" if f == "u" else f
+        try:
+            syn[f] = ora.array(src)
+        except KeyError:
+            pass
+    syn["u"] = ora.array("u_syn")
+    syn["synapse_counts"] = ora.array("synapse_counts")
+    net.upload_synapses(syn, ora.step)
+    net.upload_layout(ora.array("xloc"), ora.array("yloc"), ora.array("neuron_type_map"))
+    g = MD.growth_params(cfg["epsilon"], cfg["beta"], cfg["rho"], cfg["target_rate"], cfg["min_radius"],
+                         cfg["start_radius"], cfg["epoch_duration"])
+    net.upload_growth(g, ora.array("radii"), ora.array("rates"))
+    return net
+
+
+def _eq(a, b):
+    if a.dtype.kind == "f":
+        return np.array_equal(a.view(np.uint32 if a.itemsize == 4 else np.uint64),
+                              b.view(np.uint32 if b.itemsize == 4 else np.uint64))
+    return np.array_equal(a, b)
+
+
+def compare_state(ora: O.OracleNet, net: B.Net, cfg: dict, where: str, check_maps: bool = True) -> list[str]:
+    """Bit-exact comparison of every state array; returns the names that differ."""
+    bad = []
+    ms = max_spikes(cfg)
+    g = net.download_neurons(ms)
+    names = ["Vm", "summation_map", "nStepsInRefr", "spikeCount", "spikeCountOffset", "spike_history"]
+    if O.NEURON_CLASSES[cfg["neuron"]] == O.IZH:
+        names.append("u")
+    for f in names:
+        if not _eq(ora.array(f), g[f]):
+            bad.append(f"{where}:{f}")
+    s = net.download_synapses()
+    use = ora.array("in_use").astype(bool)
+    if not np.array_equal(use, s["in_use"].astype(bool)):
+        bad.append(f"{where}:in_use")
+    if not np.array_equal(ora.array("synapse_counts"), s["synapse_counts"]):
+        bad.append(f"{where}:synapse_counts")
+    W32 = cfg.get("ring_bits", 32) // 32
+    pairs = [("sourceNeuronIndex", "sourceNeuronIndex"), ("type", "type"), ("W", "W"), ("psr", "psr"),
+             ("decay", "decay"), ("total_delay", "total_delay"), ("delayIdx", "delayIdx")]
+    sk = O.SYN_CLASSES[cfg["synapse"]]
+    if sk in (O.DS, O.DYNSTDP):
+        pairs += [("r", "r"), ("u_syn", "u"), ("lastSpike", "lastSpike")]
+    if sk in (O.STDP, O.DYNSTDP):
+        pairs += [("delayQueuePost", "delayQueuePost"), ("delayIdxPost", "delayIdxPost")]
+    for of, gf in pairs:
+        if not _eq(ora.array(of)[use], s[gf][use]):
+            bad.append(f"{where}:{of}")
+    rq_o = ora.array("delayQueue").reshape(-1, W32)[use]
+    rq_g = s["delayQueue"].reshape(-1, W32)[use]
+    if not np.array_equal(rq_o, rq_g):
+        bad.append(f"{where}:delayQueue")
+    if cfg["conn"] == "ConnGrowth":
+        r, q = net.download_growth()
+        if not _eq(ora.array("radii"), r):
+            bad.append(f"{where}:radii")
+        if not _eq(ora.array("rates"), q):
+            bad.append(f"{where}:rates")
+    if check_maps and ora.total_synapses > 0:
+        m = net.download_index_map()
+        for f in m:
+            if not np.array_equal(ora.array(f), m[f]):
+                bad.append(f"{where}:{f}")
+    return bad

@@ -1,0 +1,58 @@
+"""GPU parity: the HIP path against the CPU oracle in deterministic mode, bit for bit.
+
+Every config in tests/golden/configs runs on both sides with the Philox noise stream, the
+project-owned transcendental functions and the wave64 summation order; after each epoch's steps,
+after each growth update and after each history update every state array must be identical:
+membrane state, refractory counters, spike histories and counts, synapse slots (source, type, W,
+psr, delay ring contents and phase, DS r/u/lastSpike, STDP post rings), index maps, rates, radii.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(hip_ctx, name, block_steps=0, epochs=None):
+    cfg = H.load_config(name)
+    ora = O.OracleNet(cfg)
+    net = H.gpu_net_from_oracle(hip_ctx, ora, cfg, block_steps=block_steps)
+    steps = H.steps_per_epoch(cfg)
+    bad = H.compare_state(ora, net, cfg, "setup")
+    n_ep = cfg["num_epochs"] if epochs is None else epochs
+    spikes = 0
+    for ep in range(1, n_ep + 1):
+        ora.advance(steps)
+        net.advance(steps)
+        net.sync()
+        spikes += int(ora.array("spikeCount").sum())
+        bad += H.compare_state(ora, net, cfg, f"ep{ep}-advanced")
+        ora.update_connections()
+        net.update_connections()
+        bad += H.compare_state(ora, net, cfg, f"ep{ep}-grown")
+        ora.update_history()
+        net.update_history()
+        net.sync()
+        bad += H.compare_state(ora, net, cfg, f"ep{ep}-history", check_maps=False)
+        if bad:
+            break
+    net.close()
+    return bad, spikes, ora.total_synapses
+
+
+@pytest.mark.parametrize("name", H.GROWTH_CONFIGS)
+def test_growth_configs_bit_exact(hip_ctx, name):
+    print(f"[parity] {name}", flush=True)
+    bad, spikes, total = _run(hip_ctx, name)
+    assert not bad, bad[:10]
+    assert spikes > 0
+    assert total > 0
+
+
+@pytest.mark.parametrize("block_steps", [1, 2, 5])
+def test_block_size_does_not_change_results(hip_ctx, block_steps):
+    print(f"[parity] conn-lif-ds-dense block_steps={block_steps}", flush=True)
+    bad, spikes, total = _run(hip_ctx, "conn-lif-ds-dense", block_steps=block_steps)
+    assert not bad, bad[:10]
