@@ -1,0 +1,204 @@
+"""Benchmark: synapse-updates/s of the 10k-neuron LIF + DS growth network on one MI355X.
+
+Workload (the benchmark-scale plain method): a 100 x 100 grid of AllLIFNeurons with the
+test-small parameter distributions (10% inhibitory, 10% starters, DynamicLayout fractions,
+seed 1), AllDSSynapses, ConnGrowth (epsilon 0.6, beta 0.1, rho 1e-4, targetRate 1.9), FP32,
+dt = 1e-4 s, epochs of 10 s (1e5 steps), maxSynapsesPerNeuron 1000, deterministic noise.
+As configured the growth rule cannot form synapses within ten epochs, so the network is
+pre-grown: every radius is set to 8.9 and one growth update runs on the GPU, which gives
+8,440,432 DS synapses (mean in-degree 844). The timed region is whole epochs of the hot path:
+`--steps` simulation steps, plus the growth update and history compilation whenever an epoch
+boundary is crossed.
+
+Output: one JSON line. `value` = synapse-updates/s = sum over timed steps of active synapses /
+wall time. `roofline` prices the dominant kernel (the fused block step kernel) with the survey's
+algorithmic bytes (24 B per DS synapse-step + 44 B per LIF neuron-step) over its average launch
+duration, measured with HIP events on the network's stream. `cpu_baseline` times the CPU oracle
+(the reference's single-threaded algorithm restated in C) on a bounded sample of the same network.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+DS_SYN_STEP_BYTES = 24
+LIF_NEURON_STEP_BYTES = 44
+GRID = 100
+R0 = 8.9
+M = 1000
+EPOCH = 10.0
+# At r0 = 8.9 the growth weights (W = overlap area * 1e-8) drive every neuron to its refractory
+# limit (about 310 Hz excitatory, 450 Hz inhibitory with its 2 ms refractory period), so the
+# spike-history budget maxFiringRate must exceed that for the reference's
+# assert(spikeCount < max_spikes) (Neurons/AllSpikingNeurons.cpp:112) to hold.
+MAX_RATE = 1000
+
+
+def workload_cfg():
+    from braingrid_amd import host
+    c = host.base_config()
+    c.update(width=GRID, height=GRID, max_synapses_per_neuron=M, max_firing_rate=MAX_RATE, epoch_duration=EPOCH,
+             num_epochs=10, layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1, seed=1)
+    return c
+
+
+def cpu_baseline(budget_s: float):
+    """Times the oracle (reference algorithm, reference noise mode, one core) on the same network."""
+    from oracle import oracle_py as O
+    cfg = O.default_config()
+    cfg.update(width=GRID, height=GRID, max_synapses_per_neuron=M, max_firing_rate=MAX_RATE, epoch_duration=EPOCH,
+               num_epochs=10, layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1, seed=1)
+    net = O.OracleNet(cfg)
+    net.set_radii(R0)
+    net.update_connections()
+    S = net.total_synapses
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        net.advance(5)
+        steps += 5
+    dt = time.perf_counter() - t0
+    net.close()
+    return {"value": S * steps / dt, "unit": "synapse-updates/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle (reference algorithm, Norm noise, sequential summation, 1 thread), "
+                      f"{steps} steps of the same pre-grown {GRID * GRID}-neuron network ({S} synapses), "
+                      f"{dt:.1f} s"}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "fused_block_10k_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--roofline-steps", type=int, default=20000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy 1 GiB device-to-device and exit (reference for HBM traffic counters)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "copies": 3}))
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs one network per GPU on a one-GPU box; "
+                                   "--gpus above 1 is not supported (see DESIGN.md for how it would shard)"}))
+        sys.exit(2)
+
+    from braingrid_amd import bghip as B
+    from braingrid_amd import host
+
+    ctx = B.Context(0)
+    cfg = workload_cfg()
+    model = host.Model(ctx, cfg)
+    model.set_radii(R0)
+    t_grow0 = time.perf_counter()
+    st = model.update_connections()
+    model.net.sync()
+    t_pregrow = time.perf_counter() - t_grow0
+    N = GRID * GRID
+    S0 = st.total_synapses
+    steps_per_epoch = int(round(EPOCH / 1e-4))
+
+    # warm-up (also settles clocks); the epoch counter runs on
+    model.advance(args.warmup)
+    model.net.sync()
+    done = args.warmup
+
+    # roofline sub-phase: back-to-back fused launches only, timed with HIP events on the net stream
+    rl_steps = min(args.roofline_steps, steps_per_epoch - done % steps_per_epoch - 1)
+    model.net.timing(True)
+    model.advance(rl_steps)
+    avg_ms, launches = model.net.timing(False)
+    done += rl_steps
+    S_rl = model.net.total_synapses
+    K = max(1, round(rl_steps / max(1, launches)))
+
+    # timed region: whole hot path including growth/history at epoch boundaries
+    synapse_updates = 0
+    growth_updates = 0
+    model.net.sync()
+    t0 = time.perf_counter()
+    remaining = args.steps
+    while remaining > 0:
+        to_boundary = steps_per_epoch - (done % steps_per_epoch)
+        n = min(remaining, to_boundary)
+        S_now = model.net.total_synapses
+        model.advance(n)
+        synapse_updates += S_now * n
+        done += n
+        remaining -= n
+        if done % steps_per_epoch == 0:
+            model.update_connections()
+            model.update_history()
+            growth_updates += 1
+    model.net.sync()
+    wall = time.perf_counter() - t0
+    err = model.net.error_bits()
+    neu = model.net.download_neurons(int(EPOCH * MAX_RATE))
+    in_epoch = done % steps_per_epoch or steps_per_epoch
+    rate_hz = float(neu["spikeCount"].mean()) / (in_epoch * 1e-4)
+
+    value = synapse_updates / wall
+    launch_bytes = (DS_SYN_STEP_BYTES * S_rl + LIF_NEURON_STEP_BYTES * N) * (rl_steps / max(1, launches))
+    achieved = launch_bytes / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
+    traffic = load_traffic()
+    out = {
+        "metric": "synapse-updates/s at 10k-LIF growth",
+        "value": value,
+        "unit": "synapse-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded parameter draws, deterministic Philox noise, pre-grown radii 8.9)",
+        "config": {"workload": "10k LIF + DS + ConnGrowth, 100x100 grid pre-grown to r0=8.9, dt=1e-4 s, "
+                               "epoch 1e5 steps", "neurons": N, "synapses": int(S0),
+                   "max_synapses_per_neuron": M, "max_firing_rate": MAX_RATE, "block_steps": K,
+                   "mean_rate_hz_last_epoch": round(rate_hz, 2), "growth_updates_timed": growth_updates,
+                   "pregrow_s": round(t_pregrow, 4)},
+        "neuron_steps_per_s": N * args.steps / wall,
+        "sim_seconds_per_hour": args.steps * 1e-4 / wall * 3600.0,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "kernel": "k_fused_block<LIF,DS,1,16>", "avg_launch_ms": avg_ms, "launches": launches,
+                     "steps_per_launch": rl_steps / max(1, launches),
+                     "algorithmic_bytes_per_launch": launch_bytes,
+                     "traffic_GBps": (traffic["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9) if traffic and avg_ms > 0 else None,
+                     "traffic_source": "profiles/fused_block_10k_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                       "copy-calibrated)" if traffic else None},
+        "device_error_bits": int(err),
+    }
+    model.close()
+    ctx.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libbghip.so")
+LIB_PATH = os.environ.get("BGHIP_LIB") or os.path.join(HERE, "lib", "libbghip.so")
 
 NEURON_LIF, NEURON_IZH = 0, 1
 SYN_SPIKING, SYN_DS, SYN_STDP, SYN_DYNSTDP = 0, 1, 2, 3

@@ -201,7 +201,13 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
     dn.dt = desc->dt;
     dn.seed = desc->noise_seed;
     dn.S = (uint64_t)dn.N * dn.M;
-    for (int t = 0; t < 4; t++) dn.decay[t] = desc->types.decay[t];
+    for (int t = 0; t < 4; t++) {
+        dn.decay[t] = desc->types.decay[t];
+        dn.tU[t] = desc->types.U[t];
+        dn.tD[t] = desc->types.D[t];
+        dn.tF[t] = desc->types.F[t];
+    }
+    dn.ds_typed = 1;
     if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(n->arena);
         delete n;
@@ -314,6 +320,7 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
     std::vector<uint32_t> meta(S, 0u), ring((size_t)W32 * S, 0u), cnt(N, 0u), hi(N, 0u);
     std::vector<uint8_t> phase(S, 0);
     int min_delay = 255;
+    int typed = 1;
     for (int d = 0; d < N; d++) {
         for (int k = 0; k < M; k++) {
             size_t s = (size_t)d * M + k;
@@ -328,6 +335,8 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
             if (stdp && h->delayIdxPost && (h->delayIdxPost[s] & 31) != (di & 31))
                 return set_err(n->ctx, BGHIP_EUNSUPPORTED, "upload_synapses: pre/post ring phases differ");
             min_delay = std::min(min_delay, del);
+            if (ds && (memcmp(&h->U[s], &dn.tU[t], 4) || memcmp(&h->D[s], &dn.tD[t], 4) || memcmp(&h->F[s], &dn.tF[t], 4)))
+                typed = 0;
             meta[s] = (uint32_t)src | ((uint32_t)t << BG_META_TYPE_SHIFT) | ((uint32_t)del << BG_META_DELAY_SHIFT) |
                       BG_META_IN_USE;
             // reference bit b (relative to delayIdx) -> absolute bit (b + phase) mod R,
@@ -347,6 +356,7 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
             return set_err(n->ctx, BGHIP_EINVAL, "upload_synapses: synapse_counts[%d] disagrees with in_use", d);
     }
     n->min_delay_uploaded = cnt.empty() ? 0 : min_delay;
+    dn.ds_typed = typed;
     CHK(h2d(n, dn.meta, meta.data(), S)); CHK(h2d(n, dn.phase, phase.data(), S));
     CHK(h2d(n, dn.ring, ring.data(), (size_t)W32 * S)); CHK(h2d(n, dn.psr, h->psr, S)); CHK(h2d(n, dn.W, h->W, S));
     CHK(h2d(n, dn.syn_count, cnt.data(), N)); CHK(h2d(n, dn.slot_hi, hi.data(), N));

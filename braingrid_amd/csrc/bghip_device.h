@@ -51,6 +51,8 @@ struct DevNet {
     float *xloc, *yloc, *radii, *rates;
     int32_t *ntype;
     float decay[4];
+    int ds_typed;          // 1: every synapse's U, D, F equal its type's creation values
+    float tU[4], tD[4], tF[4];
     uint32_t *err;
 };
 
@@ -67,7 +69,8 @@ struct NeuronRegs {
 };
 
 template <int NK>
-__device__ __forceinline__ bool neuron_advance(const DevNet &n, int i, uint64_t t, float sp, NeuronRegs &s)
+__device__ __forceinline__ bool neuron_advance(const DevNet &n, int i, uint64_t t, float sp, float noise,
+                                               NeuronRegs &s)
 {
     if (s.refr > 0) {
         --s.refr;
@@ -89,7 +92,6 @@ __device__ __forceinline__ bool neuron_advance(const DevNet &n, int i, uint64_t 
         return true;
     }
     float x = sp + s.I0;
-    float noise = bg_noise_normal(n.seed, (uint32_t)i, t);
     x = x + noise * s.Inoise;
     if (NK == NK_LIF) {
         s.Vm = s.C1 * s.Vm + s.C2 * x;
@@ -120,28 +122,38 @@ __device__ __forceinline__ void neuron_store(const DevNet &n, int i, const Neuro
 }
 
 // ---------------------------------------------------------------------------------------------
-// Dynamic-synapse PSR update on a delivered spike: AllDSSynapses::changePSR
-// (Synapses/AllDSSynapses.cpp:227-247); plain spiking: AllSpikingSynapses::changePSR (:327-334).
+// The amount a delivered spike adds to psr, with the synapse-state update that goes with it:
+// AllDSSynapses::changePSR (Synapses/AllDSSynapses.cpp:227-247) for DS / DynamicSTDP,
+// AllSpikingSynapses::changePSR (:327-334) otherwise. The caller adds the result to psr, which
+// is the same single float addition the reference's `psr += ...` performs.
 template <int SK>
-__device__ __forceinline__ float change_psr(const DevNet &n, uint64_t slot, uint64_t t, float psr, float decay)
+__device__ __forceinline__ float psr_increment(const DevNet &n, uint64_t slot, int type, uint64_t t, float decay)
 {
     if (SK == SK_DS || SK == SK_DYNSTDP) {
         float r = n.r[slot], u = n.us[slot];
         uint64_t last = n.last[slot];
         if (last != BG_NO_SPIKE) {
             float isi = (float)(t - last) * n.dt;
-            r = 1.0f + (r * (1.0f - u) - 1.0f) * bg_expf(-isi / n.Dd[slot]);
-            float U = n.U[slot];
-            u = U + u * (1.0f - U) * bg_expf(-isi / n.F[slot]);
+            float D = n.ds_typed ? n.tD[type] : n.Dd[slot];
+            float F = n.ds_typed ? n.tF[type] : n.F[slot];
+            float U = n.ds_typed ? n.tU[type] : n.U[slot];
+            r = 1.0f + (r * (1.0f - u) - 1.0f) * bg_expf(-isi / D);
+            u = U + u * (1.0f - U) * bg_expf(-isi / F);
             n.r[slot] = r;
             n.us[slot] = u;
         }
-        psr += ((n.W[slot] / decay) * u) * r;
         n.last[slot] = t;
-        return psr;
+        return ((n.W[slot] / decay) * u) * r;
     } else {
-        return psr + (n.W[slot] / decay);
+        return n.W[slot] / decay;
     }
+}
+
+template <int SK>
+__device__ __forceinline__ float change_psr(const DevNet &n, uint64_t slot, int type, uint64_t t, float psr,
+                                            float decay)
+{
+    return psr + psr_increment<SK>(n, slot, type, t, decay);
 }
 
 // getSpikeHistory (Neurons/AllSpikingNeurons.cpp:182-188)
@@ -177,7 +189,7 @@ __device__ __forceinline__ void stdp_learning(const DevNet &n, uint64_t slot, do
 // unsigned 64-bit arithmetic, exactly as the reference does; both walks are bounded by the ring
 // length max_spikes, beyond which the reference would index outside the history ring.
 template <int SK>
-__device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, uint64_t t, bool fPre, bool fPost,
+__device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, int type, uint64_t t, bool fPre, bool fPost,
                                          int idxPre, int idxPost, int total_delay, float psr, float decay)
 {
     const float dt = n.dt;
@@ -208,7 +220,7 @@ __device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, uint64_
             }
             stdp_learning(n, slot, (double)delta, (double)epost, (double)epre);
         }
-        psr = change_psr<SK == SK_DYNSTDP ? SK_DS : SK_SPIKING>(n, slot, t, psr, decay);
+        psr = change_psr<SK == SK_DYNSTDP ? SK_DS : SK_SPIKING>(n, slot, type, t, psr, decay);
     }
     if (fPost) {
         sh = spike_hist(n, idxPost, -2);

@@ -18,7 +18,8 @@ __global__ __launch_bounds__(BG_BLOCK) void k_neuron_step(DevNet n, uint64_t t)
     if (i >= n.N) return;
     NeuronRegs s;
     neuron_load<NK>(n, i, s);
-    bool fired = neuron_advance<NK>(n, i, t, n.sp[i], s);
+    const float noise = bg_noise_normal(n.seed, (uint32_t)i, t);
+    bool fired = neuron_advance<NK>(n, i, t, n.sp[i], noise, s);
     neuron_store<NK>(n, i, s);
     n.sp[i] = 0.0f;
     if (fired) n.fired_step[i] = t;
@@ -62,9 +63,9 @@ __global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
         float psr = n.psr[slot];
         const float decay = n.decay[type];
         if (SK == SK_STDP || SK == SK_DYNSTDP) {
-            if (pre || post) psr = stdp_event<SK>(n, slot, t, pre, post, src, d, delay, psr, decay);
+            if (pre || post) psr = stdp_event<SK>(n, slot, type, t, pre, post, src, d, delay, psr, decay);
         } else if (pre) {
-            psr = change_psr<SK>(n, slot, t, psr, decay);
+            psr = change_psr<SK>(n, slot, type, t, psr, decay);
         }
         psr = psr * decay;
         n.psr[slot] = psr;
@@ -81,80 +82,140 @@ __global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
 //   1. pull the spikes its sources fired during the previous block into the delay rings;
 //   2. run every synapse through the K steps in registers (ring test, PSR update, decay),
 //      accumulating the K per-step summation points in the canonical order;
-//   3. lane 0 advances the neuron K steps with those inputs and publishes a K-bit fire mask.
+//   3. advance the neuron K steps with those inputs and publish a K-bit fire mask.
 // Each synapse's meta/ring/psr are read and written once per K steps instead of once per step.
+// A synapse's first delivered spike of the block has its DS update evaluated before the step
+// loop, so the divergent transcendental work runs once per synapse and block rather than once
+// per step; later spikes in the same block (possible only when a source's refractory period is
+// shorter than the block) are handled inside the loop.
+template <int SK, int W32, int KMAX>
+__device__ __forceinline__ void fused_synapse(const DevNet &n, uint64_t slot, uint32_t meta, uint32_t (&rw)[W32],
+                                              float &psr, uint64_t t0, int K, bool pull, uint64_t tprev0,
+                                              const uint32_t *mprev, float (&acc)[KMAX])
+{
+    const uint32_t Rm = (uint32_t)n.R - 1;
+    const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
+    bool changed = false;
+    if (pull) {
+        uint32_t fm = mprev[meta & BG_META_SRC_MASK];
+        const uint32_t delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
+        while (fm) {
+            const int b0 = __ffs(fm) - 1;
+            fm &= fm - 1;
+            const uint32_t b = (uint32_t)((tprev0 + (uint64_t)b0 + delay) & Rm);
+            const uint32_t m = 1u << (b & 31);
+#pragma unroll
+            for (int w = 0; w < W32; w++)
+                if ((int)(b >> 5) == w) {
+                    if (rw[w] & m) dev_error(n, 1u);
+                    rw[w] |= m;
+                }
+            changed = true;
+        }
+    }
+    uint32_t hm = 0;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        if (k < K) {
+            const uint32_t b = (uint32_t)((t0 + (uint64_t)k) & Rm);
+            const uint32_t m = 1u << (b & 31);
+#pragma unroll
+            for (int w = 0; w < W32; w++)
+                if ((int)(b >> 5) == w && (rw[w] & m)) {
+                    rw[w] &= ~m;
+                    hm |= 1u << k;
+                }
+        }
+    }
+    const float decay = n.decay[type];
+    int k1 = -1;
+    float c1 = 0.0f;
+    if (hm) {
+        changed = true;
+        k1 = __ffs(hm) - 1;
+        c1 = psr_increment<SK>(n, slot, type, t0 + (uint64_t)k1, decay);
+        hm &= hm - 1;
+    }
+    float p = psr;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        if (k < K) {
+            p = (k == k1) ? p + c1 : p;
+            if ((hm >> k) & 1u) p = p + psr_increment<SK>(n, slot, type, t0 + (uint64_t)k, decay);
+            p = p * decay;
+            acc[k] += p;
+        }
+    }
+    psr = p;
+    if (changed) {
+#pragma unroll
+        for (int w = 0; w < W32; w++) n.ring[(uint64_t)w * n.S + slot] = rw[w];
+    }
+}
+
 template <int NK, int SK, int W32, int KMAX>
-__global__ __launch_bounds__(BG_BLOCK) void k_fused_block(DevNet n, uint64_t t0, int K, int pull,
+__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_block(DevNet n, uint64_t t0, int K, int pull,
                                                           uint64_t tprev0, int sel_in)
 {
+    __shared__ float s_noise[BG_BLOCK / BG_WAVE][KMAX];
     const int lane = threadIdx.x & (BG_WAVE - 1);
+    const int wv = threadIdx.x / BG_WAVE;
     const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
     if (d >= n.N) return;
     const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
     const bool dense = (cnt == hi);
-    const uint32_t Rm = (uint32_t)n.R - 1;
     const uint32_t *mprev = n.mask[sel_in];
+    const uint64_t row = (uint64_t)d * n.M;
+    const uint32_t *imap = n.in_map + (dense ? 0 : n.in_begin[d]);
+    // noise samples of the K steps, computed one per lane up front and read back by lane 0
+    if (lane < K) s_noise[wv][lane] = bg_noise_normal(n.seed, (uint32_t)d, t0 + (uint64_t)lane);
     float acc[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; k++) acc[k] = 0.0f;
 
-    for (uint32_t q = lane; q < cnt; q += BG_WAVE) {
-        const uint64_t slot = dense ? (uint64_t)d * n.M + q : (uint64_t)n.in_map[n.in_begin[d] + q];
-        const uint32_t meta = n.meta[slot];
-        const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
-        uint32_t rw[W32];
+    // lane-strided ranks (the canonical summation order), with the next rank's meta/ring/psr
+    // loads issued before the current rank is processed
+    uint32_t q = lane;
+    uint64_t slot = 0;
+    uint32_t meta = 0;
+    uint32_t rw[W32];
+    float psr = 0.0f;
+    if (q < cnt) {
+        slot = dense ? row + q : (uint64_t)imap[q];
+        meta = n.meta[slot];
 #pragma unroll
         for (int w = 0; w < W32; w++) rw[w] = n.ring[(uint64_t)w * n.S + slot];
-        bool changed = false;
-        if (pull) {
-            uint32_t fm = mprev[meta & BG_META_SRC_MASK];
-            const uint32_t delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
-            while (fm) {
-                const int b0 = __ffs(fm) - 1;
-                fm &= fm - 1;
-                const uint32_t b = (uint32_t)((tprev0 + (uint64_t)b0 + delay) & Rm);
-                const uint32_t m = 1u << (b & 31);
+        psr = n.psr[slot];
+    }
+    while (q < cnt) {
+        const uint32_t qn = q + BG_WAVE;
+        uint64_t slot_n = slot;
+        uint32_t meta_n = 0;
+        uint32_t rw_n[W32];
+        float psr_n = 0.0f;
+        if (qn < cnt) {
+            slot_n = dense ? row + qn : (uint64_t)imap[qn];
+            meta_n = n.meta[slot_n];
 #pragma unroll
-                for (int w = 0; w < W32; w++)
-                    if ((int)(b >> 5) == w) {
-                        if (rw[w] & m) dev_error(n, 1u);
-                        rw[w] |= m;
-                    }
-                changed = true;
-            }
+            for (int w = 0; w < W32; w++) rw_n[w] = n.ring[(uint64_t)w * n.S + slot_n];
+            psr_n = n.psr[slot_n];
         }
-        float psr = n.psr[slot];
-        const float decay = n.decay[type];
-#pragma unroll
-        for (int k = 0; k < KMAX; k++) {
-            if (k < K) {
-                const uint64_t t = t0 + (uint64_t)k;
-                const uint32_t b = (uint32_t)(t & Rm);
-                const uint32_t m = 1u << (b & 31);
-                bool hit = false;
-#pragma unroll
-                for (int w = 0; w < W32; w++)
-                    if ((int)(b >> 5) == w && (rw[w] & m)) {
-                        rw[w] &= ~m;
-                        hit = true;
-                    }
-                if (hit) {
-                    changed = true;
-                    psr = change_psr<SK>(n, slot, t, psr, decay);
-                }
-                psr = psr * decay;
-                acc[k] += psr;
-            }
-        }
+        fused_synapse<SK, W32, KMAX>(n, slot, meta, rw, psr, t0, K, pull != 0, tprev0, mprev, acc);
         n.psr[slot] = psr;
-        if (changed) {
+        q = qn;
+        slot = slot_n;
+        meta = meta_n;
 #pragma unroll
-            for (int w = 0; w < W32; w++) n.ring[(uint64_t)w * n.S + slot] = rw[w];
-        }
+        for (int w = 0; w < W32; w++) rw[w] = rw_n[w];
+        psr = psr_n;
     }
 #pragma unroll
     for (int k = 0; k < KMAX; k++)
         if (k < K) acc[k] = wave_sum(acc[k]);
+
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     if (lane == 0) {
         NeuronRegs s;
@@ -164,7 +225,7 @@ __global__ __launch_bounds__(BG_BLOCK) void k_fused_block(DevNet n, uint64_t t0,
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
             if (k < K) {
-                if (neuron_advance<NK>(n, d, t0 + (uint64_t)k, sp, s)) mask |= 1u << k;
+                if (neuron_advance<NK>(n, d, t0 + (uint64_t)k, sp, s_noise[wv][k], s)) mask |= 1u << k;
                 sp = 0.0f + acc[k];
             }
         }
@@ -174,4 +235,3 @@ __global__ __launch_bounds__(BG_BLOCK) void k_fused_block(DevNet n, uint64_t t0,
         if (mask) n.fired_step[d] = t0 + (uint64_t)(31 - __clz(mask));
     }
 }
-

@@ -1,0 +1,464 @@
+// HipSpikingModel: the host half of the MI355X model (include/bghost.h).
+//
+// Mirrors the reference's IModel life cycle (Core/IModel.h:42-152) over the C ABI of
+// include/bghip.h:
+//   setupSim          Model::setupSim (Core/Model.cpp:136-158): layout, createAllNeurons with the
+//                     global MTRand rng(1) (Utils/Global.cpp:79), ConnGrowth radii; then the GPU
+//                     arena is created and uploaded (GPUSpikingModel::setupSim,
+//                     Core/GPUSpikingModel.cu:106-145);
+//   advance           GPUSpikingModel::advance (Core/GPUSpikingModel.cu:188-225);
+//   updateConnections GPUSpikingModel::updateConnections (:246-259), entirely on the GPU;
+//   updateHistory     XmlGrowthRecorder::compileHistories (Recorders/XmlGrowthRecorder.cpp:72-97)
+//                     from downloaded spikes, rates and radii, then the device clamp/clear;
+//   saveData          XmlGrowthRecorder::saveSimData (:104-154), byte-compatible output.
+// Host arithmetic keeps the reference's float/double types, compiled with -ffp-contract=off.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <new>
+#include <string>
+#include <vector>
+#include "../../include/bghost.h"
+
+namespace bg {
+
+// MT19937 exactly as MTRand (RNG/MersenneTwister.cpp:66-291), restated from the MT19937 paper.
+class MTRand {
+  public:
+    explicit MTRand(uint32_t seed) { reseed(seed); }
+    void reseed(uint32_t seed)
+    {
+        s_[0] = seed;
+        for (int i = 1; i < 624; i++) s_[i] = 1812433253u * (s_[i - 1] ^ (s_[i - 1] >> 30)) + (uint32_t)i;
+        reload();
+    }
+    uint32_t randInt()
+    {
+        if (left_ == 0) reload();
+        --left_;
+        uint32_t y = s_[next_++];
+        y ^= (y >> 11);
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        return y ^ (y >> 18);
+    }
+    float rand() { return (float)((double)(float)randInt() * (1.0 / 4294967295.0)); }
+    float inRange(float lo, float hi)
+    {
+        float v = rand();
+        float range = hi - lo;
+        v *= range;
+        v += lo;
+        return v;
+    }
+
+  private:
+    static uint32_t twist(uint32_t m, uint32_t a, uint32_t b)
+    {
+        return m ^ (((a & 0x80000000u) | (b & 0x7fffffffu)) >> 1) ^ ((uint32_t)(-(int32_t)(b & 1u)) & 0x9908b0dfu);
+    }
+    void reload()
+    {
+        int i = 0;
+        for (; i < 227; i++) s_[i] = twist(s_[i + 397], s_[i], s_[i + 1]);
+        for (; i < 623; i++) s_[i] = twist(s_[i - 227], s_[i], s_[i + 1]);
+        s_[623] = twist(s_[396], s_[623], s_[0]);
+        left_ = 624;
+        next_ = 0;
+    }
+    uint32_t s_[624];
+    int left_ = 0, next_ = 0;
+};
+
+enum { INH = 1, EXC = 2 };
+
+class HipSpikingModel {
+  public:
+    HipSpikingModel(bghip_ctx *ctx, const bghost_params &p) : ctx_(ctx), p_(p), rng_(1u) {}
+    ~HipSpikingModel() { cleanupSim(); }
+
+    int setupSim();
+    int advance(uint64_t n) { return bghip_advance(net_, n); }
+    int advanceUntilGrowth()
+    {
+        uint64_t steps = (uint64_t)(p_.epoch_duration / p_.dt);   // Core/Simulator.cpp:159-160
+        return advance(steps);
+    }
+    int updateConnections(bghip_growth_stats *st)
+    {
+        if (p_.conn_kind != BGHOST_CONN_GROWTH) return BGHIP_OK;
+        return bghip_update_connections(net_, st);
+    }
+    int updateHistory();
+    int saveData(const char *path);
+    void cleanupSim()
+    {
+        if (net_) bghip_net_destroy(net_);
+        net_ = nullptr;
+    }
+    int setRadii(float r)
+    {
+        for (auto &x : radii_) x = r;
+        return bghip_upload_growth(net_, &growth_, radii_.data(), rates_.data());
+    }
+    const void *array(const char *name, int64_t *count);
+    bghip_net *net() { return net_; }
+    int epoch() const { return epoch_; }
+
+  private:
+    void setupLayout();
+    void createAllNeurons();
+    bghip_synapse_types synapseTypes() const;
+
+    bghip_ctx *ctx_;
+    bghost_params p_;
+    MTRand rng_;
+    bghip_net *net_ = nullptr;
+    int N_ = 0, max_spikes_ = 0, epoch_ = 0;
+    std::vector<float> xloc_, yloc_;
+    std::vector<int32_t> ntype_;
+    std::vector<uint8_t> starter_;
+    std::vector<float> Vm_, Vthresh_, Vreset_, Vrest_, Vinit_, I0_, Inoise_, Iinject_, C1_, C2_, Trefract_, Tau_, Rm_;
+    std::vector<float> A_, B_, C_, D_, u_, C3_;
+    std::vector<float> radii_, rates_, radiiHistory_, ratesHistory_, burst_, spikesHist_;
+    std::vector<int32_t> spikeCount_, spikeOffset_;
+    std::vector<uint64_t> hist_;
+    bghip_growth growth_;
+};
+
+void HipSpikingModel::setupLayout()
+{
+    const int N = N_;
+    for (int i = 0; i < N; i++) {   // Layouts/Layout.cpp:155-157, 104-106, 117-119
+        xloc_[i] = (float)(i % p_.width);
+        yloc_[i] = (float)(i / p_.width);
+        ntype_[i] = EXC;
+        starter_[i] = 0;
+    }
+    if (p_.layout_kind == BGHOST_LAYOUT_FIXED) {   // Layouts/FixedLayout.cpp:247-279
+        for (int i = 0; i < p_.n_inh; i++) ntype_[p_.inh_list[i]] = INH;
+        for (int i = 0; i < p_.n_active; i++) starter_[p_.active_list[i]] = 1;
+    } else {                                        // Layouts/DynamicLayout.cpp:98-165
+        int num_exc = (int)((double)(p_.frac_exc * N) + 0.5);
+        int num_inh = N - num_exc;
+        std::vector<int> rg(num_inh > 0 ? num_inh : 1);
+        for (int i = 0; i < num_inh; i++) rg[i] = i;
+        for (int i = num_inh; i < N; i++) {
+            int j = (int)(rng_.rand() * N);
+            if (j < num_inh) rg[j] = i;
+        }
+        for (int i = 0; i < num_inh; i++) ntype_[rg[i]] = INH;
+        uint32_t want = (uint32_t)((double)(p_.frac_starter * N) + 0.5), got = 0;
+        while (got < want) {
+            int i = (int)rng_.inRange(0.0f, (float)N);
+            if (ntype_[i] == EXC && !starter_[i]) {
+                starter_[i] = 1;
+                got++;
+            }
+        }
+    }
+}
+
+void HipSpikingModel::createAllNeurons()
+{   // AllIFNeurons::createNeuron (Neurons/AllIFNeurons.cpp:398-503), AllIZHNeurons (:413-476)
+    const float dt = p_.dt;
+    const bool izh = p_.neuron_model == BGHIP_NEURON_IZH;
+    for (int i = 0; i < N_; i++) {
+        Rm_[i] = (float)1e6;
+        Tau_[i] = (float)(3e-8 * 1e6);
+        Iinject_[i] = rng_.inRange(p_.Iinject[0], p_.Iinject[1]);
+        Inoise_[i] = rng_.inRange(p_.Inoise[0], p_.Inoise[1]);
+        Vthresh_[i] = rng_.inRange(p_.Vthresh[0], p_.Vthresh[1]);
+        Vrest_[i] = rng_.inRange(p_.Vresting[0], p_.Vresting[1]);
+        Vreset_[i] = rng_.inRange(p_.Vreset[0], p_.Vreset[1]);
+        Vinit_[i] = rng_.inRange(p_.Vinit[0], p_.Vinit[1]);
+        Vm_[i] = Vinit_[i];
+        if (Tau_[i] > 0) {
+            C1_[i] = expf(-dt / Tau_[i]);
+            C2_[i] = Rm_[i] * (1 - C1_[i]);
+        } else {
+            C1_[i] = 0.0f;
+            C2_[i] = Rm_[i];
+        }
+        I0_[i] = Iinject_[i] + Vrest_[i] / Rm_[i];
+        if (izh) C3_[i] = dt * 1000;
+        Trefract_[i] = ntype_[i] == INH ? (float)2.0e-3 : (float)3.0e-3;
+        if (starter_[i]) {
+            Vthresh_[i] = rng_.inRange(p_.starter_Vthresh[0], p_.starter_Vthresh[1]);
+            Vreset_[i] = rng_.inRange(p_.starter_Vreset[0], p_.starter_Vreset[1]);
+            Trefract_[i] = (float)3.0e-3;
+        }
+        if (izh) {
+            const float *a = ntype_[i] == EXC ? p_.excA : p_.inhA, *b = ntype_[i] == EXC ? p_.excB : p_.inhB;
+            const float *c = ntype_[i] == EXC ? p_.excC : p_.inhC, *d = ntype_[i] == EXC ? p_.excD : p_.inhD;
+            A_[i] = rng_.inRange(a[0], a[1]);
+            B_[i] = rng_.inRange(b[0], b[1]);
+            C_[i] = rng_.inRange(c[0], c[1]);
+            D_[i] = rng_.inRange(d[0], d[1]);
+            u_[i] = 0.0f;
+        }
+    }
+}
+
+bghip_synapse_types HipSpikingModel::synapseTypes() const
+{   // per-type constants of createSynapse (Synapses/AllSpikingSynapses.cpp:191-233,
+    // AllDSSynapses.cpp:180-218, AllSTDPSynapses.cpp:252-273)
+    bghip_synapse_types t;
+    memset(&t, 0, sizeof t);
+    const float tau[4] = {(float)6e-3, (float)6e-3, (float)3e-3, (float)3e-3};
+    const float delay[4] = {(float)0.8e-3, (float)0.8e-3, (float)0.8e-3, (float)1.5e-3};
+    const float U[4] = {(float)0.32, (float)0.25, (float)0.05, (float)0.5};
+    const float D[4] = {(float)0.144, (float)0.7, (float)0.125, (float)1.1};
+    const float F[4] = {(float)0.06, (float)0.02, (float)1.2, (float)0.05};
+    for (int k = 0; k < 4; k++) {
+        t.tau[k] = tau[k];
+        t.decay[k] = expf(-p_.dt / tau[k]);
+        t.total_delay[k] = (int)(delay[k] / p_.dt) + 1;
+        t.U[k] = U[k];
+        t.D[k] = D[k];
+        t.F[k] = F[k];
+    }
+    t.u0 = (float)0.4;
+    t.r0 = 1.0f;
+    t.Apos = 0.5f; t.Aneg = -0.5f; t.STDPgap = (float)2e-3; t.tauspost = (float)75e-3; t.tauspre = (float)34e-3;
+    t.taupos = (float)15e-3; t.tauneg = (float)35e-3; t.Wex = 1.0f; t.mupos = 0.0f; t.muneg = 0.0f;
+    t.useFroemkeDanSTDP = 1;
+    return t;
+}
+
+int HipSpikingModel::setupSim()
+{
+    N_ = p_.width * p_.height;
+    if (N_ <= 0) return BGHIP_EINVAL;
+    max_spikes_ = (int)(p_.epoch_duration * p_.max_firing_rate);
+    const int E = p_.num_epochs > 0 ? p_.num_epochs : 1;
+    size_t N = N_;
+    for (auto *v : {&xloc_, &yloc_, &Vm_, &Vthresh_, &Vreset_, &Vrest_, &Vinit_, &I0_, &Inoise_, &Iinject_, &C1_,
+                    &C2_, &Trefract_, &Tau_, &Rm_, &A_, &B_, &C_, &D_, &u_, &C3_, &radii_, &rates_})
+        v->assign(N, 0.0f);
+    ntype_.assign(N, EXC);
+    starter_.assign(N, 0);
+    spikeCount_.assign(N, 0);
+    spikeOffset_.assign(N, 0);
+    hist_.assign(N * (size_t)max_spikes_, ~0ull);
+    radiiHistory_.assign((size_t)(E + 1) * N, 0.0f);
+    ratesHistory_.assign((size_t)(E + 1) * N, 0.0f);
+    burst_.assign((size_t)(int)(p_.epoch_duration * p_.num_epochs), 0.0f);
+    spikesHist_.assign((size_t)(int)(p_.epoch_duration * p_.num_epochs * 100), 0.0f);
+    rng_.reseed(1u);
+    setupLayout();
+    createAllNeurons();
+    for (size_t i = 0; i < N; i++) {   // ConnGrowth::setupConnections, recorder initDefaultValues
+        radii_[i] = p_.start_radius;
+        radiiHistory_[i] = p_.start_radius;
+    }
+    growth_.epsilon = p_.epsilon; growth_.beta = p_.beta; growth_.rho = p_.rho; growth_.targetRate = p_.target_rate;
+    growth_.maxRate = p_.target_rate / p_.epsilon; growth_.minRadius = p_.min_radius;
+    growth_.startRadius = p_.start_radius; growth_.epochDuration = p_.epoch_duration;
+
+    bghip_net_desc d;
+    memset(&d, 0, sizeof d);
+    d.num_neurons = N_;
+    d.max_synapses_per_neuron = p_.max_synapses_per_neuron;
+    d.neuron_model = p_.neuron_model;
+    d.synapse_model = p_.synapse_model;
+    d.ring_bits = p_.ring_bits ? p_.ring_bits : 32;
+    d.max_spikes = max_spikes_;
+    d.dt = p_.dt;
+    d.noise_seed = p_.seed;
+    d.block_steps = p_.block_steps;
+    d.types = synapseTypes();
+    int rc = bghip_net_create(ctx_, &d, &net_);
+    if (rc) return rc;
+    std::vector<float> zero(N, 0.0f);
+    std::vector<int32_t> izero(N, 0);
+    bghip_neurons h;
+    memset(&h, 0, sizeof h);
+    h.Vm = Vm_.data(); h.Vthresh = Vthresh_.data(); h.Vreset = Vreset_.data(); h.I0 = I0_.data();
+    h.Inoise = Inoise_.data(); h.C1 = C1_.data(); h.C2 = C2_.data(); h.Trefract = Trefract_.data();
+    h.summation_map = zero.data(); h.nStepsInRefr = izero.data(); h.spikeCount = spikeCount_.data();
+    h.spikeCountOffset = spikeOffset_.data(); h.spike_history = hist_.data();
+    if (p_.neuron_model == BGHIP_NEURON_IZH) {
+        h.Aconst = A_.data(); h.Bconst = B_.data(); h.Cconst = C_.data(); h.Dconst = D_.data(); h.u = u_.data();
+        h.C3 = C3_.data();
+    }
+    if ((rc = bghip_upload_neurons(net_, &h))) return rc;
+    if ((rc = bghip_upload_layout(net_, xloc_.data(), yloc_.data(), ntype_.data()))) return rc;
+    return bghip_upload_growth(net_, &growth_, radii_.data(), rates_.data());
+}
+
+int HipSpikingModel::updateHistory()
+{
+    bghip_neurons h;
+    memset(&h, 0, sizeof h);
+    h.spikeCount = spikeCount_.data();
+    h.spikeCountOffset = spikeOffset_.data();
+    h.spike_history = hist_.data();
+    int rc = bghip_download_neurons(net_, &h);
+    if (rc) return rc;
+    const float dt = p_.dt;
+    for (int i = 0; i < N_; i++) {   // Recorders/XmlRecorder.cpp:77-100
+        for (int q = 0, idx = spikeOffset_[i]; q < spikeCount_[i]; q++, idx++) {
+            if (idx >= max_spikes_) idx = 0;
+            uint64_t s = hist_[(size_t)i * max_spikes_ + idx];
+            int i1 = (int)((double)s * (double)dt), i2 = (int)((double)s * (double)dt * 100);
+            if (i1 >= 0 && (size_t)i1 < burst_.size()) burst_[i1] = (float)((double)burst_[i1] + 1.0);
+            if (i2 >= 0 && (size_t)i2 < spikesHist_.size()) spikesHist_[i2] = (float)((double)spikesHist_[i2] + 1.0);
+        }
+    }
+    epoch_++;
+    if (p_.conn_kind == BGHOST_CONN_GROWTH) {
+        if ((rc = bghip_download_growth(net_, radii_.data(), rates_.data()))) return rc;
+        if (epoch_ <= (p_.num_epochs > 0 ? p_.num_epochs : 1)) {   // Recorders/XmlGrowthRecorder.cpp:82-96
+            for (int i = 0; i < N_; i++) {
+                ratesHistory_[(size_t)epoch_ * N_ + i] = rates_[i];
+                float r = radii_[i] < p_.min_radius ? p_.min_radius : radii_[i];
+                radiiHistory_[(size_t)epoch_ * N_ + i] = r;
+            }
+        }
+    }
+    if ((rc = bghip_update_history(net_))) return rc;
+    if (p_.conn_kind == BGHOST_CONN_GROWTH) rc = bghip_download_growth(net_, radii_.data(), rates_.data());
+    return rc;
+}
+
+static void put_matrix(FILE *f, const char *name, const float *v, int rows, int cols)
+{   // CompleteMatrix::toXML / Print (Matrix/CompleteMatrix.cpp:225-249): "%g" is ostream's default
+    fprintf(f, "   <Matrix name=\"%s\" type=\"complete\" rows=\"%d\" columns=\"%d\" multiplier=\"1.0\">\n   ", name,
+            rows, cols);
+    for (int r = 0; r < rows; r++) {
+        for (int c = 0; c < cols; c++) fprintf(f, "%g ", (double)v[(size_t)r * cols + c]);
+        fprintf(f, "\n");
+    }
+    fprintf(f, "\n</Matrix>\n");
+}
+static void put_vector(FILE *f, const char *name, const float *v, int n)
+{   // VectorMatrix::toXML / Print (Matrix/VectorMatrix.cpp:173-190)
+    fprintf(f, "   <Matrix name=\"%s\" type=\"complete\" rows=\"1\" columns=\"%d\" multiplier=\"1.0\">\n   ", name, n);
+    for (int i = 0; i < n; i++) fprintf(f, "%g ", (double)v[i]);
+    fprintf(f, "\n</Matrix>\n");
+}
+
+int HipSpikingModel::saveData(const char *path)
+{   // XmlGrowthRecorder::saveSimData (Recorders/XmlGrowthRecorder.cpp:104-154)
+    FILE *f = fopen(path, "w");
+    if (!f) return BGHIP_EINVAL;
+    const int E = p_.num_epochs > 0 ? p_.num_epochs : 1;
+    std::vector<float> types(N_), starters;
+    for (int i = 0; i < N_; i++) {
+        types[i] = (float)ntype_[i];
+        if (starter_[i]) starters.push_back((float)i);
+    }
+    fprintf(f, "<?xml version=\"1.0\" standalone=\"no\"?>\n<!-- State output file for the DCT growth modeling-->\n");
+    fprintf(f, "<SimState>\n");
+    if (p_.conn_kind == BGHOST_CONN_GROWTH) {
+        put_matrix(f, "radiiHistory", radiiHistory_.data(), E + 1, N_);
+        put_matrix(f, "ratesHistory", ratesHistory_.data(), E + 1, N_);
+    }
+    put_vector(f, "burstinessHist", burst_.data(), (int)burst_.size());
+    put_vector(f, "spikesHistory", spikesHist_.data(), (int)spikesHist_.size());
+    put_vector(f, "xloc", xloc_.data(), N_);
+    put_vector(f, "yloc", yloc_.data(), N_);
+    put_vector(f, "neuronTypes", types.data(), N_);
+    if (!starters.empty()) put_vector(f, "starterNeurons", starters.data(), (int)starters.size());
+    put_vector(f, "neuronThresh", Vthresh_.data(), N_);
+    fprintf(f, "   <Matrix name=\"Tsim\" type=\"complete\" rows=\"1\" columns=\"1\" multiplier=\"1.0\">\n");
+    fprintf(f, "   %g\n</Matrix>\n", (double)p_.epoch_duration);
+    fprintf(f, "   <Matrix name=\"simulationEndTime\" type=\"complete\" rows=\"1\" columns=\"1\" multiplier=\"1.0\">\n");
+    fprintf(f, "   %g\n</Matrix>\n", (double)((float)bghip_net_step(net_) * p_.dt));
+    fprintf(f, "</SimState>\n");
+    fclose(f);
+    return BGHIP_OK;
+}
+
+const void *HipSpikingModel::array(const char *name, int64_t *count)
+{
+    struct Item { const char *n; const void *p; size_t c; };
+    const Item items[] = {
+        {"xloc", xloc_.data(), xloc_.size()}, {"yloc", yloc_.data(), yloc_.size()},
+        {"neuron_type_map", ntype_.data(), ntype_.size()}, {"starter_map", starter_.data(), starter_.size()},
+        {"Vm", Vm_.data(), Vm_.size()}, {"Vthresh", Vthresh_.data(), Vthresh_.size()},
+        {"Vreset", Vreset_.data(), Vreset_.size()}, {"Vinit", Vinit_.data(), Vinit_.size()},
+        {"Iinject", Iinject_.data(), Iinject_.size()}, {"Inoise", Inoise_.data(), Inoise_.size()},
+        {"I0", I0_.data(), I0_.size()}, {"C1", C1_.data(), C1_.size()}, {"C2", C2_.data(), C2_.size()},
+        {"Trefract", Trefract_.data(), Trefract_.size()}, {"Aconst", A_.data(), A_.size()},
+        {"Bconst", B_.data(), B_.size()}, {"Cconst", C_.data(), C_.size()}, {"Dconst", D_.data(), D_.size()},
+        {"C3", C3_.data(), C3_.size()}, {"radii", radii_.data(), radii_.size()},
+        {"rates", rates_.data(), rates_.size()}, {"radiiHistory", radiiHistory_.data(), radiiHistory_.size()},
+        {"ratesHistory", ratesHistory_.data(), ratesHistory_.size()},
+        {"burstinessHist", burst_.data(), burst_.size()}, {"spikesHistory", spikesHist_.data(), spikesHist_.size()},
+    };
+    for (const Item &it : items)
+        if (!strcmp(it.n, name)) {
+            if (count) *count = (int64_t)it.c;
+            return it.p;
+        }
+    return nullptr;
+}
+
+}  // namespace bg
+
+struct bghost_model {
+    bg::HipSpikingModel *m;
+    std::vector<int32_t> inh, act;
+};
+
+extern "C" int bghost_model_create(bghip_ctx *ctx, const bghost_params *p, bghost_model **out)
+{
+    if (!ctx || !p || !out) return BGHIP_EINVAL;
+    bghost_model *w = new (std::nothrow) bghost_model();
+    if (!w) return BGHIP_ENOMEM;
+    bghost_params q = *p;
+    w->inh.assign(p->inh_list, p->inh_list + (p->inh_list ? p->n_inh : 0));
+    w->act.assign(p->active_list, p->active_list + (p->active_list ? p->n_active : 0));
+    q.inh_list = w->inh.data();
+    q.active_list = w->act.data();
+    const int N = p->width * p->height;
+    for (int v : w->inh)
+        if (v < 0 || v >= N) { delete w; return BGHIP_EINVAL; }
+    for (int v : w->act)
+        if (v < 0 || v >= N) { delete w; return BGHIP_EINVAL; }
+    w->m = new (std::nothrow) bg::HipSpikingModel(ctx, q);
+    if (!w->m) { delete w; return BGHIP_ENOMEM; }
+    int rc = w->m->setupSim();
+    if (rc) {
+        delete w->m;
+        delete w;
+        return rc;
+    }
+    *out = w;
+    return BGHIP_OK;
+}
+
+extern "C" void bghost_model_destroy(bghost_model *w)
+{
+    if (!w) return;
+    delete w->m;
+    delete w;
+}
+extern "C" bghip_net *bghost_model_net(bghost_model *w) { return w ? w->m->net() : nullptr; }
+extern "C" int bghost_model_advance(bghost_model *w, uint64_t n) { return w ? w->m->advance(n) : BGHIP_EINVAL; }
+extern "C" int bghost_model_advance_epoch(bghost_model *w) { return w ? w->m->advanceUntilGrowth() : BGHIP_EINVAL; }
+extern "C" int bghost_model_update_connections(bghost_model *w, bghip_growth_stats *st)
+{
+    return w ? w->m->updateConnections(st) : BGHIP_EINVAL;
+}
+extern "C" int bghost_model_update_history(bghost_model *w) { return w ? w->m->updateHistory() : BGHIP_EINVAL; }
+extern "C" int bghost_model_run_epoch(bghost_model *w)
+{
+    if (!w) return BGHIP_EINVAL;
+    int rc = w->m->advanceUntilGrowth();
+    if (!rc) rc = w->m->updateConnections(nullptr);
+    if (!rc) rc = w->m->updateHistory();
+    return rc;
+}
+extern "C" int bghost_model_save_xml(bghost_model *w, const char *path)
+{
+    return w && path ? w->m->saveData(path) : BGHIP_EINVAL;
+}
+extern "C" int bghost_model_set_radii(bghost_model *w, float r) { return w ? w->m->setRadii(r) : BGHIP_EINVAL; }
+extern "C" const void *bghost_model_array(bghost_model *w, const char *name, int64_t *count)
+{
+    return w && name ? w->m->array(name, count) : nullptr;
+}
+extern "C" int32_t bghost_model_epoch(const bghost_model *w) { return w ? w->m->epoch() : -1; }

@@ -286,8 +286,82 @@ static inline BG_HD double bg_pow(double x, double y)
     return bg_exp(y * bg_log(x));
 }
 
-/* float entry points: evaluate in double, round once. */
-static inline BG_HD float bg_expf(float x) { return (float)bg_exp((double)x); }
+/* exp for float results: the reduction of bg_exp with the series truncated after r^10
+ * (remainder below 2.5e-13 relative on |r| <= ln2/2, far below float resolution). */
+static inline BG_HD double bg_exp_for_float(double x)
+{
+    const double ln2_hi = 0x1.62e42fee00000p-1, ln2_lo = 0x1.a39ef35793c76p-33;
+    const double inv_ln2 = 1.4426950408889634;
+    if (bg_isnan(x)) return x;
+    if (x > 709.78) return bg_from_bits(0x7ff0000000000000ull);
+    if (x < -745.2) return 0.0;
+    double t = x * inv_ln2;
+    int k = (int)(t + (t >= 0.0 ? 0.5 : -0.5));
+    double r = (x - (double)k * ln2_hi) - (double)k * ln2_lo;
+    double p = 2.755731922398589e-07;
+    p = p * r + 2.7557319223985893e-06;
+    p = p * r + 2.48015873015873e-05;
+    p = p * r + 0.0001984126984126984;
+    p = p * r + 0.001388888888888889;
+    p = p * r + 0.008333333333333333;
+    p = p * r + 0.041666666666666664;
+    p = p * r + 0.16666666666666666;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return bg_scale2(p, k);
+}
+
+/* exp in single precision for the per-spike synapse updates: x = k ln2 + r with a 16-bit ln2_hi
+ * (k * ln2_hi exact for |k| < 256), |r| <= ln2/2, Taylor series to r^7 (truncation below 5e-9
+ * relative), scaled by 2^k through the exponent bits. About one ulp from the correctly rounded
+ * value; every operation is a single IEEE float operation, so host and device agree bit for bit. */
+static inline BG_HD float bg_scale2f(float x, int k)
+{
+    uint32_t b;
+    float s;
+    if (k < -126) {
+        b = (uint32_t)(-126 + 127) << 23;
+        memcpy(&s, &b, 4);
+        x *= s;
+        k += 126;
+    } else if (k > 127) {
+        b = (uint32_t)(127 + 127) << 23;
+        memcpy(&s, &b, 4);
+        x *= s;
+        k -= 127;
+    }
+    b = (uint32_t)(k + 127) << 23;
+    memcpy(&s, &b, 4);
+    return x * s;
+}
+static inline BG_HD float bg_expf(float x)
+{
+    if (x != x) return x;
+    if (x > 88.72283935546875f) {
+        uint32_t b = 0x7f800000u;
+        float inf;
+        memcpy(&inf, &b, 4);
+        return inf;
+    }
+    if (x < -103.972076416015625f) return 0.0f;
+    const float ln2_hi = 0.693145751953125f, ln2_lo = 1.428606765330187e-06f;
+    float t = x * 1.4426950216293335f;
+    int k = (int)(t + (t >= 0.0f ? 0.5f : -0.5f));
+    float kf = (float)k;
+    float r = (x - kf * ln2_hi) - kf * ln2_lo;
+    float p = 0.00019841270113829523f;
+    p = p * r + 0.0013888889225199819f;
+    p = p * r + 0.008333333767950535f;
+    p = p * r + 0.0416666679084301f;
+    p = p * r + 0.1666666716337204f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return bg_scale2f(p, k);
+}
+
+/* float entry points of the double-precision functions: evaluate in double, round once. */
 static inline BG_HD float bg_acosf(float x) { return (float)bg_acos((double)x); }
 static inline BG_HD float bg_sinf(float x) { return (float)bg_sin((double)x); }
 static inline BG_HD float bg_powf(float x, float y) { return (float)bg_pow((double)x, (double)y); }

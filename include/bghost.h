@@ -1,0 +1,76 @@
+/*
+ * bghost.h -- C ABI of the host-side model (HipSpikingModel) that drives libbghip.so.
+ *
+ * HipSpikingModel is the MI355X counterpart of GPUSpikingModel (Core/GPUSpikingModel.h): it owns
+ * the host state BrainGrid's Model::setupSim creates (layout, neuron parameters drawn from the
+ * global MTRand rng(1), growth radii, recorder histories) and runs the IModel life cycle
+ *   setupSim -> { advance x steps, updateConnections, updateHistory } per epoch -> saveData
+ * with every per-step and per-epoch computation on the GPU. The same class is what a BrainGrid
+ * build links behind IModel; this C ABI exposes it to other languages.
+ */
+#ifndef BGHOST_H
+#define BGHOST_H
+
+#include <stdint.h>
+#include "bghip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { BGHOST_LAYOUT_FIXED = 0, BGHOST_LAYOUT_DYNAMIC = 1 };
+enum { BGHOST_CONN_NONE = 0, BGHOST_CONN_GROWTH = 1 };
+
+/* The simulation parameters of a BrainGrid XML config (Core/SimulationInfo.h:89-155, neuron
+ * ranges of Neurons/AllIFNeurons.h and AllIZHNeurons.h, ConnGrowth's m_growth, layout lists). */
+typedef struct bghost_params {
+    int32_t width, height;
+    int32_t neuron_model, synapse_model;     /* BGHIP_NEURON_*, BGHIP_SYN_* */
+    int32_t layout_kind, conn_kind;          /* BGHOST_LAYOUT_*, BGHOST_CONN_* */
+    int32_t max_synapses_per_neuron, max_firing_rate, num_epochs, ring_bits;
+    float dt, epoch_duration;
+    uint32_t seed;                           /* XML Seed: key of the deterministic noise */
+    const int32_t *inh_list; int32_t n_inh;
+    const int32_t *active_list; int32_t n_active;
+    float frac_exc, frac_starter;
+    float Iinject[2], Inoise[2], Vthresh[2], Vresting[2], Vreset[2], Vinit[2];
+    float starter_Vthresh[2], starter_Vreset[2];
+    float excA[2], excB[2], excC[2], excD[2], inhA[2], inhB[2], inhC[2], inhD[2];
+    float epsilon, beta, rho, target_rate, min_radius, start_radius;
+    int32_t block_steps;                     /* 0 = automatic */
+} bghost_params;
+
+typedef struct bghost_model bghost_model;
+
+/* Model::setupSim + HipSpikingModel device setup (allocate, upload). */
+int bghost_model_create(bghip_ctx *ctx, const bghost_params *p, bghost_model **out);
+void bghost_model_destroy(bghost_model *m);
+bghip_net *bghost_model_net(bghost_model *m);
+
+/* IModel::advance n_steps times (asynchronous). */
+int bghost_model_advance(bghost_model *m, uint64_t n_steps);
+/* Simulator::advanceUntilGrowth: one epoch of steps. */
+int bghost_model_advance_epoch(bghost_model *m);
+/* IModel::updateConnections (on the GPU). */
+int bghost_model_update_connections(bghost_model *m, bghip_growth_stats *stats);
+/* IModel::updateHistory: recorder histories on the host from downloaded spikes/rates/radii
+ * (Recorders/XmlRecorder.cpp:71-104, XmlGrowthRecorder.cpp:72-97) + device clamp/clear. */
+int bghost_model_update_history(bghost_model *m);
+/* One full epoch: advance_epoch + update_connections + update_history. */
+int bghost_model_run_epoch(bghost_model *m);
+/* XmlGrowthRecorder::saveSimData (Recorders/XmlGrowthRecorder.cpp:104-154), same bytes. */
+int bghost_model_save_xml(bghost_model *m, const char *path);
+/* Overwrite all radii on host and device (pre-grown measurement states). */
+int bghost_model_set_radii(bghost_model *m, float r);
+
+/* Host arrays owned by the model; count receives the element count. Names: "xloc", "yloc",
+ * "neuron_type_map", "starter_map" (uint8), "Vthresh", "Vreset", "Vinit", "Iinject", "Inoise", "I0",
+ * "C1", "C2", "Trefract", "Aconst".."Dconst", "C3", "radiiHistory", "ratesHistory",
+ * "burstinessHist", "spikesHistory". Returns NULL for unknown names. */
+const void *bghost_model_array(bghost_model *m, const char *name, int64_t *count);
+int32_t bghost_model_epoch(const bghost_model *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
