@@ -703,13 +703,20 @@ int bgo_advance(bgo_net *n, int64_t n_steps)
 /* ------------------------------------------------------------------ growth ---------------- */
 /* Overlap area of circles around src (r1) and dest (r2), ConnGrowth::updateOverlap
  * (Connections/ConnGrowth.cpp:337-376) with delta from updateFrontiers (:319-329). */
+static float overlap_xy(float xi, float yi, float r1, float xj, float yj, float r2, float (*f_acosf)(float),
+                        float (*f_sinf)(float));
 static float overlap_area(bgo_net *n, int i, int j)
 {
     if (i == j) return 0.0f;   /* delta(i,i) keeps its initial value dist(i,i) = 0 */
-    float dx = n->xloc[i] - n->xloc[j], dy = n->yloc[i] - n->yloc[j];
+    return overlap_xy(n->xloc[i], n->yloc[i], n->radii[i], n->xloc[j], n->yloc[j], n->radii[j], n->f_acosf,
+                      n->f_sinf);
+}
+static float overlap_xy(float xi, float yi, float r1, float xj, float yj, float r2, float (*f_acosf)(float),
+                        float (*f_sinf)(float))
+{
+    float dx = xi - xj, dy = yi - yj;
     float lenAB2 = dx * dx + dy * dy;          /* Layouts/Layout.cpp:59-60 */
     float lenAB = sqrtf(lenAB2);               /* Layouts/Layout.cpp:69 */
-    float r1 = n->radii[i], r2 = n->radii[j];
     float delta = lenAB - (r1 + r2);
     if (!(delta < 0)) return 0.0f;
     float mn = r1 < r2 ? r1 : r2, mx = r1 < r2 ? r2 : r1;   /* std::min / std::max */
@@ -719,12 +726,12 @@ static float overlap_area(bgo_net *n, int i, int j)
     }
     float r12 = r1 * r1, r22 = r2 * r2;
     float cosCBA = (float)((double)(r22 + lenAB2 - r12) / (2.0 * (double)r2 * (double)lenAB));
-    float angCBA = n->f_acosf(cosCBA);
+    float angCBA = f_acosf(cosCBA);
     float angCBD = (float)(2.0 * (double)angCBA);
     float cosCAB = (float)((double)(r12 + lenAB2 - r22) / (2.0 * (double)r1 * (double)lenAB));
-    float angCAB = n->f_acosf(cosCAB);
+    float angCAB = f_acosf(cosCAB);
     float angCAD = (float)(2.0 * (double)angCAB);
-    return (float)(0.5 * (double)(r22 * (angCBD - n->f_sinf(angCBD)) + r12 * (angCAD - n->f_sinf(angCAD))));
+    return (float)(0.5 * (double)(r22 * (angCBD - f_sinf(angCBD)) + r12 * (angCAD - f_sinf(angCAD))));
 }
 
 int bgo_update_connections(bgo_net *n)
@@ -874,4 +881,33 @@ void *bgo_array(bgo_net *n, const char *name, char *dtype, int64_t *count)
     A("burstinessHist", n->burst, 'f', n->n_burst) A("spikesHistory", n->spikesHist, 'f', n->n_spikesHist)
 #undef A
     return NULL;
+}
+
+/* ------------------------------------------------------------------ test access ----------- */
+float bgo_overlap_area(float x1, float y1, float r1, float x2, float y2, float r2, int math_mode)
+{
+    return math_mode ? overlap_xy(x1, y1, r1, x2, y2, r2, own_acosf, own_sinf)
+                     : overlap_xy(x1, y1, r1, x2, y2, r2, lib_acosf, lib_sinf);
+}
+void bgo_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { bg_philox4x32_10(ctr, key, out); }
+float bgo_noise(uint64_t seed, uint32_t neuron, uint64_t step) { return bg_noise_normal(seed, neuron, step); }
+float bgo_norm_stream(uint32_t seed, int n, float *out)
+{
+    norm_t g;
+    norm_seed(&g, seed);
+    for (int i = 0; i < n; i++) out[i] = norm_next(&g);
+    return n > 0 ? out[n - 1] : 0.0f;
+}
+double bgo_detmath(int fn, double x, double y)
+{
+    switch (fn) {
+    case 0: return bg_exp(x);
+    case 1: return bg_log(x);
+    case 2: return bg_sin(x);
+    case 3: return bg_cos(x);
+    case 4: return bg_acos(x);
+    case 5: return bg_pow(x, y);
+    case 6: return bg_cos2pi(x);
+    default: return (double)bg_expf((float)x);
+    }
 }

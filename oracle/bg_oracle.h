@@ -89,6 +89,13 @@ int64_t bgo_step(const bgo_net *n);
 int64_t bgo_total_synapses(const bgo_net *n);
 const char *bgo_last_error(const bgo_net *n);
 
+/* Direct access to the pieces tests pin individually. */
+float bgo_overlap_area(float x1, float y1, float r1, float x2, float y2, float r2, int math_mode);
+void bgo_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float bgo_noise(uint64_t seed, uint32_t neuron, uint64_t step);
+float bgo_norm_stream(uint32_t seed, int n, float *out);   /* first n Norm draws */
+double bgo_detmath(int fn, double x, double y);            /* 0 exp 1 log 2 sin 3 cos 4 acos 5 pow 6 cos2pi 7 expf */
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,136 @@
+"""The CPU oracle against the compiled reference, bit for bit.
+
+Pinning of the oracle (oracle/bg_oracle.c), function by function:
+  - setup draws (MTRand, Fixed/DynamicLayout, createAllNeurons for LIF and IZH), LIF and IZH
+    advance with the Norm noise stream, spike recording, delay-ring insert/test, Spiking / DS /
+    STDP / DynamicSTDP synapse advance, sequential summation, growth (rates, radii, frontier,
+    overlap, weights, creation in the first free slot), createSynapseImap and the recorder's
+    clamp/clear: every array after setup, after every epoch's steps and after every growth update
+    equals the reference's dump (tests/golden/ref_*.npz, written by tests/golden/make_ref_fixtures.py
+    from oracle/_ref/bgref_dump) on ten configs;
+  - history compilation and output format: the formatted recorder output equals the reference's
+    own 4-2016 goldens (old/validation/4-2016/growth/test-{tiny,small}-out.xml) byte for byte;
+  - when oracle/_ref is present the same comparison also runs live against the reference.
+Functions exercised only in deterministic mode (Philox noise, own math, wave64 summation) are
+pinned in tests/test_detmath.py and tests/test_oracle_properties.py.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from tests import xmlout
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+REF_BASE = os.path.join(GOLDEN, "ref")
+CFG_BASE = os.path.join(GOLDEN, "configs")
+
+CONFIGS = {"test-tiny": (os.path.join(REF_BASE, "configfiles/test-tiny.xml"), REF_BASE)}
+for _n in ("conn-lif-ds", "conn-lif-ds-dense", "conn-lif-spiking", "conn-lif-stdp", "conn-lif-dynstdp",
+           "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"):
+    CONFIGS[_n] = (os.path.join(CFG_BASE, _n + ".xml"), CFG_BASE)
+
+
+def _cfg(name):
+    path, base = CONFIGS[name] if name in CONFIGS else (os.path.join(REF_BASE, f"configfiles/{name}.xml"), REF_BASE)
+    cfg = O.parse_config_xml(path, base)
+    if "dynamic" in name:
+        cfg.update(layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1)
+    return cfg
+
+
+def _records(npz):
+    z = np.load(npz)
+    n = len([k for k in z.files if k.endswith("__marker")])
+    out = []
+    for i in range(n):
+        tag = bytes(z[f"{i:03d}__tag"]).decode().strip()
+        arrs = {k[5:]: z[k] for k in z.files if k.startswith(f"{i:03d}_") and not k.startswith(f"{i:03d}__")}
+        out.append((tag, int(z[f"{i:03d}__marker"][0]), arrs))
+    return out
+
+
+def _compare(net, arrs, where):
+    bad = []
+    use = arrs.get("in_use")
+    m = use.astype(bool) if use is not None else None
+    for k, v in arrs.items():
+        try:
+            mine = net.array(k)
+        except KeyError:
+            continue
+        if m is not None and mine.size == m.size and k != "in_use":
+            a, b = v[m], mine[m]
+        else:
+            a, b = v, mine[:v.size]
+        if a.dtype.kind == "f":
+            eq = np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        else:
+            eq = np.array_equal(a, b)
+        if not eq:
+            bad.append(f"{where}:{k}")
+    return bad
+
+
+def _replay(name, recs):
+    cfg = _cfg(name)
+    net = O.OracleNet(cfg)
+    steps = int(np.float32(np.float32(cfg["epoch_duration"]) / np.float32(cfg["dt"])))
+    bad = []
+    for tag, ep, arrs in recs:
+        if tag == "advanced":
+            net.advance(steps)
+        elif tag == "grown":
+            net.update_connections()
+            net.update_history()
+        bad += _compare(net, arrs, f"{tag}{ep}")
+    return net, bad
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_oracle_matches_reference_fixtures(name):
+    recs = _records(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    net, bad = _replay(name, recs)
+    assert not bad, bad[:10]
+    if name != "test-tiny":
+        assert net.total_synapses > 0
+
+
+def test_oracle_matches_reference_test_small_fixture():
+    recs = _records(os.path.join(GOLDEN, "ref_test-small.npz"))
+    net, bad = _replay("test-small", recs)
+    assert not bad, bad[:10]
+
+
+@pytest.mark.parametrize("name", ["test-tiny", "test-small"])
+def test_oracle_output_equals_2016_golden_xml(name):
+    cfg = _cfg(name)
+    net = O.OracleNet(cfg)
+    for _ in range(cfg["num_epochs"]):
+        net.run_epoch()
+    mine = xmlout.oracle_growth_xml(net, cfg)
+    with open(os.path.join(REF_BASE, f"{name}-out.xml")) as f:
+        assert mine == f.read()
+
+
+@pytest.mark.skipif(not O.ref_available() or not os.path.isdir("/root/reference"),
+                    reason="compiled reference not present (built only where the reference sources are)")
+@pytest.mark.parametrize("name", ["conn-lif-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"])
+def test_oracle_matches_live_reference(name):
+    path, base = CONFIGS[name]
+    with tempfile.TemporaryDirectory() as td:
+        dump = os.path.join(td, "d.bin")
+        extra = ["--dynamic-layout", "0.9", "0.1"] if "dynamic" in name else []
+        subprocess.run([os.path.join(O.REF_DIR, "bgref_dump"), path, dump] + extra, cwd=base, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        recs = [r for r in O.read_dump(dump) if r[0] != "header"]
+        header = [r for r in O.read_dump(dump) if r[0] == "header"]
+    net, bad = _replay(name, recs)
+    assert not bad, bad[:10]
+    if header:
+        fresh = O.OracleNet(_cfg(name))
+        assert not _compare(fresh, header[0][2], "header")
