@@ -1,0 +1,139 @@
+"""GPU tests of the host model (HipSpikingModel through include/bghost.h) and of growth at scale."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from braingrid_amd import host
+from oracle import oracle_py as O
+from tests import helpers as H
+from tests import xmlout
+
+pytestmark = pytest.mark.gpu
+
+SETUP_ARRAYS = ("xloc", "yloc", "neuron_type_map", "starter_map", "Vthresh", "Vreset", "Vinit", "Iinject", "Inoise",
+                "I0", "C1", "C2", "Trefract")
+IZH_ARRAYS = ("Aconst", "Bconst", "Cconst", "Dconst", "C3")
+
+
+def _host_cfg(name):
+    c = host.read_config(os.path.join(H.CONFIG_DIR, name + ".xml"), H.CONFIG_DIR)
+    if "dynamic" in name:
+        c.update(layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1)
+    return c
+
+
+@pytest.mark.parametrize("name", ["conn-lif-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"])
+def test_host_setup_matches_oracle(hip_ctx, name):
+    print(f"[host] setup {name}", flush=True)
+    m = host.Model(hip_ctx, _host_cfg(name))
+    ora = O.OracleNet(H.load_config(name))
+    names = SETUP_ARRAYS + (IZH_ARRAYS if "izh" in name else ())
+    bad = [k for k in names if not np.array_equal(m.array(k).view(np.uint8), ora.array(k).view(np.uint8))]
+    m.close()
+    assert not bad
+
+
+def test_host_setup_matches_oracle_at_10k(hip_ctx):
+    c = host.base_config()
+    c.update(width=100, height=100, max_synapses_per_neuron=4, num_epochs=1, epoch_duration=0.01)
+    m = host.Model(hip_ctx, c)
+    oc = O.default_config()
+    oc.update(width=100, height=100, max_synapses_per_neuron=4, num_epochs=1, epoch_duration=0.01,
+              layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1)
+    ora = O.OracleNet(oc)
+    bad = [k for k in SETUP_ARRAYS if not np.array_equal(m.array(k).view(np.uint8), ora.array(k).view(np.uint8))]
+    assert int((m.array("neuron_type_map") == 1).sum()) == 1000 and int(m.array("starter_map").sum()) == 1000
+    m.close()
+    assert not bad
+
+
+@pytest.mark.parametrize("name", ["conn-lif-ds", "conn-izh-ds"])
+def test_host_model_run_and_xml_output_match_oracle(hip_ctx, name):
+    print(f"[host] run {name}", flush=True)
+    cfg = _host_cfg(name)
+    m = host.Model(hip_ctx, cfg)
+    ocfg = H.load_config(name)
+    ora = O.OracleNet(ocfg)
+    for _ in range(cfg["num_epochs"]):
+        m.run_epoch()
+        ora.run_epoch()
+    for k in ("radiiHistory", "ratesHistory", "burstinessHist", "spikesHistory"):
+        assert np.array_equal(m.array(k).view(np.uint32), ora.array(k).view(np.uint32)), k
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "out.xml")
+        m.save_xml(p)
+        got = open(p).read()
+    m.close()
+    assert got == xmlout.oracle_growth_xml(ora, ocfg)
+
+
+def test_pregrown_growth_update_matches_oracle(hip_ctx):
+    """A 30x30 grid pre-grown to radius 3.1 (in-degree up to ~120), then two short epochs."""
+    cfg = O.default_config()
+    cfg.update(width=30, height=30, max_synapses_per_neuron=160, epoch_duration=0.05, num_epochs=2,
+               layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1, max_firing_rate=1000,
+               noise_mode=1, math_mode=1, sum_mode=1)
+    ora = O.OracleNet(cfg)
+    ora.set_radii(3.1)
+    net = H.gpu_net_from_oracle(hip_ctx, ora, cfg)
+    ora.update_connections()
+    st = net.update_connections()
+    assert st.total_synapses == ora.total_synapses > 900 * 80
+    bad = H.compare_state(ora, net, cfg, "pregrown")
+    for ep in range(2):
+        ora.advance(H.steps_per_epoch(cfg))
+        net.advance(H.steps_per_epoch(cfg))
+        ora.update_connections()
+        net.update_connections()
+        ora.update_history()
+        net.update_history()
+        net.sync()
+        bad += H.compare_state(ora, net, cfg, f"ep{ep}")
+    net.close()
+    assert not bad, bad[:8]
+
+
+def test_pregrown_10k_synapse_count(hip_ctx):
+    """r0 = 8.9 on the 100x100 grid connects every pair closer than 17.8 (area > 0): 8,440,432."""
+    c = host.base_config()
+    c.update(width=100, height=100, max_synapses_per_neuron=1000, num_epochs=1, epoch_duration=10.0,
+             max_firing_rate=1000)
+    m = host.Model(hip_ctx, c)
+    m.set_radii(8.9)
+    st = m.update_connections()
+    g = np.arange(100)
+    d2 = (g[:, None] - g[None, :]) ** 2
+    # count pairs (i != j) with squared distance below 17.8^2 on the grid, by axis separation
+    cnt_axis = np.bincount(np.abs(g[:, None] - g[None, :]).ravel(), minlength=100)
+    want = 0
+    for dx in range(100):
+        for dy in range(100):
+            if 0 < dx * dx + dy * dy < 17.8 ** 2:
+                want += cnt_axis[dx] * cnt_axis[dy]
+    m.close()
+    assert st.total_synapses == want == 8440432
+    assert st.created == want
+
+
+def test_rates_agree_with_reference_noise_stream(hip_ctx):
+    """Deterministic (Philox) GPU run vs the oracle in the reference's own Norm-noise mode: the
+    noise streams differ, so only statistics can agree. 20x20 LIF growth network, 20 s."""
+    c = host.base_config()
+    c.update(width=20, height=20, max_synapses_per_neuron=40, num_epochs=2, epoch_duration=10.0, seed=3)
+    m = host.Model(hip_ctx, c)
+    for _ in range(2):
+        m.run_epoch()
+    gpu_spikes = float(m.array("burstinessHist").sum())
+    m.close()
+    oc = O.default_config()
+    oc.update(width=20, height=20, max_synapses_per_neuron=40, num_epochs=2, epoch_duration=10.0, seed=3,
+              layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1)
+    ora = O.OracleNet(oc)
+    for _ in range(2):
+        ora.run_epoch()
+    ref_spikes = float(ora.array("burstinessHist").sum())
+    print(f"[stats] spikes gpu {gpu_spikes} reference-noise oracle {ref_spikes}", flush=True)
+    assert ref_spikes > 1000
+    assert abs(gpu_spikes - ref_spikes) / ref_spikes < 0.05
