@@ -125,6 +125,11 @@ def load():
         "bghip_download_growth": (ctypes.c_int, [vp, _fp, _fp]),
         "bghip_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(ctypes.c_int64)]),
+        "bghip_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+        "bghip_batch_destroy": (None, [vp]),
+        "bghip_batch_set": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.c_int]),
+        "bghip_batch_advance": (ctypes.c_int, [vp, ctypes.c_uint64]),
+        "bghip_batch_sync": (ctypes.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -139,7 +144,8 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_upload_layout", "bghip_upload_growth", "bghip_advance", "bghip_update_connections",
             "bghip_update_history", "bghip_sync", "bghip_net_error_bits", "bghip_net_step",
             "bghip_download_neurons", "bghip_download_synapses", "bghip_total_synapses",
-            "bghip_download_index_map", "bghip_download_growth", "bghip_kernel_timing")
+            "bghip_download_index_map", "bghip_download_growth", "bghip_kernel_timing", "bghip_batch_create",
+            "bghip_batch_destroy", "bghip_batch_set", "bghip_batch_advance", "bghip_batch_sync")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -334,6 +340,32 @@ class Net:
         self._c(load().bghip_kernel_timing(self.h, 1 if enable else 0, ctypes.byref(avg), ctypes.byref(cnt)),
                 "kernel_timing")
         return avg.value, cnt.value
+
+
+class Batch:
+    """Many networks advanced by shared launches (bghip_batch_*)."""
+
+    def __init__(self, ctx: Context):
+        h = ctypes.c_void_p()
+        ctx.check(load().bghip_batch_create(ctx.h, ctypes.byref(h)), "batch_create")
+        self.ctx, self.h, self.members = ctx, h, []
+
+    def set(self, nets):
+        self.members = list(nets)
+        arr = (ctypes.c_void_p * max(1, len(nets)))(*[n.h.value if isinstance(n.h, ctypes.c_void_p) else n.h
+                                                      for n in nets])
+        self.ctx.check(load().bghip_batch_set(self.h, arr, len(nets)), "batch_set")
+
+    def advance(self, n_steps: int):
+        self.ctx.check(load().bghip_batch_advance(self.h, n_steps), "batch_advance")
+
+    def sync(self):
+        self.ctx.check(load().bghip_batch_sync(self.h), "batch_sync")
+
+    def close(self):
+        if self.h:
+            load().bghip_batch_destroy(self.h)
+            self.h = None
 
 
 def footprint(desc: NetDesc) -> int:

@@ -42,7 +42,18 @@ struct bghip_net {
     hipEvent_t ev0, ev1;
     bool timing;
     int64_t timed_launches;
+    hipEvent_t ext_ev;     // work another stream (a batch) must finish before this net's stream runs
+    bool ext_pending;
 };
+
+// Orders the network's own stream after outstanding batch work on it.
+static void net_begin(bghip_net *n)
+{
+    if (n->ext_pending) {
+        (void)hipStreamWaitEvent(n->stream, n->ext_ev, 0);
+        n->ext_pending = false;
+    }
+}
 
 static int set_err(bghip_ctx *ctx, int code, const char *fmt, ...)
 {
@@ -215,6 +226,7 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
     }
     HIP_TRY(ctx, hipEventCreate(&n->ev0));
     HIP_TRY(ctx, hipEventCreate(&n->ev1));
+    HIP_TRY(ctx, hipEventCreateWithFlags(&n->ext_ev, hipEventDisableTiming));
     // zero everything (empty synapse table, zero rings/counters), then the sentinels
     HIP_TRY(ctx, hipMemsetAsync(n->arena, 0, n->arena_bytes, n->stream));
     HIP_TRY(ctx, hipMemsetAsync(dn.fired_step, 0xFF, sizeof(uint64_t) * dn.N, n->stream));
@@ -232,6 +244,7 @@ extern "C" void bghip_net_destroy(bghip_net *n)
     (void)hipStreamSynchronize(n->stream);
     (void)hipEventDestroy(n->ev0);
     (void)hipEventDestroy(n->ev1);
+    (void)hipEventDestroy(n->ext_ev);
     (void)hipStreamDestroy(n->stream);
     (void)hipFree(n->arena);
     delete n;
@@ -259,6 +272,7 @@ static int d2h(bghip_net *n, T *dst, const T *src, size_t count)
 extern "C" int bghip_upload_neurons(bghip_net *n, const bghip_neurons *h)
 {
     if (!n || !h) return BGHIP_EINVAL;
+    net_begin(n);
     DevNet &dn = n->dn;
     size_t N = dn.N;
     if (!h->Vm || !h->Vthresh || !h->Vreset || !h->I0 || !h->Inoise || !h->C1 || !h->C2 || !h->Trefract)
@@ -302,6 +316,7 @@ static inline uint32_t ring_get(const uint32_t *w, int b) { return (w[b >> 5] >>
 extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint64_t step)
 {
     if (!n || !h) return BGHIP_EINVAL;
+    net_begin(n);
     DevNet &dn = n->dn;
     const int N = dn.N, M = dn.M, R = dn.R, W32 = dn.W32;
     const size_t S = dn.S;
@@ -380,6 +395,7 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
 extern "C" int bghip_upload_layout(bghip_net *n, const float *xloc, const float *yloc, const int32_t *ntype)
 {
     if (!n || !xloc || !yloc || !ntype) return BGHIP_EINVAL;
+    net_begin(n);
     size_t N = n->dn.N;
     for (size_t i = 0; i < N; i++)
         if (ntype[i] != 1 && ntype[i] != 2) return set_err(n->ctx, BGHIP_EINVAL, "neuron type must be INH(1)/EXC(2)");
@@ -393,6 +409,7 @@ extern "C" int bghip_upload_layout(bghip_net *n, const float *xloc, const float 
 extern "C" int bghip_upload_growth(bghip_net *n, const bghip_growth *g, const float *radii, const float *rates)
 {
     if (!n || !g || !radii) return BGHIP_EINVAL;
+    net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     n->growth = *g;
     size_t N = n->dn.N;
@@ -416,6 +433,7 @@ static void flush_pending(bghip_net *n)
 extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
 {
     if (!n) return BGHIP_EINVAL;
+    net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     DevNet &dn = n->dn;
     const int nk = n->desc.neuron_model, sk = n->desc.synapse_model;
@@ -456,6 +474,7 @@ extern "C" uint32_t bghip_net_error_bits(bghip_net *n)
 {
     uint32_t e = 0;
     if (!n) return 0;
+    net_begin(n);
     (void)hipSetDevice(n->ctx->device);
     (void)hipMemcpyAsync(&e, n->dn.err, 4, hipMemcpyDeviceToHost, n->stream);
     (void)hipStreamSynchronize(n->stream);
@@ -465,6 +484,7 @@ extern "C" uint32_t bghip_net_error_bits(bghip_net *n)
 extern "C" int bghip_sync(bghip_net *n)
 {
     if (!n) return BGHIP_EINVAL;
+    net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
     uint32_t e = bghip_net_error_bits(n);
@@ -475,6 +495,7 @@ extern "C" int bghip_sync(bghip_net *n)
 extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
 {
     if (!n) return BGHIP_EINVAL;
+    net_begin(n);
     if (!n->has_layout || !n->has_growth) return set_err(n->ctx, BGHIP_EINVAL, "layout/growth state not uploaded");
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     DevNet &dn = n->dn;
@@ -517,6 +538,7 @@ extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
 extern "C" int bghip_update_history(bghip_net *n)
 {
     if (!n) return BGHIP_EINVAL;
+    net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     unsigned nb = (unsigned)((n->dn.N + BG_BLOCK - 1) / BG_BLOCK);
     hipLaunchKernelGGL(k_update_history, dim3(nb), dim3(BG_BLOCK), 0, n->stream, n->dn, n->growth.minRadius,
@@ -529,6 +551,7 @@ extern "C" int bghip_update_history(bghip_net *n)
 extern "C" int bghip_download_neurons(bghip_net *n, bghip_neurons *h)
 {
     if (!n || !h) return BGHIP_EINVAL;
+    net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     DevNet &dn = n->dn;
     size_t N = dn.N;
@@ -549,6 +572,7 @@ extern "C" int64_t bghip_total_synapses(bghip_net *n) { return n ? (int64_t)n->t
 extern "C" int bghip_download_synapses(bghip_net *n, bghip_synapses *h)
 {
     if (!n || !h) return BGHIP_EINVAL;
+    net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     DevNet &dn = n->dn;
     const int N = dn.N, M = dn.M, R = dn.R, W32 = dn.W32;
@@ -608,6 +632,7 @@ extern "C" int bghip_download_synapses(bghip_net *n, bghip_synapses *h)
 extern "C" int bghip_download_index_map(bghip_net *n, bghip_index_map *h)
 {
     if (!n || !h) return BGHIP_EINVAL;
+    net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     DevNet &dn = n->dn;
     size_t N = dn.N, T = n->total;
@@ -639,10 +664,203 @@ extern "C" int bghip_download_index_map(bghip_net *n, bghip_index_map *h)
 extern "C" int bghip_download_growth(bghip_net *n, float *radii, float *rates)
 {
     if (!n) return BGHIP_EINVAL;
+    net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     CHK(d2h(n, radii, n->dn.radii, n->dn.N));
     CHK(d2h(n, rates, n->dn.rates, n->dn.N));
     return bghip_sync(n);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batched form.
+struct BatchGroup {
+    int nk, sk, w32;
+    bool per_step;
+    std::vector<int> members;      // indices into bghip_batch::nets
+    BatchNet *d_nets = nullptr;    // device table (nn entries)
+    uint32_t *d_base = nullptr;    // neuron prefix (nn + 1)
+    uint64_t *d_sbase = nullptr;   // slot prefix (nn + 1)
+    uint64_t *d_tprev = nullptr;   // flush parameters (nn)
+    int *d_sel = nullptr;
+    uint32_t total = 0;
+    int kmax = 1;
+};
+
+struct bghip_batch {
+    bghip_ctx *ctx;
+    hipStream_t stream;
+    hipEvent_t done;
+    std::vector<bghip_net *> nets;
+    std::vector<BatchGroup> groups;
+};
+
+static void free_groups(bghip_batch *b)
+{
+    for (auto &g : b->groups) {
+        (void)hipFree(g.d_nets);
+        (void)hipFree(g.d_base);
+        (void)hipFree(g.d_sbase);
+        (void)hipFree(g.d_tprev);
+        (void)hipFree(g.d_sel);
+    }
+    b->groups.clear();
+}
+
+extern "C" int bghip_batch_create(bghip_ctx *ctx, bghip_batch **out)
+{
+    if (!ctx || !out) return BGHIP_EINVAL;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    bghip_batch *b = new (std::nothrow) bghip_batch();
+    if (!b) return BGHIP_ENOMEM;
+    b->ctx = ctx;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return set_err(ctx, BGHIP_EDEVICE, "batch stream/event creation failed");
+    }
+    *out = b;
+    return BGHIP_OK;
+}
+
+extern "C" void bghip_batch_destroy(bghip_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->stream);
+    free_groups(b);
+    (void)hipEventDestroy(b->done);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int bghip_batch_set(bghip_batch *b, bghip_net *const *nets, int n)
+{
+    if (!b || n < 0 || (n > 0 && !nets)) return BGHIP_EINVAL;
+    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+    HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
+    free_groups(b);
+    b->nets.assign(nets, nets + n);
+    for (int i = 0; i < n; i++) {
+        bghip_net *x = nets[i];
+        if (!x || x->ctx->device != b->ctx->device) return set_err(b->ctx, BGHIP_EINVAL, "batch member %d invalid", i);
+        const int sk = x->desc.synapse_model;
+        const bool per_step = x->kmax <= 1 || sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
+        BatchGroup *grp = nullptr;
+        for (auto &g : b->groups)
+            if (g.nk == x->desc.neuron_model && g.sk == sk && g.w32 == x->dn.W32 && g.per_step == per_step) grp = &g;
+        if (!grp) {
+            BatchGroup g;
+            g.nk = x->desc.neuron_model;
+            g.sk = sk;
+            g.w32 = x->dn.W32;
+            g.per_step = per_step;
+            g.kmax = BG_KMAX;
+            b->groups.push_back(g);
+            grp = &b->groups.back();
+        }
+        grp->members.push_back(i);
+        grp->kmax = std::min(grp->kmax, x->kmax);
+    }
+    for (auto &g : b->groups) {
+        const int nn = (int)g.members.size();
+        std::vector<uint32_t> base(nn + 1, 0);
+        std::vector<uint64_t> sbase(nn + 1, 0);
+        for (int k = 0; k < nn; k++) {
+            const DevNet &dn = b->nets[g.members[k]]->dn;
+            base[k + 1] = base[k] + (uint32_t)dn.N;
+            sbase[k + 1] = sbase[k] + dn.S;
+        }
+        g.total = base[nn];
+        HIP_TRY(b->ctx, hipMalloc((void **)&g.d_nets, sizeof(BatchNet) * nn));
+        HIP_TRY(b->ctx, hipMalloc((void **)&g.d_base, sizeof(uint32_t) * (nn + 1)));
+        HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sbase, sizeof(uint64_t) * (nn + 1)));
+        HIP_TRY(b->ctx, hipMalloc((void **)&g.d_tprev, sizeof(uint64_t) * nn));
+        HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sel, sizeof(int) * nn));
+        HIP_TRY(b->ctx, hipMemcpy(g.d_base, base.data(), sizeof(uint32_t) * (nn + 1), hipMemcpyHostToDevice));
+        HIP_TRY(b->ctx, hipMemcpy(g.d_sbase, sbase.data(), sizeof(uint64_t) * (nn + 1), hipMemcpyHostToDevice));
+    }
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
+{
+    if (!b) return BGHIP_EINVAL;
+    if (b->nets.empty() || n_steps == 0) return BGHIP_OK;
+    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+    // the batch stream waits for every member's own outstanding work
+    for (bghip_net *x : b->nets) {
+        net_begin(x);
+        HIP_TRY(b->ctx, hipEventRecord(x->ext_ev, x->stream));
+        HIP_TRY(b->ctx, hipStreamWaitEvent(b->stream, x->ext_ev, 0));
+    }
+    for (auto &g : b->groups) {
+        const int nn = (int)g.members.size();
+        std::vector<BatchNet> tab(nn);
+        for (int k = 0; k < nn; k++) {
+            bghip_net *x = b->nets[g.members[k]];
+            tab[k].n = x->dn;
+            tab[k].t0 = x->step;
+            tab[k].tprev0 = x->last_t0;
+            tab[k].sel = x->sel;
+            tab[k].pull = x->pending ? 1 : 0;
+        }
+        HIP_TRY(b->ctx, hipMemcpyAsync(g.d_nets, tab.data(), sizeof(BatchNet) * nn, hipMemcpyHostToDevice, b->stream));
+        // the table copy reads host memory that is reused by the next call: wait for it
+        HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
+        if (g.per_step) {
+            for (uint64_t s = 0; s < n_steps; s++)
+                launch_step_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, g.sk, g.w32, s);
+            for (int k = 0; k < nn; k++) {
+                bghip_net *x = b->nets[g.members[k]];
+                x->step += n_steps;
+                x->pending = false;
+            }
+        } else {
+            const int Kfull = std::max(1, g.kmax);
+            const uint64_t L = (n_steps + (uint64_t)Kfull - 1) / (uint64_t)Kfull;
+            for (uint64_t l = 0; l < L; l++) {
+                const int K = (int)std::min<uint64_t>((uint64_t)Kfull, n_steps - l * (uint64_t)Kfull);
+                launch_fused_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, g.sk, g.w32, g.kmax, K, Kfull,
+                                   (int)l);
+            }
+            std::vector<uint64_t> tprev(nn);
+            std::vector<int> sel(nn);
+            for (int k = 0; k < nn; k++) {
+                bghip_net *x = b->nets[g.members[k]];
+                tprev[k] = x->step + (L - 1) * (uint64_t)Kfull;
+                sel[k] = x->sel ^ (int)(L & 1);
+                x->last_t0 = tprev[k];
+                x->sel = sel[k];
+                x->step += n_steps;
+                x->pending = false;
+            }
+            HIP_TRY(b->ctx, hipMemcpyAsync(g.d_tprev, tprev.data(), sizeof(uint64_t) * nn, hipMemcpyHostToDevice,
+                                           b->stream));
+            HIP_TRY(b->ctx, hipMemcpyAsync(g.d_sel, sel.data(), sizeof(int) * nn, hipMemcpyHostToDevice, b->stream));
+            uint64_t slots = 0;
+            for (int k = 0; k < nn; k++) slots += b->nets[g.members[k]]->dn.S;
+            if (slots) {
+                unsigned blocks = (unsigned)((slots + BG_BLOCK - 1) / BG_BLOCK);
+                hipLaunchKernelGGL(k_flush_batch, dim3(blocks), dim3(BG_BLOCK), 0, b->stream, g.d_nets, g.d_sbase, nn,
+                                   g.d_tprev, g.d_sel);
+            }
+            HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
+        }
+    }
+    HIP_TRY(b->ctx, hipGetLastError());
+    HIP_TRY(b->ctx, hipEventRecord(b->done, b->stream));
+    for (bghip_net *x : b->nets) {
+        HIP_TRY(b->ctx, hipStreamWaitEvent(x->stream, b->done, 0));
+    }
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_batch_sync(bghip_batch *b)
+{
+    if (!b) return BGHIP_EINVAL;
+    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+    HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
+    return BGHIP_OK;
 }
 
 extern "C" int bghip_kernel_timing(bghip_net *n, int enable, double *avg_ms, int64_t *count)

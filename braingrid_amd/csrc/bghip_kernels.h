@@ -12,10 +12,8 @@
 // (Neurons/AllSpikingNeurons.cpp:98-147); the outgoing/incoming ring notifications become pulls in
 // k_synapse_step.
 template <int NK>
-__global__ __launch_bounds__(BG_BLOCK) void k_neuron_step(DevNet n, uint64_t t)
+__device__ __forceinline__ void neuron_step_body(const DevNet &n, int i, uint64_t t)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n.N) return;
     NeuronRegs s;
     neuron_load<NK>(n, i, s);
     const float noise = bg_noise_normal(n.seed, (uint32_t)i, t);
@@ -25,17 +23,22 @@ __global__ __launch_bounds__(BG_BLOCK) void k_neuron_step(DevNet n, uint64_t t)
     if (fired) n.fired_step[i] = t;
 }
 
+template <int NK>
+__global__ __launch_bounds__(BG_BLOCK) void k_neuron_step(DevNet n, uint64_t t)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n.N) return;
+    neuron_step_body<NK>(n, i, t);
+}
+
 // One wavefront per destination neuron: delay-ring insert (pull of sources that fired at t),
 // ring test/clear, STDP learning, PSR update and decay, and the destination's summation in the
 // canonical lane-strided + butterfly order. AllSynapses::advanceSynapses
 // (Synapses/AllSynapses.cpp:325-331), AllSpikingSynapses::advanceSynapse (:297-319),
 // AllSTDPSynapses::advanceSynapse (Synapses/AllSTDPSynapses.cpp:283-421).
 template <int SK, int W32>
-__global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
+__device__ __forceinline__ void synapse_step_body(const DevNet &n, int d, int lane, uint64_t t)
 {
-    const int lane = threadIdx.x & (BG_WAVE - 1);
-    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
-    if (d >= n.N) return;
     const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
     const bool dense = (cnt == hi);
     const bool post = (SK == SK_STDP || SK == SK_DYNSTDP) && n.fired_step[d] == t;
@@ -73,6 +76,15 @@ __global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
     }
     acc = wave_sum(acc);
     if (lane == 0) n.sp[d] = n.sp[d] + acc;
+}
+
+template <int SK, int W32>
+__global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
+{
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
+    if (d >= n.N) return;
+    synapse_step_body<SK, W32>(n, d, lane, t);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -154,14 +166,12 @@ __device__ __forceinline__ void fused_synapse(const DevNet &n, uint64_t slot, ui
 }
 
 template <int NK, int SK, int W32, int KMAX>
-__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_block(DevNet n, uint64_t t0, int K, int pull,
-                                                          uint64_t tprev0, int sel_in)
+__device__ __forceinline__ void fused_block_body(const DevNet &n, int d, uint64_t t0, int K, int pull,
+                                                 uint64_t tprev0, int sel_in)
 {
     __shared__ float s_noise[BG_BLOCK / BG_WAVE][KMAX];
     const int lane = threadIdx.x & (BG_WAVE - 1);
     const int wv = threadIdx.x / BG_WAVE;
-    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
-    if (d >= n.N) return;
     const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
     const bool dense = (cnt == hi);
     const uint32_t *mprev = n.mask[sel_in];
@@ -235,3 +245,78 @@ __global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
         if (mask) n.fired_step[d] = t0 + (uint64_t)(31 - __clz(mask));
     }
 }
+
+template <int NK, int SK, int W32, int KMAX>
+__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_block(
+    DevNet n, uint64_t t0, int K, int pull, uint64_t tprev0, int sel_in)
+{
+    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
+    if (d >= n.N) return;
+    fused_block_body<NK, SK, W32, KMAX>(n, d, t0, K, pull, tprev0, sel_in);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Multi-network forms. One launch advances every network of a batch: work item w belongs to
+// network i with base[i] <= w < base[i+1] (found by a wave-uniform binary search), and each
+// network keeps its own step counter, noise seed and fire-mask buffers, so a network's results do
+// not depend on which other networks share the launch.
+struct BatchNet {
+    DevNet n;
+    uint64_t t0;        // step of the network at the first launch of this batch call
+    uint64_t tprev0;    // start step of its last block before this call
+    int sel;            // mask buffer holding that block's masks
+    int pull;           // 1 if those spikes are still to be inserted into the rings
+};
+
+__device__ __forceinline__ int batch_find(const uint32_t *base, int nn, uint32_t w)
+{
+    int lo = 0, hi = nn;   // base[lo] <= w < base[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (base[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int NK, int SK, int W32, int KMAX>
+__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_batch(
+    const BatchNet *nets, const uint32_t *wbase, int nn, int K, int Kfull, int launch)
+{
+    // wave-uniform by construction (blockDim is a multiple of 64); readfirstlane lets the
+    // compiler keep the network's descriptor in scalar registers
+    const uint32_t w = __builtin_amdgcn_readfirstlane(
+        (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
+    if (w >= wbase[nn]) return;
+    const int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, w));
+    const BatchNet &b = nets[i];
+    const uint64_t t0 = b.t0 + (uint64_t)launch * (uint64_t)Kfull;
+    const int sel = b.sel ^ (launch & 1);
+    const int pull = launch > 0 ? 1 : b.pull;
+    const uint64_t tprev0 = launch > 0 ? t0 - (uint64_t)Kfull : b.tprev0;
+    fused_block_body<NK, SK, W32, KMAX>(b.n, (int)(w - wbase[i]), t0, K, pull, tprev0, sel);
+}
+
+template <int NK>
+__global__ __launch_bounds__(BG_BLOCK) void k_neuron_batch(const BatchNet *nets, const uint32_t *nbase, int nn,
+                                                           uint64_t step)
+{
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nbase[nn]) return;
+    const int i = batch_find(nbase, nn, w);
+    const BatchNet &b = nets[i];
+    neuron_step_body<NK>(b.n, (int)(w - nbase[i]), b.t0 + step);
+}
+
+template <int SK, int W32>
+__global__ __launch_bounds__(BG_BLOCK) void k_synapse_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
+                                                            uint64_t step)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(
+        (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
+    if (w >= wbase[nn]) return;
+    const int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, w));
+    const BatchNet &b = nets[i];
+    synapse_step_body<SK, W32>(b.n, (int)(w - wbase[i]), threadIdx.x & (BG_WAVE - 1), b.t0 + step);
+}
+

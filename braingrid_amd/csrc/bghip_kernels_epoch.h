@@ -339,3 +339,34 @@ __global__ __launch_bounds__(BG_BLOCK) void k_sort_outgoing(const uint32_t *out_
     }
     for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) out_map[b + i] = v[i];
 }
+
+// k_flush_block over every slot of every network of a batch (sbase: 64-bit slot prefix).
+__global__ __launch_bounds__(BG_BLOCK) void k_flush_batch(const BatchNet *nets, const uint64_t *sbase, int nn,
+                                                          const uint64_t *tprev, const int *sel)
+{
+    const uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (w >= sbase[nn]) return;
+    int lo = 0, hi = nn;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (sbase[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    const DevNet &n = nets[lo].n;
+    const uint64_t slot = w - sbase[lo];
+    const uint32_t meta = n.meta[slot];
+    if (!(meta & BG_META_IN_USE)) return;
+    uint32_t fm = n.mask[sel[lo]][meta & BG_META_SRC_MASK];
+    if (!fm) return;
+    const uint32_t delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
+    const uint32_t Rm = (uint32_t)n.R - 1;
+    while (fm) {
+        const int b0 = __ffs(fm) - 1;
+        fm &= fm - 1;
+        const uint32_t bit = (uint32_t)((tprev[lo] + (uint64_t)b0 + delay) & Rm);
+        uint32_t *word = &n.ring[(uint64_t)(bit >> 5) * n.S + slot];
+        const uint32_t m = 1u << (bit & 31);
+        if (*word & m) dev_error(n, 1u);
+        *word |= m;
+    }
+}

@@ -176,6 +176,22 @@ int64_t bghip_total_synapses(bghip_net *net);
 int bghip_download_index_map(bghip_net *net, bghip_index_map *h);
 int bghip_download_growth(bghip_net *net, float *radii, float *rates);
 
+/* ---- batched form ------------------------------------------------------------------------ */
+/* A batch advances many independent networks with shared launches: Spiking/DS networks through
+ * the multi-network fused block kernel, STDP networks through multi-network per-step kernels.
+ * Networks keep their own step counters, noise seeds and state; a network's results are the same
+ * alone and in any batch. Member networks may be of different sizes and models. No counterpart in
+ * the reference, which runs one network per process (Core/GPUSpikingModel.cu). */
+typedef struct bghip_batch bghip_batch;
+int bghip_batch_create(bghip_ctx *ctx, bghip_batch **out);
+void bghip_batch_destroy(bghip_batch *b);
+/* Replace the member list (nets stay owned by the caller; a net may be in one batch at a time). */
+int bghip_batch_set(bghip_batch *b, bghip_net *const *nets, int n);
+/* Every member advances n_steps from its own current step (stream-ordered on the batch's stream;
+ * later calls on a member network wait for it). */
+int bghip_batch_advance(bghip_batch *b, uint64_t n_steps);
+int bghip_batch_sync(bghip_batch *b);
+
 /* ---- timing (measurement support) -------------------------------------------------------- */
 /* Average duration in milliseconds of the engine's fused step launches recorded with HIP events
  * on the network's stream since the last reset; count receives the number of launches timed. */
