@@ -390,8 +390,43 @@ static inline BG_HD float bg_noise_normal(uint64_t seed, uint32_t neuron, uint64
     return (float)(r * bg_cos2pi(u2));
 }
 
-/* U[0,1) double with 32 random bits, for the Poisson input stream. */
-static inline BG_HD double bg_uniform01(uint32_t w) { return (double)w * 0x1p-32; }
+/* Poisson stimulus input: the next inter-spike interval in steps, HostSInputPoisson::inputStimulus
+ * (Inputs/HostSInputPoisson.cpp:75-84) with its MTRand draws replaced by a counter-based stream.
+ * Draw j of (neuron, step) is word j mod 4 of Philox block (neuron, step, POISSON | (j / 4) << 1),
+ * mapped to a float in (0, 1] (24 bits, never 0, so the log is finite). The reference's
+ * rejection of intervals inside the refractoriness, `while (u <= exp(-isi^2 / 32))`, is bounded
+ * at BG_POISSON_MAX_REJECT redraws; the loop is left after about 1.06 draws on average at 10 Hz,
+ * so the bound only makes the worst case finite. lambda_ms is the mean interval in ms. */
+#define BG_POISSON_MAX_REJECT 64
+
+typedef struct bg_poisson_stream {
+    uint64_t seed, step;
+    uint32_t neuron, block, used;
+    uint32_t w[4];
+} bg_poisson_stream;
+
+static inline BG_HD float bg_poisson_next(bg_poisson_stream *s)
+{
+    if (s->used == 4) {
+        bg_philox_draw(s->seed, s->neuron, s->step, BG_STREAM_POISSON | (s->block << 1), s->w);
+        s->block++;
+        s->used = 0;
+    }
+    return (float)((s->w[s->used++] >> 8) + 1u) * 0x1p-24f;
+}
+
+static inline BG_HD int bg_poisson_isi_steps(uint64_t seed, uint32_t neuron, uint64_t step, float lambda_ms,
+                                             float dt)
+{
+    bg_poisson_stream s;
+    s.seed = seed; s.step = step; s.neuron = neuron; s.block = 0; s.used = 4;
+    float isi = -lambda_ms * (float)bg_log((double)bg_poisson_next(&s));
+    for (int a = 0; a < BG_POISSON_MAX_REJECT; a++) {
+        if (!(bg_poisson_next(&s) <= bg_expf(-(isi * isi) / 32.0f))) break;
+        isi = -lambda_ms * (float)bg_log((double)bg_poisson_next(&s));
+    }
+    return (int)((double)((isi / 1000.0f) / dt) + 0.5);
+}
 
 #ifdef __cplusplus
 }

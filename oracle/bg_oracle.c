@@ -129,7 +129,19 @@ struct bgo_net {
     float *radiiHistory, *ratesHistory, *burst, *spikesHist;
     int64_t n_burst, n_spikesHist;
     int32_t *scratch_slot;      /* N, -1 */
+    /* stimulus input: one DS input synapse per neuron (Inputs/SInputPoisson.cpp:124-146) */
+    uint8_t *in_mask;
+    int32_t *in_nisi, *in_type, *in_total_delay, *in_delayIdx, *in_shift;
+    float *in_W, *in_psr, *in_decay, *in_r, *in_u, *in_U, *in_D, *in_F, *in_values;
+    uint64_t *in_lastSpike;
+    uint32_t *in_ring;
+    float in_lambda;            /* mean inter-spike interval in ms */
+    int32_t in_dur, in_cycle, in_steps_in_cycle;
 };
+
+static int input_construct(bgo_net *n);
+static void input_init(bgo_net *n);
+static void input_stimulus(bgo_net *n);
 
 static int fail(bgo_net *n, const char *msg)
 {
@@ -264,7 +276,9 @@ void bgo_destroy(bgo_net *n)
                     n->Aneg, n->STDPgap, n->tauspost, n->tauspre, n->taupos, n->tauneg, n->Wex, n->mupos,
                     n->muneg, n->useFD, n->syn_counts, n->in_map, n->in_begin, n->in_count, n->out_map,
                     n->out_begin, n->out_count, n->radii, n->rates, n->radiiHistory, n->ratesHistory,
-                    n->burst, n->spikesHist, n->scratch_slot};
+                    n->burst, n->spikesHist, n->scratch_slot, n->in_mask, n->in_nisi, n->in_type,
+                    n->in_total_delay, n->in_delayIdx, n->in_shift, n->in_W, n->in_psr, n->in_decay, n->in_r,
+                    n->in_u, n->in_U, n->in_D, n->in_F, n->in_values, n->in_lastSpike, n->in_ring};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
     free(n);
 }
@@ -315,9 +329,20 @@ bgo_net *bgo_create(const bgo_params *p, char *err, int errlen)
     ALLOC(n->burst, n->n_burst); ALLOC(n->spikesHist, n->n_spikesHist);
     ALLOC(n->scratch_slot, N);
     for (int i = 0; i < N; i++) n->scratch_slot[i] = -1;
+    ALLOC(n->in_mask, N); ALLOC(n->in_nisi, N); ALLOC(n->in_type, N); ALLOC(n->in_total_delay, N);
+    ALLOC(n->in_delayIdx, N); ALLOC(n->in_shift, N); ALLOC(n->in_W, N); ALLOC(n->in_psr, N);
+    ALLOC(n->in_decay, N); ALLOC(n->in_r, N); ALLOC(n->in_u, N); ALLOC(n->in_U, N); ALLOC(n->in_D, N);
+    ALLOC(n->in_F, N); ALLOC(n->in_values, N); ALLOC(n->in_lastSpike, N); ALLOC(n->in_ring, N);
 
-    /* Model::setupSim order (Core/Model.cpp:136-158) */
+    /* The stimulus object is constructed before Simulator::setup (Core/BGDriver.cpp:76-87), so
+     * SInputRegular's asynchronous shifts are the first draws of the global rng. */
     mt_seed(&n->rng, 1u);
+    if (input_construct(n) < 0) {
+        if (err) snprintf(err, errlen, "%s", n->err);
+        bgo_destroy(n);
+        return NULL;
+    }
+    /* Model::setupSim order (Core/Model.cpp:136-158) */
     layout_setup(n);
     neurons_create(n);
     for (int i = 0; i < N; i++) {   /* ConnGrowth::setupConnections (Connections/ConnGrowth.cpp:71-85) */
@@ -327,6 +352,7 @@ bgo_net *bgo_create(const bgo_params *p, char *err, int errlen)
         n->ratesHistory[i] = 0.0f;
     }
     norm_seed(&n->norm, p->seed);
+    input_init(n);   /* Simulator::setup (Core/Simulator.cpp:40-43) */
     return n;
 oom:
     if (err) snprintf(err, errlen, "out of memory");
@@ -603,6 +629,125 @@ static void advance_synapses(bgo_net *n)
     }
 }
 
+/* ------------------------------------------------------------------ stimulus input -------- */
+/* SInputPoisson / SInputRegular constructors (Inputs/SInputPoisson.cpp:24-108,
+ * Inputs/SInputRegular.cpp:26-140). Called before the model's setup draws. */
+static int input_construct(bgo_net *n)
+{
+    const bgo_params *p = &n->p;
+    const int N = n->N;
+    const float dt = p->dt;
+    if (p->input_kind == BGO_INPUT_POISSON) {
+        float fr = p->input_fr_mean / 1000;   /* per ms */
+        n->in_lambda = 1 / fr;
+        if (p->n_input_mask < 0) {
+            for (int i = 0; i < N; i++) n->in_mask[i] = 1;
+        } else {
+            for (int k = 0; k < p->n_input_mask; k++) {
+                int i = p->input_mask[k];
+                if (i < 0 || i >= N) return fail(n, "input mask index out of range");
+                n->in_mask[i] = 1;
+            }
+        }
+    } else if (p->input_kind == BGO_INPUT_REGULAR) {
+        n->in_dur = (int)((double)(p->input_duration / dt) + 0.5);
+        int itv = (int)((double)(p->input_interval / dt) + 0.5);
+        n->in_cycle = n->in_dur + itv;
+        n->in_steps_in_cycle = 0;
+        if (n->in_cycle <= 0) return fail(n, "regular input cycle is empty");
+        for (int i = 0; i < p->height; i++)
+            for (int j = 0; j < p->width; j++)
+                n->in_values[i * p->width + j] = p->input_values[(i % 10) * 10 + j % 10];
+        if (p->input_sync == BGO_SYNC_NO) {
+            for (int i = 0; i < p->height; i++)
+                for (int j = 0; j < p->width; j++)
+                    n->in_shift[i * p->width + j] = (int)mt_inrange(&n->rng, 0.0f, (float)(n->in_cycle - 1));
+        } else if (p->input_sync == BGO_SYNC_WAVE) {
+            for (int i = 0; i < p->height; i++)
+                for (int j = 0; j < p->width; j++)
+                    n->in_shift[i * p->width + j] = (n->in_cycle / p->width) * j;
+        }
+    } else if (p->input_kind != BGO_INPUT_NONE) {
+        return fail(n, "unknown input kind");
+    }
+    return 0;
+}
+
+/* SInputPoisson::init (Inputs/SInputPoisson.cpp:124-146): one AllDSSynapses input synapse per
+ * neuron, type EI into inhibitory and EE into excitatory neurons, W = weight * 1e-8. The
+ * reference indexes this one-slot-per-neuron object with maxSynapsesPerNeuron * neuron, which is
+ * outside its arrays whenever maxSynapsesPerNeuron > 1; slot `neuron` is the intended one. */
+static void input_init(bgo_net *n)
+{
+    if (n->p.input_kind != BGO_INPUT_POISSON) return;
+    static const float tau_tab[4] = {(float)6e-3, (float)6e-3, (float)3e-3, (float)3e-3};
+    static const float delay_tab[4] = {(float)0.8e-3, (float)0.8e-3, (float)0.8e-3, (float)1.5e-3};
+    static const float U_tab[4] = {(float)0.32, (float)0.25, (float)0.05, (float)0.5};
+    static const float D_tab[4] = {(float)0.144, (float)0.7, (float)0.125, (float)1.1};
+    static const float F_tab[4] = {(float)0.06, (float)0.02, (float)1.2, (float)0.05};
+    const float dt = n->p.dt;
+    const float unit = (float)1.0e-8;   /* AllSynapses::SYNAPSE_STRENGTH_ADJUSTMENT */
+    for (int i = 0; i < n->N; i++) {
+        int t = n->ntype[i] == INH ? EI : EE;
+        n->in_type[i] = t;
+        n->in_W[i] = n->p.input_weight * unit;
+        n->in_psr[i] = 0.0f;
+        n->in_ring[i] = 0;
+        n->in_delayIdx[i] = 0;
+        n->in_total_delay[i] = (int)(delay_tab[t] / dt) + 1;
+        n->in_decay[i] = expf(-dt / tau_tab[t]);
+        n->in_r[i] = 1.0f;
+        n->in_u[i] = (float)0.4;
+        n->in_lastSpike[i] = ULONG_MAX64;
+        n->in_U[i] = U_tab[t]; n->in_D[i] = D_tab[t]; n->in_F[i] = F_tab[t];
+    }
+}
+
+/* HostSInputPoisson::inputStimulus (Inputs/HostSInputPoisson.cpp:60-94) and
+ * HostSInputRegular::inputStimulus (Inputs/HostSInputRegular.cpp:59-80), once per step before the
+ * model advance. */
+static void input_stimulus(bgo_net *n)
+{
+    const float dt = n->p.dt;
+    if (n->p.input_kind == BGO_INPUT_POISSON) {
+        const float lambda = n->in_lambda;
+        for (int i = 0; i < n->N; i++) {
+            if (!n->in_mask[i]) continue;
+            if (--n->in_nisi[i] <= 0) {
+                if (ring_insert(&n->in_ring[i], n->in_delayIdx[i], n->in_total_delay[i], 32) < 0) {
+                    fail(n, "input delay-queue collision");
+                    return;
+                }
+                if (n->p.noise_mode == 0) {
+                    float isi = -lambda * logf(mt_inrange(&n->rng, 0.0f, 1.0f));
+                    while (mt_inrange(&n->rng, 0.0f, 1.0f) <= expf(-(isi * isi) / 32))
+                        isi = -lambda * logf(mt_inrange(&n->rng, 0.0f, 1.0f));
+                    n->in_nisi[i] = (int)((double)((isi / 1000) / dt) + 0.5);
+                } else {
+                    n->in_nisi[i] = bg_poisson_isi_steps((uint64_t)n->p.seed, (uint32_t)i, n->step, lambda, dt);
+                }
+            }
+            /* AllSpikingSynapses::advanceSynapse with AllDSSynapses::changePSR */
+            if (ring_test_clear(&n->in_ring[i], &n->in_delayIdx[i], 32)) {
+                if (n->in_lastSpike[i] != ULONG_MAX64) {
+                    float isi = (float)(n->step - n->in_lastSpike[i]) * dt;
+                    n->in_r[i] = 1 + (n->in_r[i] * (1 - n->in_u[i]) - 1) * n->f_expf(-isi / n->in_D[i]);
+                    n->in_u[i] = n->in_U[i] + n->in_u[i] * (1 - n->in_U[i]) * n->f_expf(-isi / n->in_F[i]);
+                }
+                n->in_psr[i] += ((n->in_W[i] / n->in_decay[i]) * n->in_u[i]) * n->in_r[i];
+                n->in_lastSpike[i] = n->step;
+            }
+            n->in_psr[i] *= n->in_decay[i];
+            n->sp[i] += n->in_psr[i];
+        }
+    } else if (n->p.input_kind == BGO_INPUT_REGULAR) {
+        const int c = n->in_steps_in_cycle;
+        for (int i = n->N - 1; i >= 0; --i)
+            if (c >= n->in_shift[i] && c < (n->in_shift[i] + n->in_dur) % n->in_cycle) n->sp[i] += n->in_values[i];
+        n->in_steps_in_cycle = (c + 1) % n->in_cycle;
+    }
+}
+
 /* ------------------------------------------------------------------ neurons --------------- */
 static void fire_record(bgo_net *n, int i)
 {   /* AllSpikingNeurons::fire (Neurons/AllSpikingNeurons.cpp:155-173) */
@@ -693,6 +838,7 @@ int bgo_advance(bgo_net *n, int64_t n_steps)
 {
     for (int64_t s = 0; s < n_steps; s++) {
         if (n->has_error) return -1;
+        input_stimulus(n);                        /* Simulator::advanceUntilGrowth (:177-179) */
         if (advance_neurons(n) < 0) return -1;   /* SingleThreadedSpikingModel::advance (:39-43) */
         advance_synapses(n);
         n->step++;
@@ -844,6 +990,11 @@ void bgo_set_radii(bgo_net *n, float r)
 
 int64_t bgo_step(const bgo_net *n) { return (int64_t)n->step; }
 int64_t bgo_total_synapses(const bgo_net *n) { return n->total; }
+int32_t bgo_poisson_isi(uint64_t seed, uint32_t neuron, uint64_t step, float lambda_ms, float dt)
+{
+    return bg_poisson_isi_steps(seed, neuron, step, lambda_ms, dt);
+}
+
 const char *bgo_last_error(const bgo_net *n) { return n->has_error ? n->err : ""; }
 
 void *bgo_array(bgo_net *n, const char *name, char *dtype, int64_t *count)
@@ -878,6 +1029,13 @@ void *bgo_array(bgo_net *n, const char *name, char *dtype, int64_t *count)
     A("radii", n->radii, 'f', N) A("rates", n->rates, 'f', N)
     A("radiiHistory", n->radiiHistory, 'f', (int64_t)((n->p.num_epochs > 0 ? n->p.num_epochs : 1) + 1) * N)
     A("ratesHistory", n->ratesHistory, 'f', (int64_t)((n->p.num_epochs > 0 ? n->p.num_epochs : 1) + 1) * N)
+    A("in_mask", n->in_mask, 'b', N) A("in_nISIs", n->in_nisi, 'i', N) A("in_type", n->in_type, 'i', N)
+    A("in_total_delay", n->in_total_delay, 'i', N) A("in_delayIdx", n->in_delayIdx, 'i', N)
+    A("in_shift", n->in_shift, 'i', N) A("in_W", n->in_W, 'f', N) A("in_psr", n->in_psr, 'f', N)
+    A("in_decay", n->in_decay, 'f', N) A("in_r", n->in_r, 'f', N) A("in_u", n->in_u, 'f', N)
+    A("in_U", n->in_U, 'f', N) A("in_D", n->in_D, 'f', N) A("in_F", n->in_F, 'f', N)
+    A("in_values", n->in_values, 'f', N) A("in_lastSpike", n->in_lastSpike, 'q', N)
+    A("in_delayQueue", n->in_ring, 'I', N) A("in_steps_in_cycle", &n->in_steps_in_cycle, 'i', 1)
     A("burstinessHist", n->burst, 'f', n->n_burst) A("spikesHistory", n->spikesHist, 'f', n->n_spikesHist)
 #undef A
     return NULL;

@@ -15,6 +15,8 @@
  *              1 = own   : the bg_* functions of include/bg_detmath.h (bit-identical to the GPU)
  *   sum_mode   0 = sequential per-destination summation in incoming-map order (reference)
  *              1 = wave64: the fixed lane-strided 64-lane tree the HIP kernels use
+ * The Poisson stimulus input draws from the global MTRand stream in Norm mode, as the reference's
+ * HostSInputPoisson does, and from bg_poisson_isi_steps() in Philox mode.
  * Parity pinning of every function is listed in the header of tests/test_oracle_vs_reference.py.
  */
 #ifndef BG_ORACLE_H
@@ -58,7 +60,18 @@ typedef struct bgo_params {
     int32_t stdp_use_froemke_dan;
     /* synapse delay overrides per type II, IE, EI, EE in seconds (0 = reference table) */
     float delay_override[4];
+    /* stimulus input (Inputs/FSInput.cpp:40-97): BGO_INPUT_NONE / _POISSON / _REGULAR */
+    int32_t input_kind;
+    float input_fr_mean, input_weight;    /* SInputPoisson IntParams: fr_mean (Hz), weight (x 1e-8) */
+    const int32_t *input_mask;            /* SInputPoisson masked neuron list ... */
+    int32_t n_input_mask;                 /* ... its length; < 0: every neuron receives input */
+    float input_duration, input_interval; /* SInputRegular IntParams (seconds) */
+    int32_t input_sync;                   /* SInputRegular sync: 0 "yes", 1 "no", 2 "wave" */
+    float input_values[100];              /* SInputRegular <Values><I>: the 10x10 table */
 } bgo_params;
+
+enum { BGO_INPUT_NONE = 0, BGO_INPUT_POISSON = 1, BGO_INPUT_REGULAR = 2 };
+enum { BGO_SYNC_YES = 0, BGO_SYNC_NO = 1, BGO_SYNC_WAVE = 2 };
 
 typedef struct bgo_net bgo_net;
 
@@ -95,6 +108,8 @@ void bgo_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float bgo_noise(uint64_t seed, uint32_t neuron, uint64_t step);
 float bgo_norm_stream(uint32_t seed, int n, float *out);   /* first n Norm draws */
 double bgo_detmath(int fn, double x, double y);            /* 0 exp 1 log 2 sin 3 cos 4 acos 5 pow 6 cos2pi 7 expf */
+/* Poisson input inter-spike interval in steps drawn from the counter-based stream. */
+int32_t bgo_poisson_isi(uint64_t seed, uint32_t neuron, uint64_t step, float lambda_ms, float dt);
 
 #ifdef __cplusplus
 }

@@ -56,7 +56,15 @@ class Params(ctypes.Structure):
         ("stdp_gap", ctypes.c_float), ("stdp_Wex", ctypes.c_float), ("stdp_mupos", ctypes.c_float),
         ("stdp_muneg", ctypes.c_float), ("stdp_use_froemke_dan", ctypes.c_int32),
         ("delay_override", ctypes.c_float * 4),
+        ("input_kind", ctypes.c_int32), ("input_fr_mean", ctypes.c_float), ("input_weight", ctypes.c_float),
+        ("input_mask", ctypes.POINTER(ctypes.c_int32)), ("n_input_mask", ctypes.c_int32),
+        ("input_duration", ctypes.c_float), ("input_interval", ctypes.c_float), ("input_sync", ctypes.c_int32),
+        ("input_values", ctypes.c_float * 100),
     ]
+
+
+INPUT_NONE, INPUT_POISSON, INPUT_REGULAR = 0, 1, 2
+SYNC = {"yes": 0, "no": 1, "wave": 2}
 
 
 _lib = None
@@ -88,6 +96,9 @@ def lib():
         L.bgo_total_synapses.restype = ctypes.c_int64
         L.bgo_last_error.argtypes = [ctypes.c_void_p]
         L.bgo_last_error.restype = ctypes.c_char_p
+        L.bgo_poisson_isi.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float,
+                                      ctypes.c_float]
+        L.bgo_poisson_isi.restype = ctypes.c_int32
         _lib = L
     return _lib
 
@@ -108,7 +119,7 @@ def default_config() -> dict:
         excA=(0.02, 0.02), excB=(0.2, 0.2), excC=(-65, -50), excD=(2, 8),
         inhA=(0.02, 0.1), inhB=(0.2, 0.25), inhC=(-65, -65), inhD=(2, 2),
         epsilon=0.6, beta=0.1, rho=0.0001, target_rate=1.9, min_radius=0.1, start_radius=0.4,
-        ring_bits=32, noise_mode=0, math_mode=0, sum_mode=0, stdp=None, delay_override=None,
+        ring_bits=32, noise_mode=0, math_mode=0, sum_mode=0, stdp=None, delay_override=None, input=None,
     )
 
 
@@ -175,6 +186,33 @@ def parse_config_xml(path: str, base_dir: str) -> dict:
     return cfg
 
 
+def parse_input_xml(path: str, base_dir: str) -> dict:
+    """Parse a stimulus input file (Inputs/FSInput.cpp:40-97): SInputPoisson
+    (Inputs/SInputPoisson.cpp:24-108) or SInputRegular (Inputs/SInputRegular.cpp:26-110)."""
+    root = ET.parse(path).getroot()
+    parms = root if root.tag == "InputParams" else root.find("InputParams")
+    name = parms.find("IMethod").get("name")
+    ip = parms.find("IntParams")
+    if name == "SInputPoisson":
+        inp = dict(kind="poisson", fr_mean=float(ip.get("fr_mean")), weight=float(ip.get("weight")), mask=None)
+        masks = parms.find("Masks")
+        if masks is not None:
+            for node in masks:
+                if node.tag == "M":
+                    inp["mask"] = [int(float(v)) for v in _text(node).split()]
+                elif node.tag == "LayoutFiles" and node.get("maskNListFileName"):
+                    inp["mask"] = [int(float(v)) for v in
+                                   _text(ET.parse(os.path.join(base_dir, node.get("maskNListFileName"))).getroot()).split()]
+        return inp
+    if name == "SInputRegular":
+        vals = [float(v) for v in _text(parms.find("Values/I")).split()]
+        if len(vals) != 100:
+            raise ValueError("SInputRegular needs a 10x10 <I> table")
+        return dict(kind="regular", duration=float(ip.get("duration")), interval=float(ip.get("interval")),
+                    sync=ip.get("sync"), values=vals)
+    raise ValueError(f"unknown input method {name}")
+
+
 def make_params(cfg: dict) -> tuple[Params, list]:
     """Build a Params struct; returns (params, keepalive) where keepalive holds the list buffers."""
     p = Params()
@@ -213,7 +251,22 @@ def make_params(cfg: dict) -> tuple[Params, list]:
     if cfg.get("delay_override"):
         for k in range(4):
             p.delay_override[k] = cfg["delay_override"][k]
-    return p, [inh, act]
+    keep = [inh, act]
+    inp = cfg.get("input")
+    if inp and inp["kind"] == "poisson":
+        p.input_kind, p.input_fr_mean, p.input_weight = INPUT_POISSON, inp["fr_mean"], inp["weight"]
+        if inp.get("mask") is None:
+            p.n_input_mask = -1
+        else:
+            m = (ctypes.c_int32 * max(1, len(inp["mask"])))(*inp["mask"])
+            keep.append(m)
+            p.input_mask, p.n_input_mask = ctypes.cast(m, ctypes.POINTER(ctypes.c_int32)), len(inp["mask"])
+    elif inp and inp["kind"] == "regular":
+        p.input_kind, p.input_duration, p.input_interval = INPUT_REGULAR, inp["duration"], inp["interval"]
+        p.input_sync = SYNC.get(inp["sync"], -1)
+        for k, v in enumerate(inp["values"]):
+            p.input_values[k] = v
+    return p, keep
 
 
 class OracleNet:
@@ -275,6 +328,11 @@ class OracleNet:
         return lib().bgo_total_synapses(self.h)
 
     def array(self, name: str) -> np.ndarray:
+        """A copy of the named state array."""
+        return self.view(name).copy()
+
+    def view(self, name: str) -> np.ndarray:
+        """The named state array in place (writable; valid until the array is reallocated)."""
         dt = ctypes.c_char()
         cnt = ctypes.c_int64()
         ptr = lib().bgo_array(self.h, name.encode(), ctypes.byref(dt), ctypes.byref(cnt))
@@ -285,7 +343,7 @@ class OracleNet:
         if n == 0:
             return np.zeros(0, t)
         buf = (ctypes.c_char * (n * np.dtype(t).itemsize)).from_address(ptr)
-        return np.frombuffer(buf, dtype=t).copy()
+        return np.frombuffer(buf, dtype=t)
 
 
 def read_dump(path: str) -> list[tuple[str, int, dict]]:

@@ -11,7 +11,7 @@
 // translation unit read access to the members it dumps.
 //
 // usage: bgref_dump <config.xml> <dump.bin> [--max-epochs K] [--no-history]
-//                   [--dynamic-layout FRAC_EXC FRAC_STARTER]
+//                   [--dynamic-layout FRAC_EXC FRAC_STARTER] [--stim STIMULUS.xml]
 // --dynamic-layout replaces the configured layout object by a DynamicLayout with the given
 // fractions before setup: the reference's XML visitor for DynamicLayout rejects its own
 // parameters (Layouts/DynamicLayout.cpp:29-77 returns false after parsing them), so this is the
@@ -39,6 +39,8 @@
 #include "Layout.h"
 #include "DynamicLayout.h"
 #include "SynapseIndexMap.h"
+#include "SInputPoisson.h"
+#include "SInputRegular.h"
 
 bool LoadAllParameters(SimulationInfo *simInfo);
 
@@ -69,12 +71,38 @@ static void marker(const char *tag, int epoch)
     rec(tag, 'i', &e, 1, 4);
 }
 
+// Stimulus input state: SInputPoisson's interval counters and its one-slot-per-neuron DS input
+// synapses (Inputs/SInputPoisson.h), SInputRegular's cycle position (Inputs/SInputRegular.h).
+static void dump_input(SimulationInfo *si)
+{
+    int N = si->totalNeurons;
+    SInputPoisson *sp = dynamic_cast<SInputPoisson *>(si->pInput);
+    if (sp && sp->m_synapses) {
+        AllDSSynapses *ds = dynamic_cast<AllDSSynapses *>(sp->m_synapses);
+        reci("in_nISIs", sp->nISIs, N);
+        recf("in_psr", ds->psr, N);
+        recf("in_r", ds->r, N);
+        recf("in_u", ds->u, N);
+        recf("in_W", ds->W, N);
+        recq("in_lastSpike", (const unsigned long long *)ds->lastSpike, N);
+        recu("in_delayQueue", ds->delayQueue, N);
+        reci("in_delayIdx", ds->delayIdx, N);
+    }
+    SInputRegular *sr = dynamic_cast<SInputRegular *>(si->pInput);
+    if (sr && sr->values) {
+        recf("in_values", sr->values, N);
+        reci("in_shift", sr->nShiftValues, N);
+        reci("in_steps_in_cycle", &sr->nStepsInCycle, 1);
+    }
+}
+
 static void dump_state(SimulationInfo *si, Model *model, bool with_history)
 {
     int N = si->totalNeurons;
     int max_spikes = (int)(si->epochDuration * si->maxFiringRate);
     unsigned long long step = g_simulationStep;
     recq("g_simulationStep", &step, 1);
+    dump_input(si);
 
     AllIFNeurons *ifn = dynamic_cast<AllIFNeurons *>(model->m_neurons);
     if (ifn) {
@@ -162,6 +190,7 @@ int main(int argc, char **argv)
     bool with_history = true;
     bool dyn = false;
     float fexc = 0, fst = 0;
+    const char *stim = NULL;
     for (int i = 3; i < argc; i++) {
         if (!strcmp(argv[i], "--max-epochs") && i + 1 < argc) max_epochs = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--no-history")) with_history = false;
@@ -169,6 +198,8 @@ int main(int argc, char **argv)
             dyn = true;
             fexc = (float)atof(argv[++i]);
             fst = (float)atof(argv[++i]);
+        } else if (!strcmp(argv[i], "--stim") && i + 1 < argc) {
+            stim = argv[++i];
         }
     }
     SimulationInfo *si = new SimulationInfo();
@@ -177,6 +208,7 @@ int main(int argc, char **argv)
     si->stateOutputFileName = xml_out;
     if (!LoadAllParameters(si)) return 1;
     si->simRecorder = si->model->getConnections()->createRecorder(si);
+    if (stim) si->stimulusInputFileName = stim;   // the -s option (Core/BGDriver.cpp:320)
     si->pInput = FSInput::get()->CreateInstance(si);
 
     g_out = fopen(argv[2], "wb");

@@ -93,6 +93,8 @@ NLISTS = {
     "NList/I5x5.xml": ("I", [7, 18]),
     "NList/A6x6.xml": ("A", [2, 14, 22, 33]),
     "NList/I6x6.xml": ("I", [5, 19, 30]),
+    "NList/A10x10.xml": ("A", [4, 15, 26, 37, 48, 51, 62, 73, 84, 95]),
+    "NList/I10x10.xml": ("I", [9, 18, 27, 33, 45, 56, 60, 71, 82, 90]),
 }
 
 CONFIGS = {
@@ -119,15 +121,77 @@ CONFIGS = {
 }
 
 
+# Stimulus inputs (Inputs/SInputPoisson.cpp, Inputs/SInputRegular.cpp). The reference's CPU Poisson
+# input indexes its one-slot-per-neuron synapse object with maxSynapsesPerNeuron * neuron, which
+# stays inside the object only for maxSynapsesPerNeuron = 1, so the Poisson configs use M = 1 and
+# a start radius that forms no synapses.
+def poisson(fr, weight, mask=None):
+    m = f"""   <Masks><M>{" ".join(map(str, mask))}</M></Masks>\n""" if mask else ""
+    return f"""<?xml version="1.0" standalone="no"?>
+<InputParams>
+   <IMethod name="SInputPoisson"/>
+   <IntParams fr_mean="{fr}" weight="{weight}"/>
+{m}</InputParams>
+"""
+
+
+def regular(duration, interval, sync, on):
+    vals = ["1e-8" if k in on else "0.0" for k in range(100)]
+    rows = "\n".join("        " + " ".join(vals[r * 10:(r + 1) * 10]) for r in range(10))
+    return f"""<?xml version="1.0" standalone="no"?>
+<InputParams>
+   <IMethod name="SInputRegular"/>
+   <IntParams duration="{duration}" interval="{interval}" sync="{sync}"/>
+   <Values>
+    <I>
+{rows}
+    </I>
+   </Values>
+</InputParams>
+"""
+
+
+STIMULI = {
+    "stim/poisson-50hz-w25-mask.xml": poisson(50, 25, [3, 7, 11, 12, 13, 40, 41, 57, 58, 59, 60, 61, 62, 63, 64, 65,
+                                                       77, 80, 88, 99]),
+    "stim/poisson-10hz-w25.xml": poisson(10, 25),
+    "stim/regular-5ms-20ms-async.xml": regular(0.005, 0.020, "no", {0, 5, 23, 27, 44, 45, 54, 55, 72, 77, 99}),
+    "stim/regular-10ms-30ms-wave.xml": regular(0.010, 0.030, "wave", set(range(0, 100, 3))),
+}
+
+INPUT_CONFIGS = {
+    "input-lif-poisson": (dict(x=10, y=10, tsim=0.5, nsims=2, maxsyn=1, neurons=LIF, synapse="AllDSSynapses",
+                               layout=fixed_layout("NList/A10x10.xml", "NList/I10x10.xml"), start_radius=0.4),
+                          "stim/poisson-50hz-w25-mask.xml"),
+    "input-izh-poisson": (dict(x=5, y=5, tsim=0.5, nsims=2, maxsyn=1, neurons=IZH, synapse="AllDSSynapses",
+                               layout=fixed_layout("NList/A5x5.xml", "NList/I5x5.xml"), start_radius=0.4,
+                               max_rate=1000), "stim/poisson-10hz-w25.xml"),
+    "input-lif-regular-async": (dict(x=10, y=10, tsim=0.4, nsims=3, maxsyn=40, neurons=LIF, synapse="AllDSSynapses",
+                                     layout=fixed_layout("NList/A10x10.xml", "NList/I10x10.xml"), start_radius=0.6),
+                                "stim/regular-5ms-20ms-async.xml"),
+    "input-lif-regular-wave": (dict(x=10, y=10, tsim=0.5, nsims=2, maxsyn=40, neurons=LIF,
+                                    synapse="AllDynamicSTDPSynapses",
+                                    layout=fixed_layout("NList/A10x10.xml", "NList/I10x10.xml"), start_radius=0.9,
+                                    max_rate=1000), "stim/regular-10ms-30ms-wave.xml"),
+}
+
+
 def main():
     os.makedirs(os.path.join(OUT, "NList"), exist_ok=True)
+    os.makedirs(os.path.join(OUT, "stim"), exist_ok=True)
+    for rel, text in STIMULI.items():
+        with open(os.path.join(OUT, rel), "w") as f:
+            f.write(text)
+    for name, (kw, _stim) in INPUT_CONFIGS.items():
+        with open(os.path.join(OUT, name + ".xml"), "w") as f:
+            f.write(config(name, **kw))
     for rel, (tag, vals) in NLISTS.items():
         with open(os.path.join(OUT, rel), "w") as f:
             f.write(f'<?xml version="1.0" encoding="UTF-8"?>\n<{tag}>{" ".join(map(str, vals))}</{tag}>\n')
     for name, kw in CONFIGS.items():
         with open(os.path.join(OUT, name + ".xml"), "w") as f:
             f.write(config(name, **kw))
-    print("wrote", len(CONFIGS), "configs to", OUT)
+    print("wrote", len(CONFIGS) + len(INPUT_CONFIGS), "configs to", OUT)
 
 
 if __name__ == "__main__":

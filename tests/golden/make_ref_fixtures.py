@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 from oracle import oracle_py as O  # noqa: E402
+from make_configs import INPUT_CONFIGS  # noqa: E402
 
 CONFIGS = {
     # name: (config path relative to base dir, base dir, extra harness args)
@@ -30,20 +31,27 @@ for _n in ("conn-lif-ds", "conn-lif-ds-dense", "conn-lif-spiking", "conn-lif-std
            "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"):
     CONFIGS[_n] = (_n + ".xml", os.path.join(HERE, "configs"),
                    ["--dynamic-layout", "0.9", "0.1"] if "dynamic" in _n else [])
+for _n, (_kw, _stim) in INPUT_CONFIGS.items():
+    CONFIGS[_n] = (_n + ".xml", os.path.join(HERE, "configs"), ["--stim", _stim])
 
 KEEP = ("Vm", "Vthresh", "Vreset", "nStepsInRefr", "spikeCount", "spikeCountOffset", "spike_history", "u",
         "in_use", "sourceNeuronIndex", "destNeuronIndex", "W", "psr", "type", "synapse_counts", "decay", "total_delay",
         "delayQueue", "delayIdx", "r", "u_syn", "lastSpike", "delayQueuePost", "delayIdxPost", "radii", "rates",
         "incomingSynapseIndexMap", "incomingSynapseBegin", "incomingSynapseCount", "outgoingSynapseIndexMap",
         "outgoingSynapseBegin", "outgoingSynapseCount", "xloc", "yloc", "neuron_type_map", "starter_map", "Iinject",
-        "Inoise", "Vinit", "Trefract", "I0", "C1", "C2", "Aconst", "Bconst", "Cconst", "Dconst", "C3")
+        "Inoise", "Vinit", "Trefract", "I0", "C1", "C2", "Aconst", "Bconst", "Cconst", "Dconst", "C3",
+        "in_nISIs", "in_psr", "in_r", "in_u", "in_W", "in_lastSpike", "in_delayQueue", "in_delayIdx", "in_values",
+        "in_shift", "in_steps_in_cycle", "summation_map")
 
 
 def main():
     exe = os.path.join(O.REF_DIR, "bgref_dump")
     if not os.path.exists(exe):
         sys.exit("oracle/_ref/bgref_dump missing: make -f oracle/Makefile.ref")
+    only = sys.argv[1:]
     for name, (cfg, base, extra) in CONFIGS.items():
+        if only and name not in only:
+            continue
         with tempfile.TemporaryDirectory() as td:
             dump = os.path.join(td, "dump.bin")
             subprocess.run([exe, cfg, dump] + extra, cwd=base, check=True, stdout=subprocess.DEVNULL,

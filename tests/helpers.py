@@ -12,6 +12,11 @@ from oracle import oracle_py as O
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CONFIG_DIR = os.path.join(ROOT, "tests", "golden", "configs")
 
+# Validation configs with a stimulus input file (tests/golden/make_configs.py), relative to CONFIG_DIR.
+INPUT_STIM = {"input-lif-poisson": "stim/poisson-50hz-w25-mask.xml", "input-izh-poisson": "stim/poisson-10hz-w25.xml",
+              "input-lif-regular-async": "stim/regular-5ms-20ms-async.xml",
+              "input-lif-regular-wave": "stim/regular-10ms-30ms-wave.xml"}
+
 GROWTH_CONFIGS = ["conn-lif-ds", "conn-lif-ds-dense", "conn-lif-spiking", "conn-lif-stdp", "conn-lif-dynstdp",
                   "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"]
 
@@ -27,6 +32,8 @@ def load_config(name: str, deterministic: bool = True) -> dict:
     cfg = O.parse_config_xml(os.path.join(CONFIG_DIR, name + ".xml"), CONFIG_DIR)
     if "dynamic" in name:
         cfg.update(layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1)
+    if name in INPUT_STIM:
+        cfg["input"] = O.parse_input_xml(os.path.join(CONFIG_DIR, INPUT_STIM[name]), CONFIG_DIR)
     if deterministic:
         cfg.update(noise_mode=1, math_mode=1, sum_mode=1)
     return cfg
@@ -121,3 +128,33 @@ def compare_state(ora: O.OracleNet, net: B.Net, cfg: dict, where: str, check_map
             if not np.array_equal(ora.array(f), m[f]):
                 bad.append(f"{where}:{f}")
     return bad
+
+
+def izh_static_oracle(neurons: dict, syn: dict, meta: dict, seed: int, epoch_s: float = 1.0,
+                      max_rate: int = 200) -> tuple[O.OracleNet, dict]:
+    """An oracle network holding the static Izhikevich + DynamicSTDP network of
+    braingrid_amd.networks.izh_stdp_arrays, built through the oracle's own addSynapse/createSynapse.
+    Neuron constants drawn by the oracle's setup are replaced with the builder's; W, Wex and the
+    per-synapse total_delay are written after creation, as ConnStatic writes W (ConnStatic.cpp:75-80)."""
+    from braingrid_amd import networks as NW
+    N = meta["N"]
+    cfg = O.default_config()
+    cfg.update(width=N, height=1, neuron="AllIZHNeurons", synapse="AllDynamicSTDPSynapses", conn="ConnStatic",
+               layout="FixedLayout", inh_list=[int(i) for i in np.flatnonzero(meta["ntype"] == 1)], active_list=[],
+               max_synapses_per_neuron=meta["M"], max_firing_rate=max_rate, num_epochs=1, epoch_duration=epoch_s,
+               seed=seed, ring_bits=meta["ring_bits"], stdp=NW.STDP_CFG, noise_mode=1, math_mode=1, sum_mode=1,
+               Iinject=NW.IZH_NEURON["Iinject"], Inoise=NW.IZH_NEURON["Inoise"], Vthresh=NW.IZH_NEURON["Vthresh"],
+               Vresting=NW.IZH_NEURON["Vresting"], Vreset=NW.IZH_NEURON["Vreset"], Vinit=NW.IZH_NEURON["Vinit"])
+    ora = O.OracleNet(cfg)
+    ora.setup_consts = {f: ora.array(f).copy() for f in ("C1", "C2", "C3", "I0")}
+    for f in ("Vm", "Vthresh", "Vreset", "I0", "Inoise", "C1", "C2", "Trefract", "Aconst", "Bconst", "Cconst",
+              "Dconst", "u", "C3"):
+        ora.view(f)[:] = neurons[f]
+    src, dst, slot = meta["src"], meta["dst"], meta["slot"]
+    for q in range(len(src)):
+        got = ora.add_synapse(int(src[q]), int(dst[q]), float(meta["W"][q]))
+        assert got == slot[q], (q, got, slot[q])
+    ora.view("total_delay")[slot] = meta["total_delay"]
+    ora.view("Wex")[slot] = syn["Wex"][slot]
+    ora.rebuild_index_map()
+    return ora, cfg

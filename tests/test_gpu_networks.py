@@ -1,0 +1,32 @@
+"""GPU parity of the static Izhikevich + DynamicSTDP network (per-synapse delays of 11..201 steps
+on a 256-bit delay ring, STDP learning, Tsodyks-Markram PSR) against the oracle, bit-exact in the
+deterministic-noise mode. The device network is built by braingrid_amd.networks (the product
+builder) and the oracle network by the oracle's own addSynapse/createSynapse chain."""
+import pytest
+
+from braingrid_amd import networks as NW
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N,k,seed,chunks", [(200, 20, 7, (250, 750, 1000, 1000)), (1000, 100, 5, (600, 600))])
+def test_izh_dynstdp_static_parity(hip_ctx, N, k, seed, chunks):
+    print(f"\n[izh_dynstdp_static N={N} k={k} seed={seed}]", flush=True)
+    neu, syn, meta = NW.izh_stdp_arrays(N, seed, k=k)
+    ora, cfg = H.izh_static_oracle(neu, syn, meta, seed)
+    net, meta2 = NW.izh_stdp(hip_ctx, N, seed, k=k)
+    try:
+        assert meta2["M"] == meta["M"]
+        bad = H.compare_state(ora, net, cfg, "upload")
+        assert not bad, bad
+        for c in chunks:
+            ora.advance(c)
+            net.advance(c)
+            bad = H.compare_state(ora, net, cfg, f"step{ora.step}")
+            assert not bad, bad
+        assert net.error_bits() == 0
+        assert ora.array("spikeCount").sum() > 0
+    finally:
+        net.close()
+        ora.close()

@@ -8,6 +8,10 @@ Pinning of the oracle (oracle/bg_oracle.c), function by function:
     clamp/clear: every array after setup, after every epoch's steps and after every growth update
     equals the reference's dump (tests/golden/ref_*.npz, written by tests/golden/make_ref_fixtures.py
     from oracle/_ref/bgref_dump) on ten configs;
+  - stimulus input: SInputPoisson (interval counters drawn from the global MTRand stream, masks,
+    the per-neuron DS input synapse and its delay ring) and SInputRegular (10x10 value tiling,
+    asynchronous shifts drawn before the model's setup draws, wave shifts, the cycle window) on
+    four configs with the reference's -s stimulus files;
   - history compilation and output format: the formatted recorder output equals the reference's
     own 4-2016 goldens (old/validation/4-2016/growth/test-{tiny,small}-out.xml) byte for byte;
   - when oracle/_ref is present the same comparison also runs live against the reference.
@@ -23,6 +27,7 @@ import pytest
 
 from oracle import oracle_py as O
 from tests import xmlout
+from tests.helpers import INPUT_STIM
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
@@ -31,7 +36,7 @@ CFG_BASE = os.path.join(GOLDEN, "configs")
 
 CONFIGS = {"test-tiny": (os.path.join(REF_BASE, "configfiles/test-tiny.xml"), REF_BASE)}
 for _n in ("conn-lif-ds", "conn-lif-ds-dense", "conn-lif-spiking", "conn-lif-stdp", "conn-lif-dynstdp",
-           "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"):
+           "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic") + tuple(INPUT_STIM):
     CONFIGS[_n] = (os.path.join(CFG_BASE, _n + ".xml"), CFG_BASE)
 
 
@@ -40,6 +45,8 @@ def _cfg(name):
     cfg = O.parse_config_xml(path, base)
     if "dynamic" in name:
         cfg.update(layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1)
+    if name in INPUT_STIM:
+        cfg["input"] = O.parse_input_xml(os.path.join(base, INPUT_STIM[name]), base)
     return cfg
 
 
@@ -49,9 +56,14 @@ def _records(npz):
     out = []
     for i in range(n):
         tag = bytes(z[f"{i:03d}__tag"]).decode().strip()
-        arrs = {k[5:]: z[k] for k in z.files if k.startswith(f"{i:03d}_") and not k.startswith(f"{i:03d}__")}
+        arrs = {k[4:]: z[k] for k in z.files if k.startswith(f"{i:03d}_") and not k.startswith(f"{i:03d}__")}
         out.append((tag, int(z[f"{i:03d}__marker"][0]), arrs))
     return out
+
+
+# Per-slot synapse arrays: compared on in-use slots only (free slots hold stale values).
+SLOT_ARRAYS = {"sourceNeuronIndex", "destNeuronIndex", "W", "psr", "type", "decay", "total_delay", "delayQueue",
+               "delayIdx", "r", "u_syn", "lastSpike", "delayQueuePost", "delayIdxPost"}
 
 
 def _compare(net, arrs, where):
@@ -63,7 +75,7 @@ def _compare(net, arrs, where):
             mine = net.array(k)
         except KeyError:
             continue
-        if m is not None and mine.size == m.size and k != "in_use":
+        if m is not None and k in SLOT_ARRAYS:
             a, b = v[m], mine[m]
         else:
             a, b = v, mine[:v.size]
@@ -96,7 +108,7 @@ def test_oracle_matches_reference_fixtures(name):
     recs = _records(os.path.join(GOLDEN, f"ref_{name}.npz"))
     net, bad = _replay(name, recs)
     assert not bad, bad[:10]
-    if name != "test-tiny":
+    if name != "test-tiny" and not name.startswith("input-") or name.startswith("input-lif-regular"):
         assert net.total_synapses > 0
 
 
@@ -134,3 +146,24 @@ def test_oracle_matches_live_reference(name):
     if header:
         fresh = O.OracleNet(_cfg(name))
         assert not _compare(fresh, header[0][2], "header")
+
+
+@pytest.mark.parametrize("name", ["conn-lif-ds", "input-lif-poisson"])
+def test_fixture_comparison_is_not_vacuous(name):
+    """Every fixture array is matched to an oracle array, and a one-ulp change in a neuron array,
+    a synapse slot and an input counter is reported."""
+    recs = _records(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    net, bad = _replay(name, recs[:4])   # header, setup, first epoch, first growth update
+    assert not bad
+    arrs = recs[3][2]
+    for k in arrs:
+        net.array(k)   # raises KeyError for a fixture array the comparison would skip
+    assert len(arrs) >= 20
+    plants = {"Vm": lambda a: np.nextafter(a, np.float32(1), dtype=np.float32)}
+    if name.startswith("input"):
+        plants["in_nISIs"] = lambda a: a + 1
+    else:
+        plants["sourceNeuronIndex"] = lambda a: a + 1
+        assert arrs["in_use"].any()
+    for k, f in plants.items():
+        assert _compare(net, {**arrs, k: f(arrs[k])}, "plant") == [f"plant:{k}"], k
