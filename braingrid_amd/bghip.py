@@ -70,6 +70,22 @@ class Growth(ctypes.Structure):
                                               "startRadius", "epochDuration")]
 
 
+class Input(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("fr_mean", ctypes.c_float), ("weight", ctypes.c_float),
+                ("mask", _bp), ("neuron_type", _ip), ("duration", ctypes.c_float), ("interval", ctypes.c_float),
+                ("values", _fp), ("shift", _ip)]
+
+
+class InputState(ctypes.Structure):
+    _fields_ = [("nISIs", _ip), ("psr", _fp), ("r", _fp), ("u", _fp), ("lastSpike", _qp), ("delayQueue", _up),
+                ("delayIdx", _ip), ("steps_in_cycle", ctypes.c_int32)]
+
+
+INPUT_NONE, INPUT_POISSON, INPUT_REGULAR = 0, 1, 2
+INPUT_STATE_FIELDS = {"nISIs": np.int32, "psr": np.float32, "r": np.float32, "u": np.float32,
+                      "lastSpike": np.uint64, "delayQueue": np.uint32, "delayIdx": np.int32}
+
+
 class GrowthStats(ctypes.Structure):
     _fields_ = [("total_synapses", ctypes.c_int64), ("created", ctypes.c_int64), ("erased", ctypes.c_int64)]
 
@@ -125,6 +141,8 @@ def load():
         "bghip_download_growth": (ctypes.c_int, [vp, _fp, _fp]),
         "bghip_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(ctypes.c_int64)]),
+        "bghip_upload_input": (ctypes.c_int, [vp, ctypes.POINTER(Input), ctypes.POINTER(InputState)]),
+        "bghip_download_input": (ctypes.c_int, [vp, ctypes.POINTER(InputState)]),
         "bghip_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
         "bghip_batch_destroy": (None, [vp]),
         "bghip_batch_set": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.c_int]),
@@ -145,7 +163,8 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_update_history", "bghip_sync", "bghip_net_error_bits", "bghip_net_step",
             "bghip_download_neurons", "bghip_download_synapses", "bghip_total_synapses",
             "bghip_download_index_map", "bghip_download_growth", "bghip_kernel_timing", "bghip_batch_create",
-            "bghip_batch_destroy", "bghip_batch_set", "bghip_batch_advance", "bghip_batch_sync")
+            "bghip_batch_destroy", "bghip_batch_set", "bghip_batch_advance", "bghip_batch_sync",
+            "bghip_upload_input", "bghip_download_input")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -333,6 +352,44 @@ class Net:
         q = np.zeros(self.N, np.float32)
         self._c(load().bghip_download_growth(self.h, _ptr(r, _fp), _ptr(q, _fp)), "download_growth")
         return r, q
+
+    def upload_input(self, kind: int, neuron_type=None, fr_mean: float = 0.0, weight: float = 0.0, mask=None,
+                     duration: float = 0.0, interval: float = 0.0, values=None, shift=None, state: dict | None = None):
+        """bghip_upload_input: SInputPoisson (fr_mean Hz, weight x 1e-8, optional 0/1 mask) or
+        SInputRegular (duration/interval s, per-neuron values and shifts)."""
+        inp = Input()
+        keep = []
+
+        def arr(a, dt, ct):
+            if a is None:
+                return ctypes.cast(None, ct)
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return _ptr(a, ct)
+        inp.kind, inp.fr_mean, inp.weight = kind, fr_mean, weight
+        inp.mask = arr(mask, np.uint8, _bp)
+        inp.neuron_type = arr(neuron_type, np.int32, _ip)
+        inp.duration, inp.interval = duration, interval
+        inp.values = arr(values, np.float32, _fp)
+        inp.shift = arr(shift, np.int32, _ip)
+        st = None
+        if state is not None:
+            st = InputState()
+            for f, dt in INPUT_STATE_FIELDS.items():
+                if state.get(f) is not None:
+                    setattr(st, f, arr(state[f], dt, _CT[dt]))
+            st.steps_in_cycle = int(state.get("steps_in_cycle", 0))
+        self._c(load().bghip_upload_input(self.h, ctypes.byref(inp), ctypes.byref(st) if st is not None else None),
+                "upload_input")
+
+    def download_input(self) -> dict:
+        out = {f: np.zeros(self.N, dt) for f, dt in INPUT_STATE_FIELDS.items()}
+        st = InputState()
+        for f, a in out.items():
+            setattr(st, f, _ptr(a, _CT[a.dtype.type]))
+        self._c(load().bghip_download_input(self.h, ctypes.byref(st)), "download_input")
+        out["steps_in_cycle"] = st.steps_in_cycle
+        return out
 
     def timing(self, enable: bool):
         avg = ctypes.c_double()

@@ -5,6 +5,8 @@
 // orientation, per-synapse delayIdx, separate type/source/delay arrays) and the device's packed
 // slot layout only at upload and download time.
 #include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -44,6 +46,9 @@ struct bghip_net {
     int64_t timed_launches;
     hipEvent_t ext_ev;     // work another stream (a batch) must finish before this net's stream runs
     bool ext_pending;
+    uint32_t in_phase;     // Poisson input ring: reference delayIdx = (step - in_phase) mod 32
+    int32_t *in_idx_host;  // N: delayIdx of neurons without input (their synapse never advances)
+    uint8_t *in_mask_host; // N: copy of dn.in_mask
 };
 
 // Orders the network's own stream after outstanding batch work on it.
@@ -160,6 +165,8 @@ static void plan_net(bghip_net *n, Carve &c)
     c.add(&dn.syn_count, N); c.add(&dn.slot_hi, N); c.add(&dn.in_map, S); c.add(&dn.in_begin, N);
     c.add(&dn.xloc, N); c.add(&dn.yloc, N); c.add(&dn.radii, N); c.add(&dn.rates, N); c.add(&dn.ntype, N);
     c.add(&dn.err, 1);
+    c.add(&dn.in_mask, N); c.add(&dn.in_nisi, N); c.add(&dn.in_psr, N); c.add(&dn.in_r, N); c.add(&dn.in_u, N);
+    c.add(&dn.in_last, N); c.add(&dn.in_ring, N); c.add(&dn.in_values, N); c.add(&dn.in_shift, N);
     c.add(&n->d_out_map, S); c.add(&n->d_out_begin, N); c.add(&n->d_out_count, N); c.add(&n->d_cursor, N);
     c.add(&n->d_stats, 4); c.add(&n->d_total, 1);
 }
@@ -246,6 +253,8 @@ extern "C" void bghip_net_destroy(bghip_net *n)
     (void)hipEventDestroy(n->ev1);
     (void)hipEventDestroy(n->ext_ev);
     (void)hipStreamDestroy(n->stream);
+    free(n->in_idx_host);
+    free(n->in_mask_host);
     (void)hipFree(n->arena);
     delete n;
 }
@@ -669,6 +678,142 @@ extern "C" int bghip_download_growth(bghip_net *n, float *radii, float *rates)
     CHK(d2h(n, radii, n->dn.radii, n->dn.N));
     CHK(d2h(n, rates, n->dn.rates, n->dn.N));
     return bghip_sync(n);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Stimulus input. The input synapse constants are those of a freshly created AllDSSynapses
+// synapse of type EE (index 0) or EI (index 1): Synapses/AllSpikingSynapses.cpp:191-233 and
+// Synapses/AllDSSynapses.cpp:180-218, decay on the host C library (AllSpikingSynapses.cpp:344-354).
+extern "C" int bghip_upload_input(bghip_net *n, const bghip_input *in, const bghip_input_state *st)
+{
+    if (!n) return BGHIP_EINVAL;
+    net_begin(n);
+    DevNet &dn = n->dn;
+    const int N = dn.N;
+    const float dt = dn.dt;
+    if (!in || in->kind == BGHIP_INPUT_NONE) {
+        dn.in_kind = BG_INPUT_NONE;
+        return BGHIP_OK;
+    }
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    if (in->kind == BGHIP_INPUT_POISSON) {
+        if (!in->neuron_type) return set_err(n->ctx, BGHIP_EINVAL, "upload_input: Poisson input needs neuron_type");
+        if (!(in->fr_mean > 0.0f)) return set_err(n->ctx, BGHIP_EINVAL, "upload_input: fr_mean must be positive");
+        static const float tau[2] = {(float)3e-3, (float)3e-3}, delay[2] = {(float)1.5e-3, (float)0.8e-3};
+        static const float U[2] = {(float)0.5, (float)0.05}, D[2] = {(float)1.1, (float)0.125};
+        static const float F[2] = {(float)0.05, (float)1.2};
+        for (int k = 0; k < 2; k++) {
+            dn.in_decay[k] = expf(-dt / tau[k]);
+            dn.in_delay[k] = (int)(delay[k] / dt) + 1;
+            dn.in_U[k] = U[k]; dn.in_D[k] = D[k]; dn.in_F[k] = F[k];
+            if (dn.in_delay[k] < 1 || dn.in_delay[k] >= 32)
+                return set_err(n->ctx, BGHIP_EUNSUPPORTED, "upload_input: input delay does not fit its ring");
+        }
+        const float fr = in->fr_mean / 1000;   // per ms (Inputs/SInputPoisson.cpp:55-57)
+        dn.in_lambda = 1 / fr;
+        dn.in_W = in->weight * (float)1.0e-8;  // AllSynapses::SYNAPSE_STRENGTH_ADJUSTMENT
+        std::vector<uint8_t> mask(N);
+        for (int i = 0; i < N; i++) {
+            if (in->neuron_type[i] != 1 && in->neuron_type[i] != 2)
+                return set_err(n->ctx, BGHIP_EINVAL, "upload_input: neuron type must be INH(1)/EXC(2)");
+            const bool on = in->mask ? in->mask[i] != 0 : true;
+            mask[i] = on ? (uint8_t)(1 + (in->neuron_type[i] == 1 ? 1 : 0)) : 0;
+        }
+        std::vector<int32_t> nisi(N, 0);
+        std::vector<float> psr(N, 0.0f), r(N, 1.0f), u(N, (float)0.4);
+        std::vector<uint64_t> last(N, BG_NO_SPIKE);
+        std::vector<uint32_t> ring(N, 0u);
+        uint32_t phase = (uint32_t)(n->step & 31u);
+        if (st) {
+            if (st->nISIs) std::copy(st->nISIs, st->nISIs + N, nisi.begin());
+            if (st->psr) std::copy(st->psr, st->psr + N, psr.begin());
+            if (st->r) std::copy(st->r, st->r + N, r.begin());
+            if (st->u) std::copy(st->u, st->u + N, u.begin());
+            if (st->lastSpike) std::copy(st->lastSpike, st->lastSpike + N, last.begin());
+            if (st->delayQueue) {
+                // every neuron with input advances its input synapse each step, so they share one
+                // delayIdx; the synapses of neurons without input keep theirs
+                if (!st->delayIdx) return set_err(n->ctx, BGHIP_EINVAL, "upload_input: delayQueue needs delayIdx");
+                int di = -1;
+                for (int i = 0; i < N; i++) {
+                    if (st->delayIdx[i] < 0 || st->delayIdx[i] >= 32)
+                        return set_err(n->ctx, BGHIP_EINVAL, "upload_input: delayIdx out of range");
+                    if (!mask[i]) continue;
+                    if (di < 0) di = st->delayIdx[i];
+                    if (st->delayIdx[i] != di)
+                        return set_err(n->ctx, BGHIP_EUNSUPPORTED, "upload_input: input rings of different phases");
+                }
+                if (di < 0) di = 0;
+                phase = (uint32_t)((n->step - (uint64_t)di) & 31u);
+                // reference bit b (read at delayIdx when the step is `step`) -> absolute bit b + phase
+                for (int i = 0; i < N; i++) {
+                    const uint32_t q = st->delayQueue[i];
+                    ring[i] = mask[i] && phase ? (q << phase) | (q >> (32u - phase)) : q;
+                }
+            }
+        }
+        n->in_phase = phase;
+        if (!n->in_idx_host) n->in_idx_host = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+        if (!n->in_mask_host) n->in_mask_host = (uint8_t *)malloc((size_t)N);
+        if (!n->in_idx_host || !n->in_mask_host) return set_err(n->ctx, BGHIP_ENOMEM, "upload_input: host allocation");
+        for (int i = 0; i < N; i++) {
+            n->in_idx_host[i] = st && st->delayIdx ? st->delayIdx[i] : 0;
+            n->in_mask_host[i] = mask[i];
+        }
+        CHK(h2d(n, dn.in_mask, mask.data(), N)); CHK(h2d(n, dn.in_nisi, nisi.data(), N));
+        CHK(h2d(n, dn.in_psr, psr.data(), N)); CHK(h2d(n, dn.in_r, r.data(), N)); CHK(h2d(n, dn.in_u, u.data(), N));
+        CHK(h2d(n, dn.in_last, last.data(), N)); CHK(h2d(n, dn.in_ring, ring.data(), N));
+        HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+        dn.in_kind = BG_INPUT_POISSON;
+        return BGHIP_OK;
+    }
+    if (in->kind == BGHIP_INPUT_REGULAR) {
+        if (!in->values) return set_err(n->ctx, BGHIP_EINVAL, "upload_input: regular input needs values");
+        // Inputs/SInputRegular.cpp:52-55
+        const int dur = (int)((double)(in->duration / dt) + 0.5);
+        const int itv = (int)((double)(in->interval / dt) + 0.5);
+        if (dur < 0 || itv < 0 || dur + itv <= 0)
+            return set_err(n->ctx, BGHIP_EINVAL, "upload_input: empty regular-input cycle");
+        const int cycle = dur + itv;
+        std::vector<int32_t> shift(N, 0);
+        if (in->shift) std::copy(in->shift, in->shift + N, shift.begin());
+        const int c_now = st ? st->steps_in_cycle : 0;
+        if (c_now < 0 || c_now >= cycle) return set_err(n->ctx, BGHIP_EINVAL, "upload_input: steps_in_cycle out of range");
+        dn.in_dur = dur;
+        dn.in_cycle = cycle;
+        dn.in_cyc_off = ((uint64_t)c_now + (uint64_t)cycle - n->step % (uint64_t)cycle) % (uint64_t)cycle;
+        CHK(h2d(n, dn.in_values, in->values, N)); CHK(h2d(n, dn.in_shift, shift.data(), N));
+        HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+        dn.in_kind = BG_INPUT_REGULAR;
+        return BGHIP_OK;
+    }
+    return set_err(n->ctx, BGHIP_EINVAL, "upload_input: unknown input kind %d", in->kind);
+}
+
+extern "C" int bghip_download_input(bghip_net *n, bghip_input_state *st)
+{
+    if (!n || !st) return BGHIP_EINVAL;
+    net_begin(n);
+    DevNet &dn = n->dn;
+    const int N = dn.N;
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    if (dn.in_kind == BG_INPUT_REGULAR)
+        st->steps_in_cycle = (int32_t)((n->step + dn.in_cyc_off) % (uint64_t)dn.in_cycle);
+    if (dn.in_kind != BG_INPUT_POISSON) return bghip_sync(n);
+    std::vector<uint32_t> ring(N);
+    CHK(d2h(n, st->nISIs, dn.in_nisi, N)); CHK(d2h(n, st->psr, dn.in_psr, N)); CHK(d2h(n, st->r, dn.in_r, N));
+    CHK(d2h(n, st->u, dn.in_u, N)); CHK(d2h(n, st->lastSpike, dn.in_last, N));
+    CHK(d2h(n, ring.data(), dn.in_ring, N));
+    CHK(bghip_sync(n));
+    const uint32_t phase = n->in_phase & 31u;
+    const int di = (int)((n->step - (uint64_t)phase) & 31u);
+    for (int i = 0; i < N; i++) {
+        const bool on = n->in_mask_host[i] != 0;
+        if (st->delayQueue)
+            st->delayQueue[i] = on && phase ? (ring[i] >> phase) | (ring[i] << (32u - phase)) : ring[i];
+        if (st->delayIdx) st->delayIdx[i] = on ? di : n->in_idx_host[i];
+    }
+    return BGHIP_OK;
 }
 
 // ------------------------------------------------------------------------------------------------

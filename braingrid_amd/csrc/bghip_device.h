@@ -54,7 +54,23 @@ struct DevNet {
     int ds_typed;          // 1: every synapse's U, D, F equal its type's creation values
     float tU[4], tD[4], tF[4];
     uint32_t *err;
+    // stimulus input (Inputs/SInputPoisson.cpp, Inputs/SInputRegular.cpp)
+    int in_kind;                 // BG_INPUT_*
+    uint8_t *in_mask;            // Poisson: 0 no input, 1 + k input synapse of kind k (0 EE, 1 EI)
+    int32_t *in_nisi;            // Poisson: steps to the next input spike (nISIs)
+    float *in_psr, *in_r, *in_u;
+    uint64_t *in_last;
+    uint32_t *in_ring;           // 32-bit ring per neuron, absolute-time orientation (bit t mod 32)
+    float in_W, in_lambda;       // W = weight * 1e-8; mean interval in ms
+    float in_decay[2], in_U[2], in_D[2], in_F[2];
+    int in_delay[2];
+    float *in_values;            // Regular: value per neuron
+    int32_t *in_shift;           // Regular: nShiftValues
+    int in_dur, in_cycle;        // Regular: nStepsDuration, nStepsCycle
+    uint64_t in_cyc_off;         // Regular: nStepsInCycle at step t is (t + in_cyc_off) mod in_cycle
 };
+
+enum { BG_INPUT_NONE = 0, BG_INPUT_POISSON = 1, BG_INPUT_REGULAR = 2 };
 
 __device__ __forceinline__ void dev_error(const DevNet &n, uint32_t bit) { atomicOr(n.err, bit); }
 
@@ -101,6 +117,64 @@ __device__ __forceinline__ bool neuron_advance(const DevNet &n, int i, uint64_t 
         float Vb = (float)((double)Vint + (double)s.C3 * q);
         s.u = s.u + s.C3 * s.A * (s.B * Vint - s.u);
         s.Vm = (float)((double)Vb * 0.001 + (double)(s.C2 * x));
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stimulus input for neuron i at step t: HostSInputPoisson::inputStimulus
+// (Inputs/HostSInputPoisson.cpp:60-94: interval counter, preSpikeHit into the input synapse's
+// ring, redraw, then the DS input synapse's advanceSynapse) and HostSInputRegular::inputStimulus
+// (Inputs/HostSInputRegular.cpp:59-80: add the value inside the cycle window). Returns whether
+// the summation point receives an addition at step t and, in *add, the addend; the caller
+// performs the reference's single `summationPoint += add`. The input does not depend on the
+// neuron's own state, so the fused kernels evaluate a whole block of steps ahead of the neuron
+// loop.
+__device__ __forceinline__ bool input_step(const DevNet &n, int i, uint64_t t, float *add)
+{
+    if (n.in_kind == BG_INPUT_POISSON) {
+        const uint32_t m = n.in_mask[i];
+        if (m == 0) return false;
+        const int k = (int)m - 1;
+        int nisi = n.in_nisi[i] - 1;
+        uint32_t ring = n.in_ring[i];
+        const uint32_t tb = (uint32_t)t & 31u;
+        if (nisi <= 0) {
+            const uint32_t b = 1u << ((tb + (uint32_t)n.in_delay[k]) & 31u);
+            if (ring & b) dev_error(n, 1u);
+            ring |= b;
+            nisi = bg_poisson_isi_steps(n.seed, (uint32_t)i, t, n.in_lambda, n.dt);
+        }
+        n.in_nisi[i] = nisi;
+        const float decay = n.in_decay[k];
+        float psr = n.in_psr[i];
+        if ((ring >> tb) & 1u) {
+            ring &= ~(1u << tb);
+            float r = n.in_r[i], u = n.in_u[i];
+            const uint64_t last = n.in_last[i];
+            if (last != BG_NO_SPIKE) {
+                const float isi = (float)(t - last) * n.dt;
+                r = 1.0f + (r * (1.0f - u) - 1.0f) * bg_expf(-isi / n.in_D[k]);
+                u = n.in_U[k] + u * (1.0f - n.in_U[k]) * bg_expf(-isi / n.in_F[k]);
+                n.in_r[i] = r;
+                n.in_u[i] = u;
+            }
+            n.in_last[i] = t;
+            psr = psr + ((n.in_W / decay) * u) * r;
+        }
+        n.in_ring[i] = ring;
+        psr = psr * decay;
+        n.in_psr[i] = psr;
+        *add = psr;
+        return true;
+    }
+    if (n.in_kind == BG_INPUT_REGULAR) {
+        const int c = (int)((t + n.in_cyc_off) % (uint64_t)n.in_cycle);
+        const int sh = n.in_shift[i];
+        if (c >= sh && c < (sh + n.in_dur) % n.in_cycle) {
+            *add = n.in_values[i];
+            return true;
+        }
     }
     return false;
 }

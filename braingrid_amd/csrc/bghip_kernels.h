@@ -17,7 +17,9 @@ __device__ __forceinline__ void neuron_step_body(const DevNet &n, int i, uint64_
     NeuronRegs s;
     neuron_load<NK>(n, i, s);
     const float noise = bg_noise_normal(n.seed, (uint32_t)i, t);
-    bool fired = neuron_advance<NK>(n, i, t, n.sp[i], noise, s);
+    float sp = n.sp[i], add;
+    if (n.in_kind != BG_INPUT_NONE && input_step(n, i, t, &add)) sp = sp + add;
+    bool fired = neuron_advance<NK>(n, i, t, sp, noise, s);
     neuron_store<NK>(n, i, s);
     n.sp[i] = 0.0f;
     if (fired) n.fired_step[i] = t;
@@ -170,6 +172,7 @@ __device__ __forceinline__ void fused_block_body(const DevNet &n, int d, uint64_
                                                  uint64_t tprev0, int sel_in)
 {
     __shared__ float s_noise[BG_BLOCK / BG_WAVE][KMAX];
+    __shared__ float s_in[BG_BLOCK / BG_WAVE][KMAX];
     const int lane = threadIdx.x & (BG_WAVE - 1);
     const int wv = threadIdx.x / BG_WAVE;
     const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
@@ -228,6 +231,17 @@ __device__ __forceinline__ void fused_block_body(const DevNet &n, int d, uint64_
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     if (lane == 0) {
+        // stimulus input of the K steps (independent of the neuron's state), then the neuron
+        uint32_t inbits = 0;
+        if (n.in_kind != BG_INPUT_NONE) {
+            for (int k = 0; k < K; k++) {
+                float add;
+                if (input_step(n, d, t0 + (uint64_t)k, &add)) {
+                    s_in[wv][k] = add;
+                    inbits |= 1u << k;
+                }
+            }
+        }
         NeuronRegs s;
         neuron_load<NK>(n, d, s);
         float sp = n.sp[d];
@@ -235,6 +249,7 @@ __device__ __forceinline__ void fused_block_body(const DevNet &n, int d, uint64_
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
             if (k < K) {
+                if ((inbits >> k) & 1u) sp = sp + s_in[wv][k];
                 if (neuron_advance<NK>(n, d, t0 + (uint64_t)k, sp, s_noise[wv][k], s)) mask |= 1u << k;
                 sp = 0.0f + acc[k];
             }

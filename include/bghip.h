@@ -124,6 +124,39 @@ typedef struct bghip_growth_stats {
     int64_t erased;
 } bghip_growth_stats;
 
+/* Stimulus input of a network (Inputs/ISInput.h; created by FSInput::CreateInstance,
+ * Inputs/FSInput.cpp:40-97). Applied on the device at every step before the neuron advance, as
+ * Simulator::advanceUntilGrowth calls inputStimulus before IModel::advance (Core/Simulator.cpp:177-182). */
+enum { BGHIP_INPUT_NONE = 0, BGHIP_INPUT_POISSON = 1, BGHIP_INPUT_REGULAR = 2 };
+
+typedef struct bghip_input {
+    int32_t kind;                 /* BGHIP_INPUT_* */
+    /* SInputPoisson (Inputs/SInputPoisson.cpp:24-146): IntParams fr_mean (Hz) and weight; the
+     * input synapse of neuron i is a DS synapse of type EI when neuron_type[i] is INH (1), else
+     * EE, with W = weight * 1e-8. Interval draws come from the Philox stream of
+     * bg_poisson_isi_steps() in include/bg_detmath.h. */
+    float fr_mean, weight;
+    const uint8_t *mask;          /* N bytes, nonzero = neuron receives input; NULL = all (no <Masks>) */
+    const int32_t *neuron_type;   /* N (Layout::neuron_type_map); required for Poisson input */
+    /* SInputRegular (Inputs/SInputRegular.cpp:26-110, Inputs/HostSInputRegular.cpp:59-80) */
+    float duration, interval;     /* seconds */
+    const float *values;          /* N amperes (the 10x10 <Values> table already tiled over the grid) */
+    const int32_t *shift;         /* N: nShiftValues (sync "no" / "wave"); NULL = all 0 */
+} bghip_input;
+
+/* Mutable input state (for resuming and for checking): the Poisson interval counters and the
+ * per-neuron DS input synapse, in the reference's orientation (delayQueue is the 32-bit ring read
+ * at delayIdx), plus SInputRegular's nStepsInCycle. Pointers are N long; upload accepts NULL for
+ * the state a freshly constructed input has (nISIs 0, psr 0, r 1, u 0.4, no last spike, empty ring). */
+typedef struct bghip_input_state {
+    int32_t *nISIs;
+    float *psr, *r, *u;
+    uint64_t *lastSpike;
+    uint32_t *delayQueue;
+    int32_t *delayIdx;
+    int32_t steps_in_cycle;
+} bghip_input_state;
+
 /* ---- context ---------------------------------------------------------------------------- */
 /* Replaces GPUSpikingModel::setupSim's cudaSetDevice (Core/GPUSpikingModel.cu:109). */
 int bghip_ctx_create(int device, bghip_ctx **out);
@@ -151,6 +184,11 @@ int bghip_upload_synapses(bghip_net *net, const bghip_synapses *h, uint64_t step
  * EXC=2) plus growth state; needed by bghip_update_connections. */
 int bghip_upload_layout(bghip_net *net, const float *xloc, const float *yloc, const int32_t *neuron_type);
 int bghip_upload_growth(bghip_net *net, const bghip_growth *g, const float *radii, const float *rates);
+
+/* ISInput::init for SInputPoisson / SInputRegular on the device; `in` == NULL or kind NONE
+ * removes the input. The state is taken as of the network's current step. */
+int bghip_upload_input(bghip_net *net, const bghip_input *in, const bghip_input_state *state);
+int bghip_download_input(bghip_net *net, bghip_input_state *state);
 
 /* ---- simulation ------------------------------------------------------------------------- */
 /* n_steps calls of GPUSpikingModel::advance (Core/GPUSpikingModel.cu:188-225), starting at the

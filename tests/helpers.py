@@ -74,7 +74,38 @@ def gpu_net_from_oracle(ctx: B.Context, ora: O.OracleNet, cfg: dict, block_steps
     g = MD.growth_params(cfg["epsilon"], cfg["beta"], cfg["rho"], cfg["target_rate"], cfg["min_radius"],
                          cfg["start_radius"], cfg["epoch_duration"])
     net.upload_growth(g, ora.array("radii"), ora.array("rates"))
+    upload_input_from_oracle(net, ora, cfg)
     return net
+
+
+INPUT_STATE = {"nISIs": "in_nISIs", "psr": "in_psr", "r": "in_r", "u": "in_u", "lastSpike": "in_lastSpike",
+               "delayQueue": "in_delayQueue", "delayIdx": "in_delayIdx"}
+
+
+def upload_input_from_oracle(net: B.Net, ora: O.OracleNet, cfg: dict):
+    """Give the device network the oracle's stimulus input, state included."""
+    inp = cfg.get("input")
+    if not inp:
+        return
+    if inp["kind"] == "poisson":
+        net.upload_input(B.INPUT_POISSON, neuron_type=ora.array("neuron_type_map"), fr_mean=inp["fr_mean"],
+                         weight=inp["weight"], mask=ora.array("in_mask"),
+                         state={k: ora.array(v) for k, v in INPUT_STATE.items()})
+    else:
+        net.upload_input(B.INPUT_REGULAR, duration=inp["duration"], interval=inp["interval"],
+                         values=ora.array("in_values"), shift=ora.array("in_shift"),
+                         state={"steps_in_cycle": int(ora.array("in_steps_in_cycle")[0])})
+
+
+def compare_input(ora: O.OracleNet, net: B.Net, cfg: dict, where: str) -> list[str]:
+    inp = cfg.get("input")
+    if not inp:
+        return []
+    g = net.download_input()
+    if inp["kind"] == "regular":
+        ok = g["steps_in_cycle"] == int(ora.array("in_steps_in_cycle")[0])
+        return [] if ok else [f"{where}:in_steps_in_cycle"]
+    return [f"{where}:{v}" for k, v in INPUT_STATE.items() if not _eq(ora.array(v), g[k])]
 
 
 def _eq(a, b):

@@ -187,3 +187,38 @@ def test_wave64_summation_stays_within_tolerance_of_sequential():
     va, vb = a.array("Vm"), b.array("Vm")
     assert np.array_equal(a.array("spike_history"), b.array("spike_history"))
     assert np.max(np.abs(va - vb) / np.maximum(np.abs(va), 1e-6)) < 1e-5
+
+
+def test_poisson_intervals_follow_the_rejection_density():
+    """The counter-based Poisson input draws (bg_poisson_isi_steps) against the distribution the
+    reference's loop defines (Inputs/HostSInputPoisson.cpp:75-84): an exponential interval of mean
+    lambda ms, accepted with probability 1 - exp(-isi^2 / 32), then rounded to steps. Checked by
+    mean, by a Kolmogorov-Smirnov test against the numerically integrated CDF, and by the
+    independence of neighbouring counters."""
+    from scipy import integrate, stats
+    L = O.lib()
+    lam, dt = 100.0, 1e-4           # 10 Hz (fr_mean 10 -> 0.01 per ms -> lambda 100 ms)
+    n = 40000
+    isi = np.array([L.bgo_poisson_isi(1, k % 977, 1000 + k // 977, lam, dt) for k in range(n)], np.float64)
+    dens = lambda x: math.exp(-x / lam) / lam * (1.0 - math.exp(-x * x / 32.0))
+    Z = integrate.quad(dens, 0, np.inf)[0]
+    mean_ms = integrate.quad(lambda x: x * dens(x), 0, np.inf)[0] / Z
+    sd_ms = math.sqrt(integrate.quad(lambda x: x * x * dens(x), 0, np.inf)[0] / Z - mean_ms ** 2)
+    steps_per_ms = 1e-3 / dt
+    assert abs(isi.mean() / steps_per_ms - mean_ms) < 4 * sd_ms / math.sqrt(n)
+    xs = np.sort(isi / steps_per_ms)
+    grid = np.linspace(0, 1000, 2001)
+    cdf_grid = np.concatenate([[0.0], np.cumsum([integrate.quad(dens, a, b)[0] for a, b in zip(grid[:-1], grid[1:])])]) / Z
+    # rounding to whole steps moves a sample by at most 0.05 ms; compare against the CDF shifted both ways
+    emp = np.searchsorted(xs, grid, side="right") / n
+    lo = np.interp(grid - 0.05, grid, cdf_grid)
+    hi = np.interp(grid + 0.05, grid, cdf_grid)
+    dev = np.maximum(emp - hi, lo - emp).max()
+    assert dev < stats.kstwo.ppf(0.999, n)
+    # intervals inside the refractoriness are almost never accepted
+    assert (xs < 1.0).mean() < 1e-3
+    # neighbouring neurons and steps are uncorrelated
+    a = isi[:-977]
+    b = isi[977:]
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.03
+    assert abs(np.corrcoef(isi[:-1], isi[1:])[0, 1]) < 0.03
