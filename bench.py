@@ -80,9 +80,81 @@ def load_traffic():
         return json.load(f)
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batched_main(args):
+    """The batched form: a sweep of independent networks run one at a time with their own
+    launches, then all together with shared launches; same initial states, same results."""
+    from braingrid_amd import bghip as B
+    from braingrid_amd import sweep as SW
+
+    ctx = B.Context(0)
+    specs = SW.sweep_specs(args.nets, args.seed, sim_s=args.sim_seconds)
+    N_total = sum(s["side"] ** 2 for s in specs)
+    sim_total = sum(s["sim_s"] for s in specs)
+    log(f"sweep: {len(specs)} networks, {N_total} neurons, {sim_total} simulated s")
+    # alone: every `alone_stride`-th network of the sweep, run one at a time; the sweep's alone
+    # time is that sample's time scaled by the sampling ratio
+    stride = max(1, args.alone_stride)
+    sample = specs[::stride]
+    nets = SW.build(ctx, sample, log)
+    t_alone_sample = SW.run_alone(nets, log)
+    fp_alone = {x.spec["id"]: x.fingerprint() for x in nets}
+    for x in nets:
+        x.close()
+    t_alone = t_alone_sample * len(specs) / len(sample)
+    nets = SW.build(ctx, specs, log)
+    S_total = sum(x.synapses for x in nets)
+    t_batched = SW.run_batched(ctx, nets, log)
+    fp_batched = {x.spec["id"]: x.fingerprint() for x in nets if x.spec["id"] in fp_alone}
+    errs = sum(x.net.error_bits() != 0 for x in nets)
+    for x in nets:
+        x.close()
+    same = sum(fp_alone[k] == fp_batched[k] for k in fp_alone)
+    value = sim_total / t_batched * 3600.0
+    out = {
+        "metric": "sims/hr (simulated network-seconds per wall hour) of a batched sweep",
+        "value": value,
+        "unit": "sim-s/h",
+        "n_gpus": 1,
+        "steps": int(round(args.sim_seconds / 1e-4)),
+        "warmup": 0,
+        "ms_per_step": t_batched * 1e3 / (args.sim_seconds / 1e-4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded sweep; deterministic Philox noise and Poisson input)",
+        "config": {"workload": f"batched sweep: {len(specs)} networks, sides log-uniform 10..120, "
+                               f"LIF+DS+growth / Izhikevich+DynamicSTDP+Poisson, {args.sim_seconds} s each",
+                   "networks": len(specs), "neurons": N_total, "synapses": int(S_total),
+                   "lif_growth": sum(s["kind"] == "lif_growth" for s in specs),
+                   "izh_stdp": sum(s["kind"] == "izh_stdp" for s in specs), "seed": args.seed},
+        "alone_wall_s": t_alone,
+        "alone_sample": f"every {stride}th network ({len(sample)} of {len(specs)}) run alone in "
+                        f"{t_alone_sample:.1f} s, scaled by {len(specs) / len(sample):.2f}",
+        "batched_wall_s": t_batched,
+        "alone_value": sim_total / t_alone * 3600.0,
+        "speedup_vs_alone": t_alone / t_batched,
+        "identical_results": f"{same}/{len(sample)} sampled networks bit-identical alone and batched",
+        "networks_with_device_errors": int(errs),
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--form", choices=("single", "batched"), default="single",
+                    help="single: the 10k-neuron headline; batched: the multi-network sweep")
+    ap.add_argument("--nets", type=int, default=256, help="batched form: networks in the sweep")
+    ap.add_argument("--sim-seconds", type=float, default=5.0, help="batched form: simulated seconds per network")
+    ap.add_argument("--seed", type=int, default=2024, help="batched form: sweep seed")
+    ap.add_argument("--alone-stride", type=int, default=4,
+                    help="batched form: run every k-th network alone for the one-at-a-time comparison")
     ap.add_argument("--steps", type=int, default=100000)
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--roofline-steps", type=int, default=20000)
@@ -104,6 +176,9 @@ def main():
         print(json.dumps({"error": "this engine runs one network per GPU on a one-GPU box; "
                                    "--gpus above 1 is not supported (see DESIGN.md for how it would shard)"}))
         sys.exit(2)
+    if args.form == "batched":
+        batched_main(args)
+        return
 
     from braingrid_amd import bghip as B
     from braingrid_amd import host

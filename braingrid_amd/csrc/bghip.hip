@@ -49,6 +49,8 @@ struct bghip_net {
     uint32_t in_phase;     // Poisson input ring: reference delayIdx = (step - in_phase) mod 32
     int32_t *in_idx_host;  // N: delayIdx of neurons without input (their synapse never advances)
     uint8_t *in_mask_host; // N: copy of dn.in_mask
+    DevNet *d_self;        // device copy of dn (dn.self), refreshed when dn changes
+    DevNet dn_shadow;      // dn as last copied to d_self
 };
 
 // Orders the network's own stream after outstanding batch work on it.
@@ -167,6 +169,8 @@ static void plan_net(bghip_net *n, Carve &c)
     c.add(&dn.err, 1);
     c.add(&dn.in_mask, N); c.add(&dn.in_nisi, N); c.add(&dn.in_psr, N); c.add(&dn.in_r, N); c.add(&dn.in_u, N);
     c.add(&dn.in_last, N); c.add(&dn.in_ring, N); c.add(&dn.in_values, N); c.add(&dn.in_shift, N);
+    if (stdp) { c.add(&dn.ev_count, 1); c.add(&dn.ev_list, N); }
+    c.add(&n->d_self, 1);
     c.add(&n->d_out_map, S); c.add(&n->d_out_begin, N); c.add(&n->d_out_count, N); c.add(&n->d_cursor, N);
     c.add(&n->d_stats, 4); c.add(&n->d_total, 1);
 }
@@ -226,6 +230,7 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
         dn.tF[t] = desc->types.F[t];
     }
     dn.ds_typed = 1;
+    dn.self = n->d_self;
     if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(n->arena);
         delete n;
@@ -449,11 +454,24 @@ extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
     const bool per_step = n->kmax <= 1 || sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
     if (per_step) {
         flush_pending(n);
+        const bool stdp = sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
+        if (stdp && memcmp(&dn, &n->dn_shadow, sizeof(DevNet)) != 0) {
+            // the event pass reads the descriptor from device memory; refresh it once nothing
+            // queued on the stream can still be reading the old copy
+            HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+            HIP_TRY(n->ctx, hipMemcpy(n->d_self, &dn, sizeof(DevNet), hipMemcpyHostToDevice));
+            memcpy((void *)&n->dn_shadow, (const void *)&dn, sizeof(DevNet));
+        }
         for (uint64_t i = 0; i < n_steps; i++) {
             uint64_t t = n->step + i;
             launch_neuron_step(dn, n->stream, nk, t);
-            launch_synapse_step(dn, n->stream, sk, t);
-            n->timed_launches += 2;
+            if (stdp) {
+                launch_stdp_step(dn, n->d_self, n->stream, sk, t);
+                n->timed_launches += 3;
+            } else {
+                launch_synapse_step(dn, n->stream, sk, t);
+                n->timed_launches += 2;
+            }
         }
         n->step += n_steps;
     } else {
@@ -827,6 +845,7 @@ struct BatchGroup {
     uint64_t *d_sbase = nullptr;   // slot prefix (nn + 1)
     uint64_t *d_tprev = nullptr;   // flush parameters (nn)
     int *d_sel = nullptr;
+    uint32_t *d_ev = nullptr;      // per-step STDP: event counter (1) + event list (total)
     uint32_t total = 0;
     int kmax = 1;
 };
@@ -847,6 +866,7 @@ static void free_groups(bghip_batch *b)
         (void)hipFree(g.d_sbase);
         (void)hipFree(g.d_tprev);
         (void)hipFree(g.d_sel);
+        (void)hipFree(g.d_ev);
     }
     b->groups.clear();
 }
@@ -921,6 +941,10 @@ extern "C" int bghip_batch_set(bghip_batch *b, bghip_net *const *nets, int n)
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sbase, sizeof(uint64_t) * (nn + 1)));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_tprev, sizeof(uint64_t) * nn));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sel, sizeof(int) * nn));
+        if (g.per_step) {
+            HIP_TRY(b->ctx, hipMalloc((void **)&g.d_ev, sizeof(uint32_t) * ((size_t)g.total + 1)));
+            HIP_TRY(b->ctx, hipMemset(g.d_ev, 0, sizeof(uint32_t)));
+        }
         HIP_TRY(b->ctx, hipMemcpy(g.d_base, base.data(), sizeof(uint32_t) * (nn + 1), hipMemcpyHostToDevice));
         HIP_TRY(b->ctx, hipMemcpy(g.d_sbase, sbase.data(), sizeof(uint64_t) * (nn + 1), hipMemcpyHostToDevice));
     }
@@ -954,7 +978,7 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
         HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
         if (g.per_step) {
             for (uint64_t s = 0; s < n_steps; s++)
-                launch_step_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, g.sk, g.w32, s);
+                launch_step_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, g.sk, g.w32, s, g.d_ev);
             for (int k = 0; k < nn; k++) {
                 bghip_net *x = b->nets[g.members[k]];
                 x->step += n_steps;

@@ -68,6 +68,10 @@ struct DevNet {
     int32_t *in_shift;           // Regular: nShiftValues
     int in_dur, in_cycle;        // Regular: nStepsDuration, nStepsCycle
     uint64_t in_cyc_off;         // Regular: nStepsInCycle at step t is (t + in_cyc_off) mod in_cycle
+    // per-step STDP path: destinations whose synapses have a spike event this step
+    uint32_t *ev_count;          // 1 counter, zeroed by the step's neuron kernel
+    uint32_t *ev_list;           // N destination indices
+    const DevNet *self;          // this descriptor in device memory (for kernels that call out of line)
 };
 
 enum { BG_INPUT_NONE = 0, BG_INPUT_POISSON = 1, BG_INPUT_REGULAR = 2 };
@@ -260,8 +264,9 @@ __device__ __forceinline__ void stdp_learning(const DevNet &n, uint64_t slot, do
 
 // The learning part of AllSTDPSynapses::advanceSynapse (Synapses/AllSTDPSynapses.cpp:289-407).
 // The pre-spike walk over the post neuron's history computes (spikeHistory - (int64_t)step) in
-// unsigned 64-bit arithmetic, exactly as the reference does; both walks are bounded by the ring
-// length max_spikes, beyond which the reference would index outside the history ring.
+// unsigned 64-bit arithmetic, exactly as the reference does, so that walk never meets its
+// `delta <= -3 tauneg` exit and runs over the whole history ring; both walks are bounded by the
+// ring length max_spikes, beyond which the reference would index outside the history ring.
 template <int SK>
 __device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, int type, uint64_t t, bool fPre, bool fPost,
                                          int idxPre, int idxPost, int total_delay, float psr, float decay)
@@ -292,7 +297,17 @@ __device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, int typ
             } else {
                 epost = 1.0f;
             }
+            const uint32_t w0 = __float_as_uint(n.W[slot]);
             stdp_learning(n, slot, (double)delta, (double)epost, (double)epre);
+            // An entry older than step t has an interval that wraps in the unsigned arithmetic to
+            // (float)2^64 * dt, and so does every older entry after it (all are earlier steps, and
+            // t < 2^39 keeps the float conversion at 2^64). With that delta, dw is P(W) * 0, so a
+            // pass that left W bit-identical (and not NaN) leaves it so for every older entry:
+            // the rest of the walk cannot change anything and is skipped.
+            if (sh < t && t < (1ull << 39)) {
+                const float w1 = n.W[slot];
+                if (__float_as_uint(w1) == w0 && w1 == w1) break;
+            }
         }
         psr = change_psr<SK == SK_DYNSTDP ? SK_DS : SK_SPIKING>(n, slot, type, t, psr, decay);
     }

@@ -29,6 +29,7 @@ template <int NK>
 __global__ __launch_bounds__(BG_BLOCK) void k_neuron_step(DevNet n, uint64_t t)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && n.ev_count) *n.ev_count = 0;   // event list of this step's STDP passes
     if (i >= n.N) return;
     neuron_step_body<NK>(n, i, t);
 }
@@ -87,6 +88,112 @@ __global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
     const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
     if (d >= n.N) return;
     synapse_step_body<SK, W32>(n, d, lane, t);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-step STDP / DynamicSTDP synapse advance in two passes. Spike events (a pre-synaptic spike
+// leaving the delay ring, or the destination's own spike) are rare, but the learning walks and
+// double-precision arithmetic they need would set the register budget of the whole kernel. The
+// streaming pass therefore does the delay-ring inserts, the ring test and the PSR decay of every
+// event-free synapse and completes the summation of every destination without events; a
+// destination with at least one event is appended to a list and left to the event pass, which
+// handles its event synapses (learning, PSR update, decay) exactly as the one-pass body and then
+// sums all its synapses in the canonical order. Event synapses keep their ring bit set until the
+// event pass, so both passes classify them the same way.
+template <int SK, int W32>
+__device__ __forceinline__ bool stdp_stream_body(const DevNet &n, int d, int lane, uint64_t t)
+{
+    const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
+    const bool dense = (cnt == hi);
+    const bool post = n.fired_step[d] == t;
+    const int R = n.R;
+    const uint32_t tb = (uint32_t)(t & (uint64_t)(R - 1));
+    const uint64_t wbase = (uint64_t)(tb >> 5) * n.S;
+    const uint32_t m = 1u << (tb & 31);
+    float acc = 0.0f;
+    bool ev = post && cnt > 0;
+    for (uint32_t q = lane; q < cnt; q += BG_WAVE) {
+        const uint64_t slot = dense ? (uint64_t)d * n.M + q : (uint64_t)n.in_map[n.in_begin[d] + q];
+        const uint32_t meta = n.meta[slot];
+        const int src = (int)(meta & BG_META_SRC_MASK);
+        if (n.fired_step[src] == t) {   // preSpikeHit at the firing step
+            const int delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
+            const uint32_t b = (tb + (uint32_t)delay) & (uint32_t)(R - 1);
+            uint32_t *w = &n.ring[(uint64_t)(b >> 5) * n.S + slot];
+            const uint32_t mb = 1u << (b & 31);
+            if (*w & mb) dev_error(n, 1u);
+            *w |= mb;
+        }
+        if (post) continue;
+        if (n.ring[wbase + slot] & m) {
+            ev = true;
+            continue;
+        }
+        const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
+        const float psr = n.psr[slot] * n.decay[type];
+        n.psr[slot] = psr;
+        acc += psr;
+    }
+    if (__ballot(ev) != 0) return true;
+    acc = wave_sum(acc);
+    if (lane == 0) n.sp[d] = n.sp[d] + acc;
+    return false;
+}
+
+template <int SK, int W32>
+__device__ __forceinline__ void stdp_event_body(const DevNet &n, int d, int lane, uint64_t t)
+{
+    const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
+    const bool dense = (cnt == hi);
+    const bool post = n.fired_step[d] == t;
+    const int R = n.R;
+    const uint32_t tb = (uint32_t)(t & (uint64_t)(R - 1));
+    const uint64_t wbase = (uint64_t)(tb >> 5) * n.S;
+    const uint32_t m = 1u << (tb & 31);
+    float acc = 0.0f;
+    for (uint32_t q = lane; q < cnt; q += BG_WAVE) {
+        const uint64_t slot = dense ? (uint64_t)d * n.M + q : (uint64_t)n.in_map[n.in_begin[d] + q];
+        uint32_t *w = &n.ring[wbase + slot];
+        const uint32_t word = *w;
+        const bool pre = (word & m) != 0;
+        float psr = n.psr[slot];
+        if (pre || post) {
+            if (pre) *w = word & ~m;
+            const uint32_t meta = n.meta[slot];
+            const int src = (int)(meta & BG_META_SRC_MASK);
+            const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
+            const int delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
+            const float decay = n.decay[type];
+            psr = stdp_event<SK>(n, slot, type, t, pre, post, src, d, delay, psr, decay);
+            psr = psr * decay;
+            n.psr[slot] = psr;
+        }
+        acc += psr;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) n.sp[d] = n.sp[d] + acc;
+}
+
+template <int SK, int W32>
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream(DevNet n, uint64_t t)
+{
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
+    if (d >= n.N) return;
+    if (stdp_stream_body<SK, W32>(n, d, lane, t) && lane == 0) n.ev_list[atomicAdd(n.ev_count, 1u)] = (uint32_t)d;
+}
+
+// The descriptor is read from device memory: the out-of-line learning call takes its address,
+// which for a by-value kernel argument would force a private copy of it in every lane.
+template <int SK, int W32>
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_events(const DevNet *np, uint64_t t)
+{
+    const DevNet &n = *np;
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    const uint32_t nw = gridDim.x * (BG_BLOCK / BG_WAVE);
+    const uint32_t c = *n.ev_count;
+    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / BG_WAVE; i < c; i += nw)
+        stdp_event_body<SK, W32>(n, (int)n.ev_list[i], lane, t);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -314,9 +421,10 @@ __global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
 
 template <int NK>
 __global__ __launch_bounds__(BG_BLOCK) void k_neuron_batch(const BatchNet *nets, const uint32_t *nbase, int nn,
-                                                           uint64_t step)
+                                                           uint64_t step, uint32_t *ev_count)
 {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w == 0 && ev_count) *ev_count = 0;
     if (w >= nbase[nn]) return;
     const int i = batch_find(nbase, nn, w);
     const BatchNet &b = nets[i];
@@ -335,3 +443,33 @@ __global__ __launch_bounds__(BG_BLOCK) void k_synapse_batch(const BatchNet *nets
     synapse_step_body<SK, W32>(b.n, (int)(w - wbase[i]), threadIdx.x & (BG_WAVE - 1), b.t0 + step);
 }
 
+// Two-pass STDP step over a batch: the event list holds batch work-item indices.
+template <int SK, int W32>
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
+                                                                uint64_t step, uint32_t *ev_count, uint32_t *ev_list)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(
+        (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
+    if (w >= wbase[nn]) return;
+    const int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, w));
+    const BatchNet &b = nets[i];
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    if (stdp_stream_body<SK, W32>(b.n, (int)(w - wbase[i]), lane, b.t0 + step) && lane == 0)
+        ev_list[atomicAdd(ev_count, 1u)] = w;
+}
+
+template <int SK, int W32>
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_events_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
+                                                                uint64_t step, const uint32_t *ev_count,
+                                                                const uint32_t *ev_list)
+{
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    const uint32_t nw = gridDim.x * (BG_BLOCK / BG_WAVE);
+    const uint32_t c = *ev_count;
+    for (uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) / BG_WAVE; k < c; k += nw) {
+        const uint32_t w = __builtin_amdgcn_readfirstlane(ev_list[k]);
+        const int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, w));
+        const BatchNet &b = nets[i];
+        stdp_event_body<SK, W32>(b.n, (int)(w - wbase[i]), lane, b.t0 + step);
+    }
+}

@@ -1,6 +1,8 @@
-// Per-step neuron and synapse kernels (see k_neuron_step / k_synapse_step in bghip_kernels.h).
+// Per-step neuron and synapse kernels (see k_neuron_step / k_synapse_step / k_stdp_* in
+// bghip_kernels.h).
 #include "bghip_kernels.h"
 #include "bghip_launch.h"
+#include <algorithm>
 
 void launch_neuron_step(const DevNet &dn, hipStream_t s, int nk, uint64_t t)
 {
@@ -34,4 +36,37 @@ void launch_synapse_step(const DevNet &dn, hipStream_t s, int sk, uint64_t t)
     case SK_STDP: go_w<SK_STDP>(dn, s, t); break;
     default: go_w<SK_DYNSTDP>(dn, s, t); break;
     }
+}
+
+// Event pass grid: enough waves to keep every CU busy when many destinations have events, few
+// enough that an event-free step costs one short launch.
+static unsigned event_blocks(uint64_t waves_needed)
+{
+    const uint64_t cap = 1024;   // 4096 waves
+    const uint64_t b = (waves_needed + (BG_BLOCK / BG_WAVE) - 1) / (BG_BLOCK / BG_WAVE);
+    return (unsigned)std::max<uint64_t>(1, std::min(cap, b));
+}
+
+template <int SK, int W32>
+static void stdp_go(const DevNet &dn, const DevNet *d_self, hipStream_t s, uint64_t t)
+{
+    unsigned blocks = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
+    hipLaunchKernelGGL((k_stdp_stream<SK, W32>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t);
+    hipLaunchKernelGGL((k_stdp_events<SK, W32>), dim3(event_blocks((uint64_t)dn.N)), dim3(BG_BLOCK), 0, s, d_self, t);
+}
+template <int SK>
+static void stdp_go_w(const DevNet &dn, const DevNet *d_self, hipStream_t s, uint64_t t)
+{
+    switch (dn.W32) {
+    case 1: stdp_go<SK, 1>(dn, d_self, s, t); break;
+    case 2: stdp_go<SK, 2>(dn, d_self, s, t); break;
+    case 4: stdp_go<SK, 4>(dn, d_self, s, t); break;
+    default: stdp_go<SK, 8>(dn, d_self, s, t); break;
+    }
+}
+
+void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int sk, uint64_t t)
+{
+    if (sk == SK_STDP) stdp_go_w<SK_STDP>(dn, d_self, s, t);
+    else stdp_go_w<SK_DYNSTDP>(dn, d_self, s, t);
 }

@@ -1,0 +1,163 @@
+"""Parameter sweeps of many independent networks: the batched form.
+
+A sweep is a list of network specifications (grid side, family, growth target rate tR, excitatory
+fraction fE, seed, simulated seconds). `build(ctx, spec)` creates one network on the device;
+`run_alone` advances the networks one after another with their own launches, `run_batched`
+advances all of them together with shared launches (bghip_batch_*) and does the per-network
+epoch work (growth update, history) in between. Both produce bit-identical network states.
+
+Families:
+  lif_growth -- AllLIFNeurons + AllDSSynapses + ConnGrowth with the test-small parameter
+                distributions (configfiles/test-small.xml), DynamicLayout with fraction fE and 10%
+                starters, target rate tR, 1 s epochs, each followed by the growth update and the
+                growth recorder's history step;
+  izh_stdp   -- the static Izhikevich + DynamicSTDP network of braingrid_amd.networks with
+                min(100, N - 1) outgoing synapses per neuron and a 10 Hz Poisson input of weight 25
+                (configfiles/SInputPoisson/i_10hz_w25.xml) into every neuron; 1 s epochs, each
+                followed by the history step.
+"""
+from __future__ import annotations
+
+import hashlib
+import time
+
+import numpy as np
+
+from . import bghip as B
+from . import host
+from . import networks as NW
+
+TR_CHOICES = (0.1, 1.0, 1.9)
+FE_CHOICES = (0.80, 0.90, 0.98)
+EPOCH_S = 1.0
+DT = 1e-4
+LIF_M = 200
+LIF_MAX_RATE = 200
+IZH_MAX_RATE = 400
+POISSON_HZ, POISSON_WEIGHT = 10.0, 25.0
+
+
+def net_seed(global_seed: int, net_id: int) -> int:
+    h = hashlib.blake2b(f"{global_seed}:{net_id}".encode(), digest_size=4).digest()
+    return int.from_bytes(h, "little") or 1
+
+
+def sweep_specs(n_nets: int = 256, seed: int = 2024, side_lo: int = 10, side_hi: int = 120, sim_s: float = 5.0,
+                frac_lif: float = 0.5):
+    """Sides log-uniform on [side_lo, side_hi]; families alternate by a seeded draw."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(n_nets):
+        side = int(round(float(np.exp(rng.uniform(np.log(side_lo), np.log(side_hi))))))
+        kind = "lif_growth" if rng.random() < frac_lif else "izh_stdp"
+        specs.append(dict(id=k, side=side, kind=kind, tR=float(rng.choice(TR_CHOICES)),
+                          fE=float(rng.choice(FE_CHOICES)), seed=net_seed(seed, k), sim_s=float(sim_s)))
+    return specs
+
+
+class SweepNet:
+    """One network of a sweep with its per-epoch work."""
+
+    def __init__(self, ctx: B.Context, spec: dict):
+        self.spec = spec
+        self.kind = spec["kind"]
+        self.N = spec["side"] ** 2
+        self.epochs = int(round(spec["sim_s"] / EPOCH_S))
+        self.steps_per_epoch = int(round(EPOCH_S / DT))
+        self.model = None
+        if self.kind == "lif_growth":
+            c = host.base_config()
+            c.update(width=spec["side"], height=spec["side"], layout="DynamicLayout", frac_exc=spec["fE"],
+                     frac_starter=0.1, target_rate=spec["tR"], seed=spec["seed"], epoch_duration=EPOCH_S,
+                     num_epochs=self.epochs, max_synapses_per_neuron=LIF_M, max_firing_rate=LIF_MAX_RATE)
+            self.model = host.Model(ctx, c)
+            self.net = self.model.net
+            self.max_spikes = int(np.float32(EPOCH_S) * np.float32(LIF_MAX_RATE))
+        else:
+            k = min(100, self.N - 1)
+            self.net, meta = NW.izh_stdp(ctx, self.N, spec["seed"], k=k, max_rate=IZH_MAX_RATE, epoch_s=EPOCH_S)
+            self.net.upload_input(B.INPUT_POISSON, neuron_type=meta["ntype"], fr_mean=POISSON_HZ,
+                                  weight=POISSON_WEIGHT)
+            self.max_spikes = meta["max_spikes"]
+        self.synapses = self.net.total_synapses
+
+    def end_epoch(self):
+        if self.model is not None:
+            self.model.update_connections()
+            self.model.update_history()
+        else:
+            self.net.update_history()
+
+    def fingerprint(self) -> str:
+        """Digest of the network's full neuron state and synapse weights/PSRs."""
+        h = hashlib.blake2b(digest_size=16)
+        neu = self.net.download_neurons(self.max_spikes)
+        for f in ("Vm", "spikeCount", "spikeCountOffset", "spike_history", "nStepsInRefr"):
+            h.update(np.ascontiguousarray(neu[f]).tobytes())
+        syn = self.net.download_synapses()
+        for f in ("in_use", "W", "psr"):
+            h.update(np.ascontiguousarray(syn[f]).tobytes())
+        if self.model is not None:
+            r, q = self.net.download_growth()
+            h.update(r.tobytes())
+            h.update(q.tobytes())
+        return h.hexdigest()
+
+    def close(self):
+        if self.model is not None:
+            self.model.close()
+        else:
+            self.net.close()
+
+
+def build(ctx: B.Context, specs, log=None):
+    nets = []
+    for s in specs:
+        nets.append(SweepNet(ctx, s))
+        if log and len(nets) % 32 == 0:
+            log(f"built {len(nets)}/{len(specs)} networks")
+    return nets
+
+
+def run_alone(nets, log=None) -> float:
+    """Each network on its own, one after another (plain per-network launches). Returns seconds."""
+    for x in nets:
+        x.net.sync()
+    t0 = time.perf_counter()
+    for k, x in enumerate(nets):
+        for _ in range(x.epochs):
+            x.net.advance(x.steps_per_epoch)
+            x.end_epoch()
+        x.net.sync()
+        if log and (k + 1) % 32 == 0:
+            log(f"alone: {k + 1}/{len(nets)} networks done, {time.perf_counter() - t0:.1f} s")
+    return time.perf_counter() - t0
+
+
+def run_batched(ctx: B.Context, nets, log=None) -> float:
+    """All networks advanced by shared launches, epoch by epoch. Returns seconds."""
+    for x in nets:
+        x.net.sync()
+    batch = B.Batch(ctx)
+    t0 = time.perf_counter()
+    live = [x for x in nets if x.epochs > 0]
+    epoch = 0
+    members = None
+    while live:
+        if members != len(live):
+            batch.set([x.net for x in live])
+            members = len(live)
+        steps = live[0].steps_per_epoch
+        batch.advance(steps)
+        for x in live:
+            x.end_epoch()
+        epoch += 1
+        live = [x for x in live if x.epochs > epoch]
+        if log:
+            log(f"batched: epoch {epoch} done, {time.perf_counter() - t0:.1f} s")
+    batch.sync()
+    for x in nets:
+        x.net.sync()
+    wall = time.perf_counter() - t0
+    batch.close()
+    return wall

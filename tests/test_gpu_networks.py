@@ -30,3 +30,31 @@ def test_izh_dynstdp_static_parity(hip_ctx, N, k, seed, chunks):
     finally:
         net.close()
         ora.close()
+
+
+def test_izh_dynstdp_multi_epoch_full_rings(hip_ctx):
+    """Several epochs with a short spike-history ring, so that the rings fill and wrap and the
+    pre-spike learning walks cross stale entries of earlier epochs: the device's early exit from
+    those walks must leave every weight exactly as the oracle's full walk does."""
+    N, k, seed = 300, 30, 21
+    print(f"\n[izh_dynstdp_multi_epoch N={N} k={k} seed={seed}]", flush=True)
+    epoch_s, max_rate = 0.5, 80
+    neu, syn, meta = NW.izh_stdp_arrays(N, seed, k=k, max_rate=max_rate, epoch_s=epoch_s)
+    ora, cfg = H.izh_static_oracle(neu, syn, meta, seed, epoch_s=epoch_s, max_rate=max_rate)
+    net, _ = NW.izh_stdp(hip_ctx, N, seed, k=k, max_rate=max_rate, epoch_s=epoch_s)
+    steps = int(round(epoch_s / 1e-4))
+    try:
+        wrapped = False
+        for ep in range(5):
+            ora.advance(steps)
+            net.advance(steps)
+            wrapped |= bool((ora.array("spikeCount") + ora.array("spikeCountOffset") >= meta["max_spikes"]).any())
+            bad = H.compare_state(ora, net, cfg, f"epoch{ep}")
+            assert not bad, bad
+            ora.update_history()
+            net.update_history()
+        assert wrapped, "no history ring wrapped; the test does not reach the stale-entry walks"
+        assert net.error_bits() == 0
+    finally:
+        net.close()
+        ora.close()
