@@ -50,6 +50,7 @@ struct bghip_net {
     int32_t *in_idx_host;  // N: delayIdx of neurons without input (their synapse never advances)
     uint8_t *in_mask_host; // N: copy of dn.in_mask
     DevNet *d_self;        // device copy of dn (dn.self), refreshed when dn changes
+    uint32_t *d_ev_off;    // per-step STDP: shard offsets of the event lists (dest, then syn)
     DevNet dn_shadow;      // dn as last copied to d_self
 };
 
@@ -143,6 +144,35 @@ struct Carve {
     }
 };
 
+// Segment offsets of the sharded event lists for destinations (work items) [base_i, base_i + N_i)
+// with M_i slots each: shard s holds the work items w = s (mod BG_EV_SHARDS). Returns false when
+// the synapse list would not be addressable with 32-bit offsets.
+struct EvPart { uint64_t base, N, M; };
+static bool ev_offsets(const std::vector<EvPart> &parts, std::vector<uint32_t> &dest_off, std::vector<uint32_t> &syn_off)
+{
+    const uint64_t NS = BG_EV_SHARDS;
+    std::vector<uint64_t> dc(NS, 0), sc(NS, 0);
+    auto below = [&](uint64_t x, uint64_t r) { return (x + NS - 1 - r) / NS; };   // #{w < x : w = r mod NS}
+    for (const EvPart &p : parts)
+        for (uint64_t r = 0; r < NS; r++) {
+            const uint64_t k = below(p.base + p.N, r) - below(p.base, r);
+            dc[r] += k;
+            sc[r] += k * p.M;
+        }
+    dest_off.assign(NS + 1, 0);
+    syn_off.assign(NS + 1, 0);
+    uint64_t ad = 0, as = 0;
+    for (uint64_t r = 0; r < NS; r++) {
+        dest_off[r] = (uint32_t)ad;
+        syn_off[r] = (uint32_t)as;
+        ad += dc[r];
+        as += sc[r];
+    }
+    dest_off[NS] = (uint32_t)ad;
+    syn_off[NS] = (uint32_t)as;
+    return as < (1ull << 32);
+}
+
 static void plan_net(bghip_net *n, Carve &c)
 {
     const bghip_net_desc &d = n->desc;
@@ -169,7 +199,10 @@ static void plan_net(bghip_net *n, Carve &c)
     c.add(&dn.err, 1);
     c.add(&dn.in_mask, N); c.add(&dn.in_nisi, N); c.add(&dn.in_psr, N); c.add(&dn.in_r, N); c.add(&dn.in_u, N);
     c.add(&dn.in_last, N); c.add(&dn.in_ring, N); c.add(&dn.in_values, N); c.add(&dn.in_shift, N);
-    if (stdp) { c.add(&dn.ev_count, 1); c.add(&dn.ev_list, N); }
+    if (stdp) {
+        c.add(&dn.ev.count, 2 * BG_EV_SHARDS * BG_EV_PAD); c.add(&n->d_ev_off, 2 * (BG_EV_SHARDS + 1));
+        c.add(&dn.ev.dest, N); c.add(&dn.ev.syn, S);
+    }
     c.add(&n->d_self, 1);
     c.add(&n->d_out_map, S); c.add(&n->d_out_begin, N); c.add(&n->d_out_count, N); c.add(&n->d_cursor, N);
     c.add(&n->d_stats, 4); c.add(&n->d_total, 1);
@@ -231,6 +264,11 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
     }
     dn.ds_typed = 1;
     dn.self = n->d_self;
+    dn.out_begin = n->d_out_begin;
+    dn.out_count = n->d_out_count;
+    dn.out_map = n->d_out_map;
+    dn.ev.dest_off = n->d_ev_off;
+    dn.ev.syn_off = n->d_ev_off ? n->d_ev_off + BG_EV_SHARDS + 1 : nullptr;
     if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(n->arena);
         delete n;
@@ -243,6 +281,14 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
     HIP_TRY(ctx, hipMemsetAsync(n->arena, 0, n->arena_bytes, n->stream));
     HIP_TRY(ctx, hipMemsetAsync(dn.fired_step, 0xFF, sizeof(uint64_t) * dn.N, n->stream));
     HIP_TRY(ctx, hipMemsetAsync(dn.hist, 0xFF, sizeof(uint64_t) * (size_t)dn.N * dn.max_spikes, n->stream));
+    std::vector<uint32_t> doff, soff;
+    if (n->d_ev_off) {
+        if (!ev_offsets({{0, (uint64_t)dn.N, (uint64_t)dn.M}}, doff, soff))
+            return set_err(ctx, BGHIP_EUNSUPPORTED, "network too large for the event lists");
+        doff.insert(doff.end(), soff.begin(), soff.end());
+        HIP_TRY(ctx, hipMemcpyAsync(n->d_ev_off, doff.data(), sizeof(uint32_t) * doff.size(), hipMemcpyHostToDevice,
+                                    n->stream));
+    }
     HIP_TRY(ctx, hipStreamSynchronize(n->stream));
     n->kmax = block_kmax(n);
     *out = n;
@@ -304,6 +350,15 @@ extern "C" int bghip_upload_neurons(bghip_net *n, const bghip_neurons *h)
     CHK(h2d(n, dn.sp, h->summation_map, N)); CHK(h2d(n, dn.refr, h->nStepsInRefr, N));
     CHK(h2d(n, dn.cnt, h->spikeCount, N)); CHK(h2d(n, dn.off, h->spikeCountOffset, N));
     CHK(h2d(n, dn.hist, h->spike_history, N * (size_t)dn.max_spikes));
+    // last firing step = the newest history entry (getSpikeHistory(i, -1))
+    std::vector<uint64_t> last(N, BG_NO_SPIKE);
+    if (h->spike_history && h->spikeCount && h->spikeCountOffset && dn.max_spikes > 0)
+        for (size_t i = 0; i < N; i++) {
+            const int ms = dn.max_spikes;
+            const int k = (int)(((int64_t)h->spikeCount[i] + h->spikeCountOffset[i] + ms - 1) % ms);
+            last[i] = h->spike_history[i * (size_t)ms + (size_t)k];
+        }
+    CHK(h2d(n, dn.fired_step, last.data(), N));
     if (izh) {
         CHK(h2d(n, dn.A, h->Aconst, N)); CHK(h2d(n, dn.B, h->Bconst, N)); CHK(h2d(n, dn.C, h->Cconst, N));
         CHK(h2d(n, dn.D, h->Dconst, N)); CHK(h2d(n, dn.u, h->u, N)); CHK(h2d(n, dn.C3, h->C3, N));
@@ -322,6 +377,35 @@ static int rebuild_incoming(bghip_net *n)
     HIP_TRY(n->ctx, hipMemcpyAsync(&n->total, n->d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, n->stream));
     HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
     n->out_valid = false;
+    return BGHIP_OK;
+}
+
+// The outgoing map (Core/SynapseIndexMap.h: outgoingSynapseIndexMap/Begin/Count) from the incoming
+// one: count per source, scan, place, then sort each source's segment ascending (the reference's
+// map is stable in iSyn order). Built when first needed after the synapse table changed: by the
+// per-step STDP path, which delivers spikes through it, or by an index-map download.
+static int build_outgoing(bghip_net *n)
+{
+    DevNet &dn = n->dn;
+    const size_t N = dn.N, T = n->total;
+    if (n->out_valid) return BGHIP_OK;
+    HIP_TRY(n->ctx, hipMemsetAsync(n->d_out_count, 0, 4 * N, n->stream));
+    HIP_TRY(n->ctx, hipMemsetAsync(n->d_cursor, 0, 4 * N, n->stream));
+    if (T > 0) {
+        unsigned nb = (unsigned)((T + BG_BLOCK - 1) / BG_BLOCK);
+        hipLaunchKernelGGL(k_count_outgoing, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, dn.in_map, (uint64_t)T,
+                           n->d_out_count);
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, n->stream, n->d_out_count, n->d_out_begin, dn.N,
+                           n->d_total);
+        hipLaunchKernelGGL(k_place_outgoing, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, dn.in_map, (uint64_t)T,
+                           n->d_out_begin, n->d_cursor, n->d_out_map);
+        hipLaunchKernelGGL(k_sort_outgoing, dim3(dn.N), dim3(BG_BLOCK), 0, n->stream, n->d_out_begin,
+                           n->d_out_count, n->d_out_map);
+    } else {
+        HIP_TRY(n->ctx, hipMemsetAsync(n->d_out_begin, 0, 4 * N, n->stream));
+    }
+    HIP_TRY(n->ctx, hipGetLastError());
+    n->out_valid = true;
     return BGHIP_OK;
 }
 
@@ -452,9 +536,20 @@ extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
     DevNet &dn = n->dn;
     const int nk = n->desc.neuron_model, sk = n->desc.synapse_model;
     const bool per_step = n->kmax <= 1 || sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
-    if (per_step) {
+    if (n->total == 0) {
+        // no synapses: neurons only, many steps per launch
+        flush_pending(n);
+        for (uint64_t done = 0; done < n_steps;) {
+            const int k = (int)std::min<uint64_t>(BG_QUIET_STEPS, n_steps - done);
+            launch_neuron_only(dn, n->stream, nk, n->step + done, k);
+            done += (uint64_t)k;
+            n->timed_launches += 1;
+        }
+        n->step += n_steps;
+    } else if (per_step) {
         flush_pending(n);
         const bool stdp = sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
+        if (stdp) CHK(build_outgoing(n));
         if (stdp && memcmp(&dn, &n->dn_shadow, sizeof(DevNet)) != 0) {
             // the event pass reads the descriptor from device memory; refresh it once nothing
             // queued on the stream can still be reading the old copy
@@ -464,11 +559,11 @@ extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
         }
         for (uint64_t i = 0; i < n_steps; i++) {
             uint64_t t = n->step + i;
-            launch_neuron_step(dn, n->stream, nk, t);
             if (stdp) {
-                launch_stdp_step(dn, n->d_self, n->stream, sk, t);
-                n->timed_launches += 3;
+                launch_stdp_step(dn, n->d_self, n->stream, nk, sk, t);
+                n->timed_launches += 4;
             } else {
+                launch_neuron_step(dn, n->stream, nk, t);
                 launch_synapse_step(dn, n->stream, sk, t);
                 n->timed_launches += 2;
             }
@@ -663,21 +758,7 @@ extern "C" int bghip_download_index_map(bghip_net *n, bghip_index_map *h)
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     DevNet &dn = n->dn;
     size_t N = dn.N, T = n->total;
-    if (!n->out_valid && T > 0) {
-        HIP_TRY(n->ctx, hipMemsetAsync(n->d_out_count, 0, 4 * N, n->stream));
-        HIP_TRY(n->ctx, hipMemsetAsync(n->d_cursor, 0, 4 * N, n->stream));
-        unsigned nb = (unsigned)((T + BG_BLOCK - 1) / BG_BLOCK);
-        hipLaunchKernelGGL(k_count_outgoing, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, dn.in_map, (uint64_t)T,
-                           n->d_out_count);
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, n->stream, n->d_out_count, n->d_out_begin, dn.N,
-                           n->d_total);
-        hipLaunchKernelGGL(k_place_outgoing, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, dn.in_map, (uint64_t)T,
-                           n->d_out_begin, n->d_cursor, n->d_out_map);
-        hipLaunchKernelGGL(k_sort_outgoing, dim3(dn.N), dim3(BG_BLOCK), 0, n->stream, n->d_out_begin,
-                           n->d_out_count, n->d_out_map);
-        HIP_TRY(n->ctx, hipGetLastError());
-        n->out_valid = true;
-    }
+    CHK(build_outgoing(n));
     std::vector<uint32_t> cnt(N);
     CHK(d2h(n, h->incomingSynapseIndexMap, dn.in_map, T)); CHK(d2h(n, h->incomingSynapseBegin, dn.in_begin, N));
     CHK(d2h(n, cnt.data(), dn.syn_count, N));
@@ -836,16 +917,27 @@ extern "C" int bghip_download_input(bghip_net *n, bghip_input_state *st)
 
 // ------------------------------------------------------------------------------------------------
 // Batched form.
+// How a network advances: neurons only (no synapses), fused temporal blocks, or per step.
+enum { BG_MODE_QUIET = 0, BG_MODE_FUSED = 1, BG_MODE_STEP = 2 };
+static int net_mode(const bghip_net *x)
+{
+    if (x->total == 0) return BG_MODE_QUIET;
+    const int sk = x->desc.synapse_model;
+    return (x->kmax <= 1 || sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP) ? BG_MODE_STEP : BG_MODE_FUSED;
+}
+
 struct BatchGroup {
     int nk, sk, w32;
-    bool per_step;
+    int mode;
     std::vector<int> members;      // indices into bghip_batch::nets
     BatchNet *d_nets = nullptr;    // device table (nn entries)
     uint32_t *d_base = nullptr;    // neuron prefix (nn + 1)
     uint64_t *d_sbase = nullptr;   // slot prefix (nn + 1)
     uint64_t *d_tprev = nullptr;   // flush parameters (nn)
     int *d_sel = nullptr;
-    uint32_t *d_ev = nullptr;      // per-step STDP: event counter (1) + event list (total)
+    EvLists ev = {};               // per-step STDP: sharded event lists of the group
+    uint32_t *d_ev = nullptr;      // their counters, offsets and destination list in one allocation
+    uint2 *d_evs = nullptr;        // their synapse list (sum of the members' slots)
     uint32_t total = 0;
     int kmax = 1;
 };
@@ -856,6 +948,7 @@ struct bghip_batch {
     hipEvent_t done;
     std::vector<bghip_net *> nets;
     std::vector<BatchGroup> groups;
+    std::vector<int> modes;        // net_mode of each member when the groups were formed
 };
 
 static void free_groups(bghip_batch *b)
@@ -867,6 +960,7 @@ static void free_groups(bghip_batch *b)
         (void)hipFree(g.d_tprev);
         (void)hipFree(g.d_sel);
         (void)hipFree(g.d_ev);
+        (void)hipFree(g.d_evs);
     }
     b->groups.clear();
 }
@@ -898,27 +992,29 @@ extern "C" void bghip_batch_destroy(bghip_batch *b)
     delete b;
 }
 
-extern "C" int bghip_batch_set(bghip_batch *b, bghip_net *const *nets, int n)
+// Groups the members by (neuron model, synapse model, ring words, mode); each group advances with
+// shared launches and its own device tables.
+static int regroup(bghip_batch *b)
 {
-    if (!b || n < 0 || (n > 0 && !nets)) return BGHIP_EINVAL;
-    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
     HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
     free_groups(b);
-    b->nets.assign(nets, nets + n);
+    const int n = (int)b->nets.size();
+    b->modes.assign(n, 0);
     for (int i = 0; i < n; i++) {
-        bghip_net *x = nets[i];
-        if (!x || x->ctx->device != b->ctx->device) return set_err(b->ctx, BGHIP_EINVAL, "batch member %d invalid", i);
-        const int sk = x->desc.synapse_model;
-        const bool per_step = x->kmax <= 1 || sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
+        bghip_net *x = b->nets[i];
+        const int mode = net_mode(x);
+        b->modes[i] = mode;
+        const int sk = mode == BG_MODE_QUIET ? -1 : x->desc.synapse_model;
+        const int w32 = mode == BG_MODE_QUIET ? 0 : x->dn.W32;
         BatchGroup *grp = nullptr;
         for (auto &g : b->groups)
-            if (g.nk == x->desc.neuron_model && g.sk == sk && g.w32 == x->dn.W32 && g.per_step == per_step) grp = &g;
+            if (g.nk == x->desc.neuron_model && g.sk == sk && g.w32 == w32 && g.mode == mode) grp = &g;
         if (!grp) {
             BatchGroup g;
             g.nk = x->desc.neuron_model;
             g.sk = sk;
-            g.w32 = x->dn.W32;
-            g.per_step = per_step;
+            g.w32 = w32;
+            g.mode = mode;
             g.kmax = BG_KMAX;
             b->groups.push_back(g);
             grp = &b->groups.back();
@@ -941,14 +1037,42 @@ extern "C" int bghip_batch_set(bghip_batch *b, bghip_net *const *nets, int n)
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sbase, sizeof(uint64_t) * (nn + 1)));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_tprev, sizeof(uint64_t) * nn));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sel, sizeof(int) * nn));
-        if (g.per_step) {
-            HIP_TRY(b->ctx, hipMalloc((void **)&g.d_ev, sizeof(uint32_t) * ((size_t)g.total + 1)));
-            HIP_TRY(b->ctx, hipMemset(g.d_ev, 0, sizeof(uint32_t)));
+        if (g.mode == BG_MODE_STEP && (g.sk == BGHIP_SYN_STDP || g.sk == BGHIP_SYN_DYNSTDP)) {
+            std::vector<EvPart> parts(nn);
+            for (int k = 0; k < nn; k++) {
+                const DevNet &dn = b->nets[g.members[k]]->dn;
+                parts[k] = {base[k], (uint64_t)dn.N, (uint64_t)dn.M};
+            }
+            std::vector<uint32_t> doff, soff;
+            if (!ev_offsets(parts, doff, soff))
+                return set_err(b->ctx, BGHIP_EUNSUPPORTED, "batch too large for the event lists");
+            const size_t nc = 2 * BG_EV_SHARDS * BG_EV_PAD, no = 2 * (BG_EV_SHARDS + 1);
+            HIP_TRY(b->ctx, hipMalloc((void **)&g.d_ev, sizeof(uint32_t) * (nc + no + (size_t)g.total + 1)));
+            HIP_TRY(b->ctx, hipMalloc((void **)&g.d_evs, sizeof(uint2) * (size_t)std::max<uint32_t>(soff[BG_EV_SHARDS], 1)));
+            doff.insert(doff.end(), soff.begin(), soff.end());
+            HIP_TRY(b->ctx, hipMemset(g.d_ev, 0, sizeof(uint32_t) * nc));
+            HIP_TRY(b->ctx, hipMemcpy(g.d_ev + nc, doff.data(), sizeof(uint32_t) * no, hipMemcpyHostToDevice));
+            g.ev.count = g.d_ev;
+            g.ev.dest_off = g.d_ev + nc;
+            g.ev.syn_off = g.d_ev + nc + BG_EV_SHARDS + 1;
+            g.ev.dest = g.d_ev + nc + no;
+            g.ev.syn = g.d_evs;
         }
         HIP_TRY(b->ctx, hipMemcpy(g.d_base, base.data(), sizeof(uint32_t) * (nn + 1), hipMemcpyHostToDevice));
         HIP_TRY(b->ctx, hipMemcpy(g.d_sbase, sbase.data(), sizeof(uint64_t) * (nn + 1), hipMemcpyHostToDevice));
     }
     return BGHIP_OK;
+}
+
+extern "C" int bghip_batch_set(bghip_batch *b, bghip_net *const *nets, int n)
+{
+    if (!b || n < 0 || (n > 0 && !nets)) return BGHIP_EINVAL;
+    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+    for (int i = 0; i < n; i++)
+        if (!nets[i] || nets[i]->ctx->device != b->ctx->device)
+            return set_err(b->ctx, BGHIP_EINVAL, "batch member %d invalid", i);
+    b->nets.assign(nets, nets + n);
+    return regroup(b);
 }
 
 extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
@@ -959,9 +1083,17 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
     // the batch stream waits for every member's own outstanding work
     for (bghip_net *x : b->nets) {
         net_begin(x);
+        if (x->desc.synapse_model == BGHIP_SYN_STDP || x->desc.synapse_model == BGHIP_SYN_DYNSTDP)
+            CHK(build_outgoing(x));
         HIP_TRY(b->ctx, hipEventRecord(x->ext_ev, x->stream));
         HIP_TRY(b->ctx, hipStreamWaitEvent(b->stream, x->ext_ev, 0));
     }
+    // a growth update or an upload may have moved a member to another mode
+    for (size_t i = 0; i < b->nets.size(); i++)
+        if (net_mode(b->nets[i]) != b->modes[i]) {
+            CHK(regroup(b));
+            break;
+        }
     for (auto &g : b->groups) {
         const int nn = (int)g.members.size();
         std::vector<BatchNet> tab(nn);
@@ -976,9 +1108,20 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
         HIP_TRY(b->ctx, hipMemcpyAsync(g.d_nets, tab.data(), sizeof(BatchNet) * nn, hipMemcpyHostToDevice, b->stream));
         // the table copy reads host memory that is reused by the next call: wait for it
         HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
-        if (g.per_step) {
+        if (g.mode == BG_MODE_QUIET) {
+            for (uint64_t done = 0; done < n_steps;) {
+                const int k = (int)std::min<uint64_t>(BG_QUIET_STEPS, n_steps - done);
+                launch_neuron_only_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, done, k);
+                done += (uint64_t)k;
+            }
+            for (int k = 0; k < nn; k++) {
+                bghip_net *x = b->nets[g.members[k]];
+                x->step += n_steps;
+                x->pending = false;
+            }
+        } else if (g.mode == BG_MODE_STEP) {
             for (uint64_t s = 0; s < n_steps; s++)
-                launch_step_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, g.sk, g.w32, s, g.d_ev);
+                launch_step_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, g.sk, g.w32, s, g.ev);
             for (int k = 0; k < nn; k++) {
                 bghip_net *x = b->nets[g.members[k]];
                 x->step += n_steps;

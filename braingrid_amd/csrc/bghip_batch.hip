@@ -58,21 +58,28 @@ static void syn_w(const BatchNet *nets, const uint32_t *base, int nn, uint32_t w
     }
 }
 
+// Event passes: G blocks per event-list shard, more for larger batches (each shard holds about
+// total / BG_EV_SHARDS destinations).
+static unsigned event_grid(uint32_t total, unsigned gmax)
+{
+    const unsigned g = (unsigned)(((uint64_t)total + 16 * BG_EV_SHARDS - 1) / (16 * BG_EV_SHARDS));
+    return BG_EV_SHARDS * std::max(1u, std::min(gmax, g));
+}
+
 template <int SK, int W32>
 static void stdp(const BatchNet *nets, const uint32_t *base, int nn, uint32_t waves, hipStream_t s, uint64_t step,
-                 uint32_t *ev)
+                 const EvLists &ev)
 {
-    unsigned blocks = (unsigned)(((uint64_t)waves * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
-    hipLaunchKernelGGL((k_stdp_stream_batch<SK, W32>), dim3(blocks), dim3(BG_BLOCK), 0, s, nets, base, nn, step, ev,
-                       ev + 1);
-    // event pass: up to 4096 waves looping over the event list
-    unsigned eb = std::min(blocks, 1024u);
-    hipLaunchKernelGGL((k_stdp_events_batch<SK, W32>), dim3(eb), dim3(BG_BLOCK), 0, s, nets, base, nn, step,
-                       (const uint32_t *)ev, (const uint32_t *)(ev + 1));
+    const uint64_t sw = ((uint64_t)waves + BG_STREAM_DESTS - 1) / BG_STREAM_DESTS;
+    const unsigned blocks = (unsigned)((sw * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
+    hipLaunchKernelGGL((k_stdp_stream_batch<SK, W32>), dim3(blocks), dim3(BG_BLOCK), 0, s, nets, base, nn, step, ev);
+    hipLaunchKernelGGL(k_stdp_event_syn_batch<SK>, dim3(event_grid(waves, 4)), dim3(BG_BLOCK), 0, s, nets, base, nn,
+                       step, ev);
+    hipLaunchKernelGGL(k_stdp_event_sum_batch, dim3(event_grid(waves, 8)), dim3(BG_BLOCK), 0, s, nets, base, nn, ev);
 }
 template <int SK>
 static void stdp_w(const BatchNet *nets, const uint32_t *base, int nn, uint32_t waves, hipStream_t s, int w32,
-                   uint64_t step, uint32_t *ev)
+                   uint64_t step, const EvLists &ev)
 {
     switch (w32) {
     case 1: stdp<SK, 1>(nets, base, nn, waves, s, step, ev); break;
@@ -82,26 +89,33 @@ static void stdp_w(const BatchNet *nets, const uint32_t *base, int nn, uint32_t 
     }
 }
 
-void launch_step_batch(const BatchNet *nets, const uint32_t *base, int nn, uint32_t total, hipStream_t s, int nk,
-                       int sk, int w32, uint64_t step, uint32_t *ev)
+void launch_neuron_only_batch(const BatchNet *nets, const uint32_t *base, int nn, uint32_t total, hipStream_t s,
+                              int nk, uint64_t step, int steps)
 {
-    const bool two_pass = (sk == SK_STDP || sk == SK_DYNSTDP) && ev;
-    uint32_t *evc = two_pass ? ev : nullptr;
-    unsigned nb = (unsigned)((total + BG_BLOCK - 1) / BG_BLOCK);
+    const unsigned nb = (unsigned)((total + BG_BLOCK - 1) / BG_BLOCK);
     if (nk == NK_LIF)
-        hipLaunchKernelGGL(k_neuron_batch<NK_LIF>, dim3(nb), dim3(BG_BLOCK), 0, s, nets, base, nn, step, evc);
+        hipLaunchKernelGGL(k_neuron_only_batch<NK_LIF>, dim3(nb), dim3(BG_BLOCK), 0, s, nets, base, nn, step, steps);
     else
-        hipLaunchKernelGGL(k_neuron_batch<NK_IZH>, dim3(nb), dim3(BG_BLOCK), 0, s, nets, base, nn, step, evc);
-    switch (sk) {
-    case SK_SPIKING: syn_w<SK_SPIKING>(nets, base, nn, total, s, w32, step); break;
-    case SK_DS: syn_w<SK_DS>(nets, base, nn, total, s, w32, step); break;
-    case SK_STDP:
-        if (two_pass) stdp_w<SK_STDP>(nets, base, nn, total, s, w32, step, ev);
-        else syn_w<SK_STDP>(nets, base, nn, total, s, w32, step);
-        break;
-    default:
-        if (two_pass) stdp_w<SK_DYNSTDP>(nets, base, nn, total, s, w32, step, ev);
-        else syn_w<SK_DYNSTDP>(nets, base, nn, total, s, w32, step);
-        break;
+        hipLaunchKernelGGL(k_neuron_only_batch<NK_IZH>, dim3(nb), dim3(BG_BLOCK), 0, s, nets, base, nn, step, steps);
+}
+
+void launch_step_batch(const BatchNet *nets, const uint32_t *base, int nn, uint32_t total, hipStream_t s, int nk,
+                       int sk, int w32, uint64_t step, const EvLists &ev)
+{
+    const unsigned nb = (unsigned)((total + BG_BLOCK - 1) / BG_BLOCK);
+    if (sk == SK_STDP || sk == SK_DYNSTDP) {
+        if (nk == NK_LIF)
+            hipLaunchKernelGGL(k_stdp_neurons_batch<NK_LIF>, dim3(nb), dim3(BG_BLOCK), 0, s, nets, base, nn, step, ev);
+        else
+            hipLaunchKernelGGL(k_stdp_neurons_batch<NK_IZH>, dim3(nb), dim3(BG_BLOCK), 0, s, nets, base, nn, step, ev);
+        if (sk == SK_STDP) stdp_w<SK_STDP>(nets, base, nn, total, s, w32, step, ev);
+        else stdp_w<SK_DYNSTDP>(nets, base, nn, total, s, w32, step, ev);
+        return;
     }
+    if (nk == NK_LIF)
+        hipLaunchKernelGGL(k_neuron_batch<NK_LIF>, dim3(nb), dim3(BG_BLOCK), 0, s, nets, base, nn, step);
+    else
+        hipLaunchKernelGGL(k_neuron_batch<NK_IZH>, dim3(nb), dim3(BG_BLOCK), 0, s, nets, base, nn, step);
+    if (sk == SK_SPIKING) syn_w<SK_SPIKING>(nets, base, nn, total, s, w32, step);
+    else syn_w<SK_DS>(nets, base, nn, total, s, w32, step);
 }

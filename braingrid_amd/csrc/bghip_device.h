@@ -24,6 +24,20 @@
 enum { NK_LIF = 0, NK_IZH = 1 };
 enum { SK_SPIKING = 0, SK_DS = 1, SK_STDP = 2, SK_DYNSTDP = 3 };
 
+// Lists of the spike events of one per-step STDP step. Destination (work item) w belongs to shard
+// w mod BG_EV_SHARDS; each shard has its own counters, on separate 64-byte lines, and its own
+// segments of the two lists, sized for the worst case (every synapse of every destination of the
+// shard), so appends never overflow and concurrent waves rarely meet on one counter.
+#define BG_EV_SHARDS 128
+#define BG_EV_PAD 16
+struct EvLists {
+    uint32_t *count;            // destinations of shard s at [s * PAD], synapses at [(SHARDS + s) * PAD]
+    const uint32_t *dest_off;   // SHARDS + 1 segment offsets into dest
+    const uint32_t *syn_off;    // SHARDS + 1 segment offsets into syn
+    uint32_t *dest;             // destinations with events
+    uint2 *syn;                 // event synapses as (destination, rank in its row)
+};
+
 struct DevNet {
     int N, M, R, W32, max_spikes;
     float dt;
@@ -68,9 +82,9 @@ struct DevNet {
     int32_t *in_shift;           // Regular: nShiftValues
     int in_dur, in_cycle;        // Regular: nStepsDuration, nStepsCycle
     uint64_t in_cyc_off;         // Regular: nStepsInCycle at step t is (t + in_cyc_off) mod in_cycle
-    // per-step STDP path: destinations whose synapses have a spike event this step
-    uint32_t *ev_count;          // 1 counter, zeroed by the step's neuron kernel
-    uint32_t *ev_list;           // N destination indices
+    // per-step STDP path: the step's spike events, in sharded lists (see EvLists)
+    EvLists ev;
+    const uint32_t *out_begin, *out_count, *out_map;   // outgoing map (synapses of each source)
     const DevNet *self;          // this descriptor in device memory (for kernels that call out of line)
 };
 
@@ -134,53 +148,83 @@ __device__ __forceinline__ bool neuron_advance(const DevNet &n, int i, uint64_t 
 // performs the reference's single `summationPoint += add`. The input does not depend on the
 // neuron's own state, so the fused kernels evaluate a whole block of steps ahead of the neuron
 // loop.
-__device__ __forceinline__ bool input_step(const DevNet &n, int i, uint64_t t, float *add)
+struct InputRegs {
+    uint32_t m;        // Poisson: mask byte
+    int nisi;
+    uint32_t ring;
+    float psr, r, u;
+    uint64_t last;
+    float value;       // Regular
+    int shift;
+};
+
+__device__ __forceinline__ void input_load(const DevNet &n, int i, InputRegs &q)
+{
+    q.m = 0;
+    if (n.in_kind == BG_INPUT_POISSON) {
+        q.m = n.in_mask[i];
+        if (q.m) {
+            q.nisi = n.in_nisi[i]; q.ring = n.in_ring[i]; q.psr = n.in_psr[i];
+            q.r = n.in_r[i]; q.u = n.in_u[i]; q.last = n.in_last[i];
+        }
+    } else if (n.in_kind == BG_INPUT_REGULAR) {
+        q.value = n.in_values[i];
+        q.shift = n.in_shift[i];
+    }
+}
+
+__device__ __forceinline__ void input_store(const DevNet &n, int i, const InputRegs &q)
+{
+    if (n.in_kind == BG_INPUT_POISSON && q.m) {
+        n.in_nisi[i] = q.nisi; n.in_ring[i] = q.ring; n.in_psr[i] = q.psr;
+        n.in_r[i] = q.r; n.in_u[i] = q.u; n.in_last[i] = q.last;
+    }
+}
+
+__device__ __forceinline__ bool input_step_regs(const DevNet &n, int i, uint64_t t, InputRegs &q, float *add)
 {
     if (n.in_kind == BG_INPUT_POISSON) {
-        const uint32_t m = n.in_mask[i];
-        if (m == 0) return false;
-        const int k = (int)m - 1;
-        int nisi = n.in_nisi[i] - 1;
-        uint32_t ring = n.in_ring[i];
+        if (q.m == 0) return false;
+        const int k = (int)q.m - 1;
         const uint32_t tb = (uint32_t)t & 31u;
-        if (nisi <= 0) {
+        if (--q.nisi <= 0) {
             const uint32_t b = 1u << ((tb + (uint32_t)n.in_delay[k]) & 31u);
-            if (ring & b) dev_error(n, 1u);
-            ring |= b;
-            nisi = bg_poisson_isi_steps(n.seed, (uint32_t)i, t, n.in_lambda, n.dt);
+            if (q.ring & b) dev_error(n, 1u);
+            q.ring |= b;
+            q.nisi = bg_poisson_isi_steps(n.seed, (uint32_t)i, t, n.in_lambda, n.dt);
         }
-        n.in_nisi[i] = nisi;
         const float decay = n.in_decay[k];
-        float psr = n.in_psr[i];
-        if ((ring >> tb) & 1u) {
-            ring &= ~(1u << tb);
-            float r = n.in_r[i], u = n.in_u[i];
-            const uint64_t last = n.in_last[i];
-            if (last != BG_NO_SPIKE) {
-                const float isi = (float)(t - last) * n.dt;
-                r = 1.0f + (r * (1.0f - u) - 1.0f) * bg_expf(-isi / n.in_D[k]);
-                u = n.in_U[k] + u * (1.0f - n.in_U[k]) * bg_expf(-isi / n.in_F[k]);
-                n.in_r[i] = r;
-                n.in_u[i] = u;
+        if ((q.ring >> tb) & 1u) {
+            q.ring &= ~(1u << tb);
+            if (q.last != BG_NO_SPIKE) {
+                const float isi = (float)(t - q.last) * n.dt;
+                q.r = 1.0f + (q.r * (1.0f - q.u) - 1.0f) * bg_expf(-isi / n.in_D[k]);
+                q.u = n.in_U[k] + q.u * (1.0f - n.in_U[k]) * bg_expf(-isi / n.in_F[k]);
             }
-            n.in_last[i] = t;
-            psr = psr + ((n.in_W / decay) * u) * r;
+            q.last = t;
+            q.psr = q.psr + ((n.in_W / decay) * q.u) * q.r;
         }
-        n.in_ring[i] = ring;
-        psr = psr * decay;
-        n.in_psr[i] = psr;
-        *add = psr;
+        q.psr = q.psr * decay;
+        *add = q.psr;
         return true;
     }
     if (n.in_kind == BG_INPUT_REGULAR) {
         const int c = (int)((t + n.in_cyc_off) % (uint64_t)n.in_cycle);
-        const int sh = n.in_shift[i];
-        if (c >= sh && c < (sh + n.in_dur) % n.in_cycle) {
-            *add = n.in_values[i];
+        if (c >= q.shift && c < (q.shift + n.in_dur) % n.in_cycle) {
+            *add = q.value;
             return true;
         }
     }
     return false;
+}
+
+__device__ __forceinline__ bool input_step(const DevNet &n, int i, uint64_t t, float *add)
+{
+    InputRegs q;
+    input_load(n, i, q);
+    const bool r = input_step_regs(n, i, t, q, add);
+    input_store(n, i, q);
+    return r;
 }
 
 template <int NK>

@@ -12,7 +12,7 @@
 // (Neurons/AllSpikingNeurons.cpp:98-147); the outgoing/incoming ring notifications become pulls in
 // k_synapse_step.
 template <int NK>
-__device__ __forceinline__ void neuron_step_body(const DevNet &n, int i, uint64_t t)
+__device__ __forceinline__ bool neuron_step_body(const DevNet &n, int i, uint64_t t)
 {
     NeuronRegs s;
     neuron_load<NK>(n, i, s);
@@ -23,13 +23,13 @@ __device__ __forceinline__ void neuron_step_body(const DevNet &n, int i, uint64_
     neuron_store<NK>(n, i, s);
     n.sp[i] = 0.0f;
     if (fired) n.fired_step[i] = t;
+    return fired;
 }
 
 template <int NK>
 __global__ __launch_bounds__(BG_BLOCK) void k_neuron_step(DevNet n, uint64_t t)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && n.ev_count) *n.ev_count = 0;   // event list of this step's STDP passes
     if (i >= n.N) return;
     neuron_step_body<NK>(n, i, t);
 }
@@ -91,87 +91,178 @@ __global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Per-step STDP / DynamicSTDP synapse advance in two passes. Spike events (a pre-synaptic spike
-// leaving the delay ring, or the destination's own spike) are rare, but the learning walks and
-// double-precision arithmetic they need would set the register budget of the whole kernel. The
-// streaming pass therefore does the delay-ring inserts, the ring test and the PSR decay of every
-// event-free synapse and completes the summation of every destination without events; a
-// destination with at least one event is appended to a list and left to the event pass, which
-// handles its event synapses (learning, PSR update, decay) exactly as the one-pass body and then
-// sums all its synapses in the canonical order. Event synapses keep their ring bit set until the
-// event pass, so both passes classify them the same way.
-template <int SK, int W32>
-__device__ __forceinline__ bool stdp_stream_body(const DevNet &n, int d, int lane, uint64_t t)
+// Networks without synapses (growth networks before their first connection, for instance): one
+// thread per neuron advances `steps` steps with neuron and stimulus-input state in registers. The
+// summation point then only ever holds 0 (the sum over no synapses) plus stimulus input, exactly
+// as in the per-step and fused paths.
+template <int NK>
+__device__ __forceinline__ void neuron_only_body(const DevNet &n, int i, uint64_t t0, int steps)
 {
-    const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
-    const bool dense = (cnt == hi);
-    const bool post = n.fired_step[d] == t;
-    const int R = n.R;
-    const uint32_t tb = (uint32_t)(t & (uint64_t)(R - 1));
-    const uint64_t wbase = (uint64_t)(tb >> 5) * n.S;
-    const uint32_t m = 1u << (tb & 31);
-    float acc = 0.0f;
-    bool ev = post && cnt > 0;
-    for (uint32_t q = lane; q < cnt; q += BG_WAVE) {
-        const uint64_t slot = dense ? (uint64_t)d * n.M + q : (uint64_t)n.in_map[n.in_begin[d] + q];
-        const uint32_t meta = n.meta[slot];
-        const int src = (int)(meta & BG_META_SRC_MASK);
-        if (n.fired_step[src] == t) {   // preSpikeHit at the firing step
-            const int delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
-            const uint32_t b = (tb + (uint32_t)delay) & (uint32_t)(R - 1);
-            uint32_t *w = &n.ring[(uint64_t)(b >> 5) * n.S + slot];
-            const uint32_t mb = 1u << (b & 31);
-            if (*w & mb) dev_error(n, 1u);
-            *w |= mb;
-        }
-        if (post) continue;
-        if (n.ring[wbase + slot] & m) {
-            ev = true;
-            continue;
-        }
-        const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
-        const float psr = n.psr[slot] * n.decay[type];
-        n.psr[slot] = psr;
-        acc += psr;
+    NeuronRegs s;
+    neuron_load<NK>(n, i, s);
+    InputRegs q;
+    input_load(n, i, q);
+    float sp = n.sp[i];
+    uint64_t last = BG_NO_SPIKE;
+    for (int k = 0; k < steps; k++) {
+        const uint64_t t = t0 + (uint64_t)k;
+        float add;
+        if (input_step_regs(n, i, t, q, &add)) sp = sp + add;
+        if (neuron_advance<NK>(n, i, t, sp, bg_noise_normal(n.seed, (uint32_t)i, t), s)) last = t;
+        sp = 0.0f;
     }
-    if (__ballot(ev) != 0) return true;
-    acc = wave_sum(acc);
-    if (lane == 0) n.sp[d] = n.sp[d] + acc;
-    return false;
+    neuron_store<NK>(n, i, s);
+    input_store(n, i, q);
+    n.sp[i] = sp;
+    if (last != BG_NO_SPIKE) n.fired_step[i] = last;
+}
+
+template <int NK>
+__global__ __launch_bounds__(BG_BLOCK) void k_neuron_only(DevNet n, uint64_t t0, int steps)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n.N) return;
+    neuron_only_body<NK>(n, i, t0, steps);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-step STDP / DynamicSTDP path. The reference's step is: every neuron advances (a spike
+// sets a bit in the delay ring of each of its outgoing synapses, preSpikeHit), then every synapse
+// advances (Synapses/AllSTDPSynapses.cpp:283-421). Here a step is four launches:
+//   1. k_stdp_neurons: thread per neuron; then, for each neuron of the wave that fired, all 64
+//      lanes set the ring bits of its outgoing synapses (outgoing map);
+//   2. k_stdp_stream: wave per destination; every synapse without a spike event (no pre-spike
+//      leaving its ring, destination did not fire) gets its PSR decay; a destination without
+//      events finishes its summation here. Event synapses are listed (ring bit still set, PSR
+//      untouched) and so is their destination;
+//   3. k_stdp_event_syn: thread per event synapse: ring clear, learning walks, PSR update, decay;
+//   4. k_stdp_event_sum: wave per listed destination: the canonical summation of all its PSRs.
+// Spike events are rare, and their learning walks and double-precision arithmetic would set the
+// register budget of a single fused kernel; split this way the streaming pass runs at full
+// occupancy and the event work runs with every lane busy. Event synapses are independent of one
+// another (each reads its own weight and the two neurons' spike histories, which no synapse
+// writes), so pass 3 may run them in any order.
+
+// preSpikeHit of every outgoing synapse of `src` at step t, by the whole wave.
+__device__ __forceinline__ void push_spike(const DevNet &n, int src, uint64_t t, int lane)
+{
+    const uint32_t beg = n.out_begin[src], c = n.out_count[src];
+    const uint32_t Rm = (uint32_t)n.R - 1;
+    const uint32_t tb = (uint32_t)t & Rm;
+    for (uint32_t q = lane; q < c; q += BG_WAVE) {
+        const uint32_t slot = n.out_map[beg + q];
+        const uint32_t b = (tb + ((n.meta[slot] >> BG_META_DELAY_SHIFT) & 0xFFu)) & Rm;
+        uint32_t *w = &n.ring[(uint64_t)(b >> 5) * n.S + slot];
+        const uint32_t m = 1u << (b & 31);
+        const uint32_t v = *w;
+        if (v & m) dev_error(n, 1u);
+        *w = v | m;
+    }
+}
+
+template <int NK>
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons(DevNet n, uint64_t t)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    if (i < 2 * BG_EV_SHARDS) n.ev.count[i * BG_EV_PAD] = 0;   // launches have at least 256 threads
+    const bool fired = i < n.N && neuron_step_body<NK>(n, i, t);
+    uint64_t fm = __ballot(fired);
+    while (fm) {
+        const int l = __ffsll((unsigned long long)fm) - 1;
+        fm &= fm - 1;
+        push_spike(n, __shfl(i, l), t, lane);
+    }
+}
+
+// Streaming pass for destination d (work item w). Returns whether d has events, in which case
+// its summation is left to the event passes.
+// Whether a post-spike event of this synapse can change its weight: the post-spike walk
+// (stdp_event) reads the pre neuron's newest spike first and stops at once when that spike is
+// out of the learning window, leaving the synapse as an event-free one. fired_step holds every
+// neuron's newest history entry (getSpikeHistory(i, -1)); the test is the walk's own.
+__device__ __forceinline__ bool post_walk_learns(const DevNet &n, uint64_t slot, uint32_t meta, uint64_t t)
+{
+    const uint64_t sh = n.fired_step[meta & BG_META_SRC_MASK];
+    if (sh == BG_NO_SPIKE) return false;
+    const int total_delay = (int)((meta >> BG_META_DELAY_SHIFT) & 0xFFu);
+    const float delta = (float)(uint64_t)(t - sh - (uint64_t)(int64_t)total_delay) * n.dt;
+    return !(delta <= 0 || (double)delta >= 3.0 * (double)n.taupos[slot]);
+}
+
+// Classifies and, when event-free, decays the synapses of ranks qa = q0 + lane and qb = qa + 64
+// whose ring word, meta and psr are given; lists event synapses.
+__device__ __forceinline__ void stdp_stream_pair(const DevNet &n, uint64_t t, uint32_t m, bool post, uint32_t w,
+                                                 uint32_t shard, const EvLists &ev, int lane, bool va, uint32_t qa,
+                                                 uint64_t sa, uint32_t ra, uint32_t ma, float pa, bool vb, uint32_t qb,
+                                                 uint64_t sb, uint32_t rb, uint32_t mb, float pb, float &acc,
+                                                 bool &any)
+{
+    const bool ea = va && ((ra & m) != 0 || (post && post_walk_learns(n, sa, ma, t)));
+    const bool eb = vb && ((rb & m) != 0 || (post && post_walk_learns(n, sb, mb, t)));
+    if (va && !ea) {
+        pa = pa * n.decay[(ma >> BG_META_TYPE_SHIFT) & 3];
+        n.psr[sa] = pa;
+        acc += pa;
+    }
+    if (vb && !eb) {
+        pb = pb * n.decay[(mb >> BG_META_TYPE_SHIFT) & 3];
+        n.psr[sb] = pb;
+        acc += pb;
+    }
+    const uint64_t ema = __ballot(ea), emb = __ballot(eb);
+    if (ema | emb) {
+        any = true;
+        const uint32_t na = (uint32_t)__popcll(ema);
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ev.count[(BG_EV_SHARDS + shard) * BG_EV_PAD], na + (uint32_t)__popcll(emb));
+        base = __shfl(base, 0) + ev.syn_off[shard];
+        const uint64_t below = (1ull << lane) - 1ull;
+        if (ea) ev.syn[base + (uint32_t)__popcll(ema & below)] = make_uint2(w, qa);
+        if (eb) ev.syn[base + na + (uint32_t)__popcll(emb & below)] = make_uint2(w, qb);
+    }
 }
 
 template <int SK, int W32>
-__device__ __forceinline__ void stdp_event_body(const DevNet &n, int d, int lane, uint64_t t)
+__device__ __forceinline__ bool stdp_stream_body(const DevNet &n, int d, int lane, uint64_t t, uint32_t w,
+                                                 const EvLists &ev)
 {
+    const uint32_t shard = w % BG_EV_SHARDS;
     const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
     const bool dense = (cnt == hi);
     const bool post = n.fired_step[d] == t;
-    const int R = n.R;
-    const uint32_t tb = (uint32_t)(t & (uint64_t)(R - 1));
+    const float sp0 = n.sp[d];
+    const uint32_t tb = (uint32_t)(t & (uint64_t)(n.R - 1));
     const uint64_t wbase = (uint64_t)(tb >> 5) * n.S;
     const uint32_t m = 1u << (tb & 31);
+    const uint64_t row = (uint64_t)d * n.M;
+    const uint32_t *imap = n.in_map + (dense ? 0 : n.in_begin[d]);
     float acc = 0.0f;
-    for (uint32_t q = lane; q < cnt; q += BG_WAVE) {
-        const uint64_t slot = dense ? (uint64_t)d * n.M + q : (uint64_t)n.in_map[n.in_begin[d] + q];
-        uint32_t *w = &n.ring[wbase + slot];
-        const uint32_t word = *w;
-        const bool pre = (word & m) != 0;
-        float psr = n.psr[slot];
-        if (pre || post) {
-            if (pre) *w = word & ~m;
-            const uint32_t meta = n.meta[slot];
-            const int src = (int)(meta & BG_META_SRC_MASK);
-            const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
-            const int delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
-            const float decay = n.decay[type];
-            psr = stdp_event<SK>(n, slot, type, t, pre, post, src, d, delay, psr, decay);
-            psr = psr * decay;
-            n.psr[slot] = psr;
+    bool any = false;
+    // two lane-strided ranks per pass, all their loads issued before either is used
+    for (uint32_t q0 = 0; q0 < cnt; q0 += 2 * BG_WAVE) {
+        const uint32_t qa = q0 + (uint32_t)lane, qb = qa + BG_WAVE;
+        const bool va = qa < cnt, vb = qb < cnt;
+        uint64_t sa = 0, sb = 0;
+        uint32_t ra = 0, rb = 0, ma = 0, mb = 0;
+        float pa = 0.0f, pb = 0.0f;
+        if (va) {
+            sa = dense ? row + qa : (uint64_t)imap[qa];
+            ra = n.ring[wbase + sa]; ma = n.meta[sa]; pa = n.psr[sa];
         }
-        acc += psr;
+        if (vb) {
+            sb = dense ? row + qb : (uint64_t)imap[qb];
+            rb = n.ring[wbase + sb]; mb = n.meta[sb]; pb = n.psr[sb];
+        }
+        stdp_stream_pair(n, t, m, post, w, shard, ev, lane, va, qa, sa, ra, ma, pa, vb, qb, sb, rb, mb, pb, acc, any);
+    }
+    if (any) {
+        if (lane == 0) ev.dest[ev.dest_off[shard] + atomicAdd(&ev.count[shard * BG_EV_PAD], 1u)] = w;
+        return true;
     }
     acc = wave_sum(acc);
-    if (lane == 0) n.sp[d] = n.sp[d] + acc;
+    if (lane == 0) n.sp[d] = sp0 + acc;
+    return false;
 }
 
 template <int SK, int W32>
@@ -180,20 +271,70 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream(DevNet n, uint64_t t)
     const int lane = threadIdx.x & (BG_WAVE - 1);
     const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
     if (d >= n.N) return;
-    if (stdp_stream_body<SK, W32>(n, d, lane, t) && lane == 0) n.ev_list[atomicAdd(n.ev_count, 1u)] = (uint32_t)d;
+    stdp_stream_body<SK, W32>(n, d, lane, t, (uint32_t)d, n.ev);
+}
+
+// One event synapse (destination d, rank q): the body of the reference's advanceSynapse for a
+// synapse with fPre or fPost.
+template <int SK>
+__device__ __forceinline__ void stdp_event_syn_body(const DevNet &n, int d, uint32_t q, uint64_t t)
+{
+    const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
+    const uint64_t slot = cnt == hi ? (uint64_t)d * n.M + q : (uint64_t)n.in_map[n.in_begin[d] + q];
+    const uint32_t tb = (uint32_t)(t & (uint64_t)(n.R - 1));
+    uint32_t *wp = &n.ring[(uint64_t)(tb >> 5) * n.S + slot];
+    const uint32_t m = 1u << (tb & 31);
+    const uint32_t word = *wp;
+    const bool pre = (word & m) != 0;
+    if (pre) *wp = word & ~m;
+    const bool post = n.fired_step[d] == t;
+    const uint32_t meta = n.meta[slot];
+    const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
+    const float decay = n.decay[type];
+    const float psr = stdp_event<SK>(n, slot, type, t, pre, post, (int)(meta & BG_META_SRC_MASK), d,
+                                     (int)((meta >> BG_META_DELAY_SHIFT) & 0xFFu), n.psr[slot], decay);
+    n.psr[slot] = psr * decay;
 }
 
 // The descriptor is read from device memory: the out-of-line learning call takes its address,
 // which for a by-value kernel argument would force a private copy of it in every lane.
-template <int SK, int W32>
-__global__ __launch_bounds__(BG_BLOCK) void k_stdp_events(const DevNet *np, uint64_t t)
+// Grids of the event passes: BG_EV_SHARDS * G blocks; block b works on shard b mod BG_EV_SHARDS
+// together with the other G - 1 blocks of that shard.
+template <int SK>
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn(const DevNet *np, uint64_t t)
 {
     const DevNet &n = *np;
+    const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
+    const uint32_t G = gridDim.x / BG_EV_SHARDS;
+    const uint32_t c = n.ev.count[(BG_EV_SHARDS + s) * BG_EV_PAD];
+    const uint2 *list = n.ev.syn + n.ev.syn_off[s];
+    for (uint32_t k = g * BG_BLOCK + threadIdx.x; k < c; k += G * BG_BLOCK) {
+        const uint2 e = list[k];
+        stdp_event_syn_body<SK>(n, (int)e.x, e.y, t);
+    }
+}
+
+__device__ __forceinline__ void stdp_sum_body(const DevNet &n, int d, int lane)
+{
+    const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
+    const bool dense = (cnt == hi);
+    float acc = 0.0f;
+    for (uint32_t q = lane; q < cnt; q += BG_WAVE) {
+        const uint64_t slot = dense ? (uint64_t)d * n.M + q : (uint64_t)n.in_map[n.in_begin[d] + q];
+        acc += n.psr[slot];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) n.sp[d] = n.sp[d] + acc;
+}
+
+static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum(DevNet n)
+{
     const int lane = threadIdx.x & (BG_WAVE - 1);
-    const uint32_t nw = gridDim.x * (BG_BLOCK / BG_WAVE);
-    const uint32_t c = *n.ev_count;
-    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / BG_WAVE; i < c; i += nw)
-        stdp_event_body<SK, W32>(n, (int)n.ev_list[i], lane, t);
+    const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
+    const uint32_t G = gridDim.x / BG_EV_SHARDS, wpb = BG_BLOCK / BG_WAVE;
+    const uint32_t c = n.ev.count[s * BG_EV_PAD];
+    const uint32_t *list = n.ev.dest + n.ev.dest_off[s];
+    for (uint32_t k = g * wpb + threadIdx.x / BG_WAVE; k < c; k += G * wpb) stdp_sum_body(n, (int)list[k], lane);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -390,6 +531,15 @@ struct BatchNet {
     int pull;           // 1 if those spikes are still to be inserted into the rings
 };
 
+// Network of thread work item w when the block's first item belongs to network i0 (found once per
+// block with batch_find): a block spans few networks, so a short forward walk suffices.
+__device__ __forceinline__ int batch_walk(const uint32_t *base, int nn, uint32_t w, int i0)
+{
+    int i = i0;
+    while (i + 1 < nn && base[i + 1] <= w) i++;
+    return i;
+}
+
 __device__ __forceinline__ int batch_find(const uint32_t *base, int nn, uint32_t w)
 {
     int lo = 0, hi = nn;   // base[lo] <= w < base[hi]
@@ -421,12 +571,12 @@ __global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
 
 template <int NK>
 __global__ __launch_bounds__(BG_BLOCK) void k_neuron_batch(const BatchNet *nets, const uint32_t *nbase, int nn,
-                                                           uint64_t step, uint32_t *ev_count)
+                                                           uint64_t step)
 {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w == 0 && ev_count) *ev_count = 0;
+    const uint32_t w0 = blockIdx.x * blockDim.x;
     if (w >= nbase[nn]) return;
-    const int i = batch_find(nbase, nn, w);
+    const int i = batch_walk(nbase, nn, w, __builtin_amdgcn_readfirstlane(batch_find(nbase, nn, w0)));
     const BatchNet &b = nets[i];
     neuron_step_body<NK>(b.n, (int)(w - nbase[i]), b.t0 + step);
 }
@@ -443,33 +593,91 @@ __global__ __launch_bounds__(BG_BLOCK) void k_synapse_batch(const BatchNet *nets
     synapse_step_body<SK, W32>(b.n, (int)(w - wbase[i]), threadIdx.x & (BG_WAVE - 1), b.t0 + step);
 }
 
-// Two-pass STDP step over a batch: the event list holds batch work-item indices.
-template <int SK, int W32>
-__global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
-                                                                uint64_t step, uint32_t *ev_count, uint32_t *ev_list)
+// The per-step STDP passes over a batch; event lists hold batch work-item indices.
+template <int NK>
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons_batch(const BatchNet *nets, const uint32_t *nbase, int nn,
+                                                                 uint64_t step, EvLists ev)
 {
-    const uint32_t w = __builtin_amdgcn_readfirstlane(
-        (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
-    if (w >= wbase[nn]) return;
-    const int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, w));
-    const BatchNet &b = nets[i];
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (BG_WAVE - 1);
-    if (stdp_stream_body<SK, W32>(b.n, (int)(w - wbase[i]), lane, b.t0 + step) && lane == 0)
-        ev_list[atomicAdd(ev_count, 1u)] = w;
+    if (w < 2 * BG_EV_SHARDS) ev.count[w * BG_EV_PAD] = 0;   // launches have at least 256 threads
+    bool fired = false;
+    int ni = 0, li = 0;
+    const int i0 = __builtin_amdgcn_readfirstlane(batch_find(nbase, nn, blockIdx.x * blockDim.x));
+    if (w < nbase[nn]) {
+        ni = batch_walk(nbase, nn, w, i0);
+        li = (int)(w - nbase[ni]);
+        fired = neuron_step_body<NK>(nets[ni].n, li, nets[ni].t0 + step);
+    }
+    uint64_t fm = __ballot(fired);
+    while (fm) {
+        const int l = __ffsll((unsigned long long)fm) - 1;
+        fm &= fm - 1;
+        const int bi = __builtin_amdgcn_readfirstlane(__shfl(ni, l));
+        const BatchNet &b = nets[bi];
+        push_spike(b.n, __shfl(li, l), b.t0 + step, lane);
+    }
 }
 
+// BG_STREAM_DESTS consecutive destinations per wave (one: the per-destination latency chain is
+// what bounds this pass, and more waves in flight hide it better than fewer, longer ones).
+#define BG_STREAM_DESTS 1
 template <int SK, int W32>
-__global__ __launch_bounds__(BG_BLOCK) void k_stdp_events_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
-                                                                uint64_t step, const uint32_t *ev_count,
-                                                                const uint32_t *ev_list)
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
+                                                                uint64_t step, EvLists ev)
+{
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(
+        (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
+    const uint32_t g0 = wv * BG_STREAM_DESTS, total = wbase[nn];
+    if (g0 >= total) return;
+    const uint32_t g1 = min(g0 + BG_STREAM_DESTS, total);
+    int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, g0));
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    for (uint32_t g = g0; g < g1; g++) {
+        while (wbase[i + 1] <= g) i++;
+        const BatchNet &b = nets[i];
+        stdp_stream_body<SK, W32>(b.n, (int)(g - wbase[i]), lane, b.t0 + step, g, ev);
+    }
+}
+
+template <int SK>
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
+                                                                   uint64_t step, EvLists ev)
+{
+    const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
+    const uint32_t G = gridDim.x / BG_EV_SHARDS;
+    const uint32_t c = ev.count[(BG_EV_SHARDS + s) * BG_EV_PAD];
+    const uint2 *list = ev.syn + ev.syn_off[s];
+    for (uint32_t k = g * BG_BLOCK + threadIdx.x; k < c; k += G * BG_BLOCK) {
+        const uint2 e = list[k];
+        const int i = batch_find(wbase, nn, e.x);
+        const BatchNet &b = nets[i];
+        stdp_event_syn_body<SK>(b.n, (int)(e.x - wbase[i]), e.y, b.t0 + step);
+    }
+}
+
+static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum_batch(const BatchNet *nets, const uint32_t *wbase,
+                                                                          int nn, EvLists ev)
 {
     const int lane = threadIdx.x & (BG_WAVE - 1);
-    const uint32_t nw = gridDim.x * (BG_BLOCK / BG_WAVE);
-    const uint32_t c = *ev_count;
-    for (uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) / BG_WAVE; k < c; k += nw) {
-        const uint32_t w = __builtin_amdgcn_readfirstlane(ev_list[k]);
+    const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
+    const uint32_t G = gridDim.x / BG_EV_SHARDS, wpb = BG_BLOCK / BG_WAVE;
+    const uint32_t c = ev.count[s * BG_EV_PAD];
+    const uint32_t *list = ev.dest + ev.dest_off[s];
+    for (uint32_t k = g * wpb + threadIdx.x / BG_WAVE; k < c; k += G * wpb) {
+        const uint32_t w = __builtin_amdgcn_readfirstlane(list[k]);
         const int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, w));
-        const BatchNet &b = nets[i];
-        stdp_event_body<SK, W32>(b.n, (int)(w - wbase[i]), lane, b.t0 + step);
+        stdp_sum_body(nets[i].n, (int)(w - wbase[i]), lane);
     }
+}
+
+template <int NK>
+__global__ __launch_bounds__(BG_BLOCK) void k_neuron_only_batch(const BatchNet *nets, const uint32_t *nbase, int nn,
+                                                                uint64_t step, int steps)
+{
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nbase[nn]) return;
+    const int i = batch_walk(nbase, nn, w, __builtin_amdgcn_readfirstlane(batch_find(nbase, nn, blockIdx.x * blockDim.x)));
+    const BatchNet &b = nets[i];
+    neuron_only_body<NK>(b.n, (int)(w - nbase[i]), b.t0 + step, steps);
 }

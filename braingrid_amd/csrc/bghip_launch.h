@@ -9,11 +9,16 @@ void launch_fused_lif(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_
 void launch_fused_izh(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_t t0, int K, int pull,
                       uint64_t tprev0, int sel);
 void launch_neuron_step(const DevNet &dn, hipStream_t s, int nk, uint64_t t);
+// steps per neuron-only launch
+#define BG_QUIET_STEPS 10000
+void launch_neuron_only(const DevNet &dn, hipStream_t s, int nk, uint64_t t0, int steps);
 void launch_synapse_step(const DevNet &dn, hipStream_t s, int sk, uint64_t t);
-void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int sk, uint64_t t);
+void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int nk, int sk, uint64_t t);
 
 struct BatchNet;
 void launch_fused_batch(const BatchNet *nets, const uint32_t *wbase, int nn, uint32_t waves, hipStream_t s, int nk,
                         int sk, int w32, int kmax, int K, int Kfull, int launch);
+void launch_neuron_only_batch(const BatchNet *nets, const uint32_t *base, int nn, uint32_t total, hipStream_t s,
+                              int nk, uint64_t step, int steps);
 void launch_step_batch(const BatchNet *nets, const uint32_t *base, int nn, uint32_t total, hipStream_t s, int nk,
-                       int sk, int w32, uint64_t step, uint32_t *ev);
+                       int sk, int w32, uint64_t step, const EvLists &ev);

@@ -11,6 +11,13 @@ void launch_neuron_step(const DevNet &dn, hipStream_t s, int nk, uint64_t t)
     else hipLaunchKernelGGL(k_neuron_step<NK_IZH>, dim3(nb), dim3(BG_BLOCK), 0, s, dn, t);
 }
 
+void launch_neuron_only(const DevNet &dn, hipStream_t s, int nk, uint64_t t0, int steps)
+{
+    unsigned nb = (unsigned)((dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    if (nk == NK_LIF) hipLaunchKernelGGL(k_neuron_only<NK_LIF>, dim3(nb), dim3(BG_BLOCK), 0, s, dn, t0, steps);
+    else hipLaunchKernelGGL(k_neuron_only<NK_IZH>, dim3(nb), dim3(BG_BLOCK), 0, s, dn, t0, steps);
+}
+
 template <int SK, int W32>
 static void go(const DevNet &dn, hipStream_t s, uint64_t t)
 {
@@ -38,35 +45,42 @@ void launch_synapse_step(const DevNet &dn, hipStream_t s, int sk, uint64_t t)
     }
 }
 
-// Event pass grid: enough waves to keep every CU busy when many destinations have events, few
-// enough that an event-free step costs one short launch.
-static unsigned event_blocks(uint64_t waves_needed)
+// Event passes: G blocks per event-list shard, more for larger networks (each shard holds about
+// N / BG_EV_SHARDS destinations).
+static unsigned event_grid(int N, unsigned gmax)
 {
-    const uint64_t cap = 1024;   // 4096 waves
-    const uint64_t b = (waves_needed + (BG_BLOCK / BG_WAVE) - 1) / (BG_BLOCK / BG_WAVE);
-    return (unsigned)std::max<uint64_t>(1, std::min(cap, b));
+    const unsigned g = (unsigned)(((uint64_t)N + 16 * BG_EV_SHARDS - 1) / (16 * BG_EV_SHARDS));
+    return BG_EV_SHARDS * std::max(1u, std::min(gmax, g));
 }
 
-template <int SK, int W32>
+template <int NK, int SK, int W32>
 static void stdp_go(const DevNet &dn, const DevNet *d_self, hipStream_t s, uint64_t t)
 {
-    unsigned blocks = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
-    hipLaunchKernelGGL((k_stdp_stream<SK, W32>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t);
-    hipLaunchKernelGGL((k_stdp_events<SK, W32>), dim3(event_blocks((uint64_t)dn.N)), dim3(BG_BLOCK), 0, s, d_self, t);
+    const unsigned nb = (unsigned)((dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    const unsigned wb = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
+    hipLaunchKernelGGL(k_stdp_neurons<NK>, dim3(nb), dim3(BG_BLOCK), 0, s, dn, t);
+    hipLaunchKernelGGL((k_stdp_stream<SK, W32>), dim3(wb), dim3(BG_BLOCK), 0, s, dn, t);
+    hipLaunchKernelGGL(k_stdp_event_syn<SK>, dim3(event_grid(dn.N, 4)), dim3(BG_BLOCK), 0, s, d_self, t);
+    hipLaunchKernelGGL(k_stdp_event_sum, dim3(event_grid(dn.N, 8)), dim3(BG_BLOCK), 0, s, dn);
 }
-template <int SK>
+template <int NK, int SK>
 static void stdp_go_w(const DevNet &dn, const DevNet *d_self, hipStream_t s, uint64_t t)
 {
     switch (dn.W32) {
-    case 1: stdp_go<SK, 1>(dn, d_self, s, t); break;
-    case 2: stdp_go<SK, 2>(dn, d_self, s, t); break;
-    case 4: stdp_go<SK, 4>(dn, d_self, s, t); break;
-    default: stdp_go<SK, 8>(dn, d_self, s, t); break;
+    case 1: stdp_go<NK, SK, 1>(dn, d_self, s, t); break;
+    case 2: stdp_go<NK, SK, 2>(dn, d_self, s, t); break;
+    case 4: stdp_go<NK, SK, 4>(dn, d_self, s, t); break;
+    default: stdp_go<NK, SK, 8>(dn, d_self, s, t); break;
     }
 }
 
-void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int sk, uint64_t t)
+void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int nk, int sk, uint64_t t)
 {
-    if (sk == SK_STDP) stdp_go_w<SK_STDP>(dn, d_self, s, t);
-    else stdp_go_w<SK_DYNSTDP>(dn, d_self, s, t);
+    if (nk == NK_LIF) {
+        if (sk == SK_STDP) stdp_go_w<NK_LIF, SK_STDP>(dn, d_self, s, t);
+        else stdp_go_w<NK_LIF, SK_DYNSTDP>(dn, d_self, s, t);
+    } else {
+        if (sk == SK_STDP) stdp_go_w<NK_IZH, SK_STDP>(dn, d_self, s, t);
+        else stdp_go_w<NK_IZH, SK_DYNSTDP>(dn, d_self, s, t);
+    }
 }
