@@ -145,14 +145,65 @@ def batched_main(args):
     print(json.dumps(out), flush=True)
 
 
+def served_main(args):
+    """The served form: an open-loop Poisson stream of simulation jobs, continuously batched."""
+    from braingrid_amd import bghip as B
+    from braingrid_amd import serve as S
+
+    ctx = B.Context(0)
+    free, total = ctx.mem_info()
+    budget = int(free * args.hbm_fraction)
+    jobs = S.served_requests(args.requests, rate=args.rate, seed=args.seed)
+    eng = S.DeviceEngine(ctx)
+    srv = S.Server(engine=eng, budget=budget, log=log)
+    log(f"served: {len(jobs)} requests at {args.rate}/s, budget {budget / 2**30:.1f} GiB of {total / 2**30:.1f} GiB")
+    srv.run(jobs, progress_every=20.0)
+    st = srv.stats(total_bytes=total)
+    eng.close()
+    ctx.close()
+    out = {
+        "metric": "completed simulation jobs/s of an open-loop Poisson job stream (continuous batching)",
+        "value": st["throughput_jobs_per_s"],
+        "unit": "jobs/s",
+        "n_gpus": 1,
+        "steps": int(sum(j.spec["sim_s"] for j in jobs if j.state == S.DONE) / 1e-4),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded request stream; deterministic Philox noise and Poisson input)",
+        "config": {"workload": f"served: {args.requests} requests, Poisson arrivals at {args.rate}/s, sides "
+                               "log-uniform 30..200, 70% LIF+DS+growth / 30% Izhikevich+DynamicSTDP+Poisson, "
+                               "1..20 s each", "requests": args.requests, "rate": args.rate, "seed": args.seed,
+                   "hbm_budget_bytes": budget},
+        "latency_p50_s": st["latency_p50_s"],
+        "latency_p99_s": st["latency_p99_s"],
+        "latency_mean_s": st["latency_mean_s"],
+        "sim_seconds_per_hour": st["sim_seconds_per_hour"],
+        "hbm_occupancy_mean": st.get("hbm_occupancy_mean"),
+        "hbm_occupancy_max": st.get("hbm_occupancy_max"),
+        "wall_s": st["wall_s"],
+        "counts": st["counts"],
+        "phase_s": st["host_phase_s"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--form", choices=("single", "batched"), default="single",
-                    help="single: the 10k-neuron headline; batched: the multi-network sweep")
+    ap.add_argument("--form", choices=("single", "batched", "served"), default="single",
+                    help="single: the 10k-neuron headline; batched: the multi-network sweep; "
+                         "served: the open-loop job stream")
     ap.add_argument("--nets", type=int, default=256, help="batched form: networks in the sweep")
     ap.add_argument("--sim-seconds", type=float, default=5.0, help="batched form: simulated seconds per network")
     ap.add_argument("--seed", type=int, default=2024, help="batched form: sweep seed")
+    ap.add_argument("--requests", type=int, default=2000, help="served form: requests in the stream")
+    ap.add_argument("--rate", type=float, default=20.0, help="served form: mean arrival rate (requests/s)")
+    ap.add_argument("--hbm-fraction", type=float, default=0.9,
+                    help="served form: share of the free HBM the resident networks may occupy")
     ap.add_argument("--alone-stride", type=int, default=4,
                     help="batched form: run every k-th network alone for the one-at-a-time comparison")
     ap.add_argument("--steps", type=int, default=100000)
@@ -178,6 +229,9 @@ def main():
         sys.exit(2)
     if args.form == "batched":
         batched_main(args)
+        return
+    if args.form == "served":
+        served_main(args)
         return
 
     from braingrid_amd import bghip as B

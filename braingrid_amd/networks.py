@@ -49,6 +49,22 @@ def lif_growth(ctx: B.Context, side: int, target_rate: float, frac_exc: float, s
                    epochs=epochs, cfg=c)
 
 
+def _random_targets(rng, N: int, k: int) -> np.ndarray:
+    """k distinct targets per source, none equal to the source: row s holds k of the N - 1 other
+    neurons. Drawn with replacement for all rows at once; rows that came out with a repeat are
+    drawn again without replacement (a few percent of rows at k = 100)."""
+    if k > N - 1:
+        raise ValueError(f"{k} outgoing synapses need at least {k + 1} neurons")
+    if 2 * k > N - 1:   # dense: a permutation per row
+        t = np.argsort(rng.random((N, N - 1)), axis=1)[:, :k]
+    else:
+        t = rng.integers(0, N - 1, size=(N, k))
+        srt = np.sort(t, axis=1)
+        for r in np.flatnonzero((srt[:, 1:] == srt[:, :-1]).any(axis=1)):
+            t[r] = rng.choice(N - 1, size=k, replace=False)
+    return t + (t >= np.arange(N)[:, None])
+
+
 def izh_stdp_arrays(N: int, seed: int, k: int = 100, dt: float = 1e-4, ring_bits: int = 256, max_rate: int = 200,
                     epoch_s: float = 1.0):
     """Host arrays of the static Izhikevich + STDP network (also used by tests to load the oracle)."""
@@ -80,11 +96,7 @@ def izh_stdp_arrays(N: int, seed: int, k: int = 100, dt: float = 1e-4, ring_bits
                    u=np.zeros(N, F32), C3=np.full(N, F32(F32(dt) * F32(1000)), F32),
                    spikeCount=np.zeros(N, np.int32), spikeCountOffset=np.zeros(N, np.int32),
                    spike_history=np.full(N * max_spikes, np.iinfo(np.uint64).max, np.uint64))
-    # k distinct random targets per source (no self connection)
-    targets = np.empty((N, k), np.int64)
-    for s in range(N):
-        t = rng.choice(N - 1, size=k, replace=False)
-        targets[s] = t + (t >= s)
+    targets = _random_targets(rng, N, k)
     w = np.where(exc[:, None], rng.uniform(0.0, 6.0, (N, k)), rng.uniform(-5.0, 0.0, (N, k))) * W_UNIT
     delay_ms = rng.integers(1, 21, (N, k))
     src = np.repeat(np.arange(N), k)
@@ -127,12 +139,17 @@ def izh_stdp_arrays(N: int, seed: int, k: int = 100, dt: float = 1e-4, ring_bits
     return neurons, syn, meta
 
 
+def izh_stdp_desc(meta: dict, seed: int) -> B.NetDesc:
+    return MD.net_desc(meta["N"], meta["M"], B.NEURON_IZH, B.SYN_DYNSTDP, meta["max_spikes"], 1e-4, seed,
+                       meta["ring_bits"], 0, STDP_CFG)
+
+
 def izh_stdp(ctx: B.Context, N: int, seed: int, k: int = 100, ring_bits: int = 256, max_rate: int = 200,
-             epoch_s: float = 1.0):
-    neurons, syn, meta = izh_stdp_arrays(N, seed, k, ring_bits=ring_bits, max_rate=max_rate, epoch_s=epoch_s)
-    desc = MD.net_desc(N, meta["M"], B.NEURON_IZH, B.SYN_DYNSTDP, meta["max_spikes"], 1e-4, seed, ring_bits, 0,
-                       STDP_CFG)
-    net = B.Net(ctx, desc)
+             epoch_s: float = 1.0, arrays=None):
+    """Create the network on the device; `arrays` may hold izh_stdp_arrays' result computed earlier."""
+    neurons, syn, meta = arrays or izh_stdp_arrays(N, seed, k, ring_bits=ring_bits, max_rate=max_rate,
+                                                   epoch_s=epoch_s)
+    net = B.Net(ctx, izh_stdp_desc(meta, seed))
     net.upload_neurons(neurons)
     net.upload_synapses(syn, 0)
     return net, meta

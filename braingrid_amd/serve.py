@@ -1,0 +1,287 @@
+"""Served form: simulation jobs arriving over time, run with continuous batching on one GPU.
+
+A job is one network specification (the families of braingrid_amd.sweep: grid side, lif_growth or
+izh_stdp, tR, fE, seed, simulated seconds). The server keeps every admitted network resident in
+HBM from admission to completion and advances all of them together, one 1 s epoch per tick, with
+shared launches (bghip_batch_*); between ticks it runs each network's epoch work (growth update,
+history), retires finished jobs, and admits queued ones. Admission is by device footprint: the sum
+of the resident networks' arenas (bghip_net_footprint; growth is tiled, so there is no N^2 term)
+never exceeds the HBM budget. A job whose footprint alone exceeds the budget is rejected; a job
+whose specification is invalid, or that raises a device error, fails on its own; a job can be
+cancelled while queued or running. A network's results do not depend on which other networks
+shared its launches.
+
+The scheduling logic talks to an engine (DeviceEngine here; tests substitute a CPU one), so the
+policy is testable without a GPU.
+"""
+from __future__ import annotations
+
+import heapq
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import sweep as SW
+
+QUEUED, ACTIVE, DONE, FAILED, REJECTED, CANCELLED = "queued", "active", "done", "failed", "rejected", "cancelled"
+
+
+@dataclass
+class Job:
+    id: int
+    spec: dict
+    arrival: float                  # seconds after the stream started
+    state: str = QUEUED
+    plan: object = None
+    handle: object = None
+    footprint: int = 0
+    admitted: float | None = None   # absolute clock times
+    finished: float | None = None
+    epochs_done: int = 0
+    epochs: int = 0
+    skipped: int = 0                # times a later job was admitted ahead of it
+    error: str | None = None
+    result: dict | None = None
+
+
+def served_requests(n: int = 2000, rate: float = 20.0, seed: int = 7, side_lo: int = 30, side_hi: int = 200,
+                    frac_lif: float = 0.7, dur_lo: int = 1, dur_hi: int = 20):
+    """Open-loop request stream: exponential inter-arrival times of mean 1/rate, sides log-uniform
+    on [side_lo, side_hi], 70% lif_growth / 30% izh_stdp, tR and fE as in the sweep, durations
+    uniform on whole seconds dur_lo..dur_hi (the growth epoch is 1 s)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        side = int(round(float(np.exp(rng.uniform(np.log(side_lo), np.log(side_hi))))))
+        kind = "lif_growth" if rng.random() < frac_lif else "izh_stdp"
+        spec = dict(id=k, side=side, kind=kind, tR=float(rng.choice(SW.TR_CHOICES)),
+                    fE=float(rng.choice(SW.FE_CHOICES)), seed=SW.net_seed(seed, k),
+                    sim_s=float(rng.integers(dur_lo, dur_hi + 1)))
+        out.append(Job(id=k, spec=spec, arrival=t))
+    return out
+
+
+class DeviceEngine:
+    """Networks on the GPU through the C ABI."""
+
+    def __init__(self, ctx):
+        from . import bghip as B
+        self.ctx = ctx
+        self.batch = B.Batch(ctx)
+        self._members = None
+
+    def total_bytes(self) -> int:
+        return self.ctx.mem_info()[1]
+
+    def plan(self, spec):
+        return SW.plan(spec)
+
+    def footprint(self, plan) -> int:
+        return SW.footprint(plan)
+
+    def create(self, spec, plan):
+        return SW.SweepNet(self.ctx, spec, plan)
+
+    def advance(self, handles, steps: int):
+        key = tuple(id(h) for h in handles)
+        if key != self._members:
+            self.batch.set([h.net for h in handles])
+            self._members = key
+        self.batch.advance(steps)
+
+    def end_epoch(self, h):
+        h.end_epoch()
+        bits = h.net.error_bits()
+        if bits:
+            raise RuntimeError(f"device error bits 0x{bits:x}")
+
+    def result(self, h) -> dict:
+        return {"fingerprint": h.fingerprint(), "synapses": h.net.total_synapses, "step": h.net.step}
+
+    def release(self, h):
+        h.close()
+
+    def close(self):
+        self.batch.close()
+
+
+@dataclass
+class Server:
+    engine: object
+    budget: int                          # device bytes the resident networks may occupy
+    epoch_steps: int = int(round(SW.EPOCH_S / SW.DT))
+    lookahead: int = 16                  # queued jobs considered per admission round
+    max_skips: int = 32                  # after this many bypasses the oldest job blocks the queue
+    clock: object = time.perf_counter
+    log: object = None
+    jobs: dict = field(default_factory=dict)
+    queue: list = field(default_factory=list)      # heap of (arrival, id)
+    active: list = field(default_factory=list)     # admitted jobs in admission order
+    used: int = 0
+    t0: float = 0.0
+    occupancy: list = field(default_factory=list)  # (clock, used bytes) per tick
+    ticks: int = 0
+    phase_s: dict = field(default_factory=lambda: {"admit": 0.0, "advance": 0.0, "epoch": 0.0})
+
+    # --- requests -----------------------------------------------------------------------------
+    def submit(self, job: Job):
+        self.jobs[job.id] = job
+        job.epochs = int(round(job.spec.get("sim_s", 0) / SW.EPOCH_S)) if isinstance(job.spec.get("sim_s"), (int, float)) else 0
+        heapq.heappush(self.queue, (job.arrival, job.id))
+
+    def cancel(self, job_id: int) -> bool:
+        job = self.jobs.get(job_id)
+        if job is None or job.state not in (QUEUED, ACTIVE):
+            return False
+        if job.state == ACTIVE:
+            self._release(job)
+            self.active.remove(job)
+        job.state = CANCELLED
+        job.finished = self.clock()
+        return True
+
+    # --- one scheduling round -----------------------------------------------------------------
+    def _release(self, job: Job):
+        if job.handle is not None:
+            self.engine.release(job.handle)
+            job.handle = None
+        self.used -= job.footprint
+        job.plan = None
+
+    def _fail(self, job: Job, state: str, err: str):
+        job.state = state
+        job.error = err
+        job.finished = self.clock()
+        if self.log:
+            self.log(f"job {job.id} {state}: {err}")
+
+    def admit(self):
+        """Admits queued jobs in arrival order while they fit; up to `lookahead` jobs that do not
+        fit yet are passed over (and later ones admitted) before the round ends."""
+        considered = 0
+        kept = []
+        head_blocked = False
+        while self.queue and considered < self.lookahead:
+            arrival, jid = heapq.heappop(self.queue)
+            job = self.jobs[jid]
+            if job.state != QUEUED:
+                continue
+            if head_blocked:
+                considered += 1
+                kept.append((arrival, jid))
+                continue
+            try:
+                if job.plan is None:
+                    job.plan = self.engine.plan(job.spec)
+                    job.footprint = int(self.engine.footprint(job.plan))
+            except Exception as e:          # an invalid specification fails this job only
+                self._fail(job, FAILED, f"{type(e).__name__}: {e}")
+                continue
+            if job.footprint > self.budget:
+                job.plan = None
+                self._fail(job, REJECTED, f"footprint {job.footprint} B exceeds the budget {self.budget} B")
+                continue
+            if self.used + job.footprint > self.budget:
+                considered += 1
+                job.skipped += 1
+                kept.append((arrival, jid))
+                # an old job that keeps being bypassed stops later ones from overtaking it
+                if job.skipped > self.max_skips:
+                    head_blocked = True
+                continue
+            try:
+                job.handle = self.engine.create(job.spec, job.plan)
+            except Exception as e:
+                job.plan = None
+                self._fail(job, FAILED, f"{type(e).__name__}: {e}")
+                continue
+            self.used += job.footprint
+            job.state = ACTIVE
+            job.admitted = self.clock()
+            self.active.append(job)
+        for item in kept:
+            heapq.heappush(self.queue, item)
+
+    def tick(self):
+        """Admit, advance every resident network by one epoch, do the epoch work, retire."""
+        c0 = self.clock()
+        self.admit()
+        c1 = self.clock()
+        self.phase_s["admit"] += c1 - c0
+        self.occupancy.append((c1, self.used))
+        if not self.active:
+            return
+        self.engine.advance([j.handle for j in self.active], self.epoch_steps)
+        c2 = self.clock()
+        self.phase_s["advance"] += c2 - c1
+        for job in list(self.active):
+            try:
+                self.engine.end_epoch(job.handle)
+            except Exception as e:
+                self.active.remove(job)
+                self._release(job)
+                self._fail(job, FAILED, f"{type(e).__name__}: {e}")
+                continue
+            job.epochs_done += 1
+            if job.epochs_done >= job.epochs:
+                job.result = self.engine.result(job.handle)
+                self.active.remove(job)
+                self._release(job)
+                job.state = DONE
+                job.finished = self.clock()
+        self.phase_s["epoch"] += self.clock() - c2
+        self.ticks += 1
+
+    # --- open-loop driver ---------------------------------------------------------------------
+    def run(self, jobs, sleep=time.sleep, progress_every: float = 30.0):
+        """Feed `jobs` at their arrival times (seconds after the start) and serve until every job
+        has left the system."""
+        pending = sorted(jobs, key=lambda j: j.arrival)
+        self.t0 = self.clock()
+        k = 0
+        last_log = self.t0
+        while k < len(pending) or self.queue or self.active:
+            now = self.clock() - self.t0
+            while k < len(pending) and pending[k].arrival <= now:
+                self.submit(pending[k])
+                k += 1
+            if not self.active and not self.queue:
+                if k < len(pending):
+                    sleep(max(0.0, pending[k].arrival - (self.clock() - self.t0)))
+                continue
+            self.tick()
+            if self.log and self.clock() - last_log >= progress_every:
+                last_log = self.clock()
+                c = self.counts()
+                self.log(f"t={last_log - self.t0:.0f}s submitted={k} queued={len(self.queue)} "
+                         f"active={len(self.active)} done={c.get(DONE, 0)} hbm={self.used / 2**30:.1f}GiB")
+
+    # --- reporting ----------------------------------------------------------------------------
+    def counts(self) -> dict:
+        out = {}
+        for j in self.jobs.values():
+            out[j.state] = out.get(j.state, 0) + 1
+        return out
+
+    def stats(self, total_bytes: int | None = None) -> dict:
+        done = [j for j in self.jobs.values() if j.state == DONE]
+        lat = np.array([j.finished - (self.t0 + j.arrival) for j in done]) if done else np.zeros(0)
+        span = (max(j.finished for j in done) - self.t0) if done else 0.0
+        occ = np.array([u for _, u in self.occupancy], dtype=np.float64) if self.occupancy else np.zeros(1)
+        sim_s = sum(j.spec["sim_s"] for j in done)
+        out = {"completed": len(done), "counts": self.counts(), "wall_s": span,
+               "throughput_jobs_per_s": len(done) / span if span > 0 else 0.0,
+               "sim_seconds_per_hour": sim_s / span * 3600.0 if span > 0 else 0.0,
+               "latency_p50_s": float(np.percentile(lat, 50)) if lat.size else None,
+               "latency_p99_s": float(np.percentile(lat, 99)) if lat.size else None,
+               "latency_mean_s": float(lat.mean()) if lat.size else None,
+               "hbm_resident_mean_bytes": float(occ.mean()), "hbm_resident_max_bytes": float(occ.max()),
+               "ticks": self.ticks, "budget_bytes": self.budget,
+               "host_phase_s": {k: round(v, 3) for k, v in self.phase_s.items()}}
+        if total_bytes:
+            out["hbm_occupancy_mean"] = float(occ.mean()) / total_bytes
+            out["hbm_occupancy_max"] = float(occ.max()) / total_bytes
+        return out

@@ -25,6 +25,7 @@ import numpy as np
 
 from . import bghip as B
 from . import host
+from . import model as MD
 from . import networks as NW
 
 TR_CHOICES = (0.1, 1.0, 1.9)
@@ -55,10 +56,51 @@ def sweep_specs(n_nets: int = 256, seed: int = 2024, side_lo: int = 10, side_hi:
     return specs
 
 
+MAX_SIDE = 512   # N = side^2 stays under the engine's 262,144-neuron limit
+
+
+def lif_config(spec: dict) -> dict:
+    c = host.base_config()
+    c.update(width=spec["side"], height=spec["side"], layout="DynamicLayout", frac_exc=spec["fE"], frac_starter=0.1,
+             target_rate=spec["tR"], seed=spec["seed"], epoch_duration=EPOCH_S,
+             num_epochs=int(round(spec["sim_s"] / EPOCH_S)), max_synapses_per_neuron=LIF_M,
+             max_firing_rate=LIF_MAX_RATE)
+    return c
+
+
+def plan(spec: dict) -> dict:
+    """Host-side preparation of a network: validates the specification, draws the static
+    connectivity of an Izhikevich network and gives the device description (for the footprint).
+    Raises ValueError for a specification the engine cannot run."""
+    kind = spec.get("kind")
+    side = spec.get("side")
+    sim_s = spec.get("sim_s")
+    if kind not in ("lif_growth", "izh_stdp"):
+        raise ValueError(f"unknown network family {kind!r}")
+    if not isinstance(side, (int, np.integer)) or not 2 <= side <= MAX_SIDE:
+        raise ValueError(f"grid side {side!r} outside 2..{MAX_SIDE}")
+    if not isinstance(sim_s, (int, float)) or sim_s <= 0 or abs(sim_s / EPOCH_S - round(sim_s / EPOCH_S)) > 1e-9:
+        raise ValueError(f"simulated time {sim_s!r} is not a positive whole number of {EPOCH_S} s epochs")
+    N = int(side) ** 2
+    if kind == "lif_growth":
+        if not (spec.get("tR", 0) > 0 and 0 < spec.get("fE", 0) <= 1):
+            raise ValueError("lif_growth needs tR > 0 and 0 < fE <= 1")
+        max_spikes = int(np.float32(EPOCH_S) * np.float32(LIF_MAX_RATE))
+        desc = MD.net_desc(N, LIF_M, B.NEURON_LIF, B.SYN_DS, max_spikes, DT, spec["seed"])
+        return dict(kind=kind, desc=desc, arrays=None)
+    arrays = NW.izh_stdp_arrays(N, spec["seed"], k=min(100, N - 1), max_rate=IZH_MAX_RATE, epoch_s=EPOCH_S)
+    return dict(kind=kind, desc=NW.izh_stdp_desc(arrays[2], spec["seed"]), arrays=arrays)
+
+
+def footprint(p: dict) -> int:
+    """Device bytes of a planned network (its arena; growth is tiled, so no N^2 term)."""
+    return int(B.footprint(p["desc"]))
+
+
 class SweepNet:
     """One network of a sweep with its per-epoch work."""
 
-    def __init__(self, ctx: B.Context, spec: dict):
+    def __init__(self, ctx: B.Context, spec: dict, planned: dict | None = None):
         self.spec = spec
         self.kind = spec["kind"]
         self.N = spec["side"] ** 2
@@ -66,16 +108,13 @@ class SweepNet:
         self.steps_per_epoch = int(round(EPOCH_S / DT))
         self.model = None
         if self.kind == "lif_growth":
-            c = host.base_config()
-            c.update(width=spec["side"], height=spec["side"], layout="DynamicLayout", frac_exc=spec["fE"],
-                     frac_starter=0.1, target_rate=spec["tR"], seed=spec["seed"], epoch_duration=EPOCH_S,
-                     num_epochs=self.epochs, max_synapses_per_neuron=LIF_M, max_firing_rate=LIF_MAX_RATE)
-            self.model = host.Model(ctx, c)
+            self.model = host.Model(ctx, lif_config(spec))
             self.net = self.model.net
             self.max_spikes = int(np.float32(EPOCH_S) * np.float32(LIF_MAX_RATE))
         else:
-            k = min(100, self.N - 1)
-            self.net, meta = NW.izh_stdp(ctx, self.N, spec["seed"], k=k, max_rate=IZH_MAX_RATE, epoch_s=EPOCH_S)
+            arrays = planned["arrays"] if planned else None
+            self.net, meta = NW.izh_stdp(ctx, self.N, spec["seed"], k=min(100, self.N - 1), max_rate=IZH_MAX_RATE,
+                                         epoch_s=EPOCH_S, arrays=arrays)
             self.net.upload_input(B.INPUT_POISSON, neuron_type=meta["ntype"], fr_mean=POISSON_HZ,
                                   weight=POISSON_WEIGHT)
             self.max_spikes = meta["max_spikes"]

@@ -1,0 +1,35 @@
+"""The served form on the GPU: jobs of both families arriving over time, admitted under a tight HBM
+budget so that the resident set changes from tick to tick; every job's final state must be
+bit-identical to the same job run alone."""
+import pytest
+
+from braingrid_amd import serve as S
+from braingrid_amd import sweep as SW
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_jobs_equal_alone(hip_ctx):
+    jobs = S.served_requests(10, rate=50.0, seed=11, side_lo=10, side_hi=28, frac_lif=0.5, dur_lo=1, dur_hi=3)
+    jobs.append(S.Job(id=100, spec=dict(id=100, side=0, kind="izh_stdp", sim_s=1.0, seed=1), arrival=0.01))
+    print(f"\n[served {len(jobs)} jobs]", flush=True)
+    eng = S.DeviceEngine(hip_ctx)
+    fps = [SW.footprint(SW.plan(j.spec)) for j in jobs if j.spec["side"] > 0]
+    budget = 3 * max(fps)
+    srv = S.Server(engine=eng, budget=budget)
+    srv.run(jobs)
+    eng.close()
+    counts = srv.counts()
+    assert counts.get(S.DONE) == 10 and counts.get(S.FAILED) == 1, counts
+    assert max(u for _, u in srv.occupancy) <= budget
+    assert srv.stats()["latency_p99_s"] > 0
+    for j in jobs:
+        if j.state != S.DONE:
+            continue
+        alone = SW.SweepNet(hip_ctx, j.spec)
+        for _ in range(alone.epochs):
+            alone.net.advance(alone.steps_per_epoch)
+            alone.end_epoch()
+        fp = alone.fingerprint()
+        alone.close()
+        assert fp == j.result["fingerprint"], (j.id, j.spec)
