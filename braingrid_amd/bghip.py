@@ -142,6 +142,9 @@ def load():
         "bghip_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(ctypes.c_int64)]),
         "bghip_upload_input": (ctypes.c_int, [vp, ctypes.POINTER(Input), ctypes.POINTER(InputState)]),
+        "bghip_recorder_enable": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                                 _fp, _fp]),
+        "bghip_recorder_download": (ctypes.c_int, [vp, _fp, _fp, _fp, _fp]),
         "bghip_download_input": (ctypes.c_int, [vp, ctypes.POINTER(InputState)]),
         "bghip_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
         "bghip_batch_destroy": (None, [vp]),
@@ -164,7 +167,7 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_download_neurons", "bghip_download_synapses", "bghip_total_synapses",
             "bghip_download_index_map", "bghip_download_growth", "bghip_kernel_timing", "bghip_batch_create",
             "bghip_batch_destroy", "bghip_batch_set", "bghip_batch_advance", "bghip_batch_sync",
-            "bghip_upload_input", "bghip_download_input")
+            "bghip_upload_input", "bghip_download_input", "bghip_recorder_enable", "bghip_recorder_download")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -352,6 +355,26 @@ class Net:
         q = np.zeros(self.N, np.float32)
         self._c(load().bghip_download_growth(self.h, _ptr(r, _fp), _ptr(q, _fp)), "download_growth")
         return r, q
+
+    def recorder_enable(self, n_burst: int, n_spikes_hist: int, num_epochs: int, with_growth: bool = False,
+                        rates0=None, radii0=None):
+        """bghip_recorder_enable: spike histograms (and, with_growth, rates/radii rows) kept on the device."""
+        r0 = None if rates0 is None else np.ascontiguousarray(rates0, np.float32)
+        q0 = None if radii0 is None else np.ascontiguousarray(radii0, np.float32)
+        self._c(load().bghip_recorder_enable(self.h, n_burst, n_spikes_hist, num_epochs, 1 if with_growth else 0,
+                                             _ptr(r0, _fp), _ptr(q0, _fp)), "recorder_enable")
+        self._rec = (n_burst, n_spikes_hist, num_epochs, with_growth)
+
+    def recorder_download(self) -> dict:
+        nb, nsh, E, growth = self._rec
+        out = {"burstinessHist": np.zeros(nb, np.float32), "spikesHistory": np.zeros(nsh, np.float32)}
+        if growth:
+            out["ratesHistory"] = np.zeros((E + 1) * self.N, np.float32)
+            out["radiiHistory"] = np.zeros((E + 1) * self.N, np.float32)
+        self._c(load().bghip_recorder_download(self.h, _ptr(out["burstinessHist"], _fp), _ptr(out["spikesHistory"], _fp),
+                                               _ptr(out.get("ratesHistory"), _fp), _ptr(out.get("radiiHistory"), _fp)),
+                "recorder_download")
+        return out
 
     def upload_input(self, kind: int, neuron_type=None, fr_mean: float = 0.0, weight: float = 0.0, mask=None,
                      duration: float = 0.0, interval: float = 0.0, values=None, shift=None, state: dict | None = None):

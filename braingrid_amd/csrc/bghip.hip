@@ -49,6 +49,16 @@ struct bghip_net {
     uint32_t in_phase;     // Poisson input ring: reference delayIdx = (step - in_phase) mod 32
     int32_t *in_idx_host;  // N: delayIdx of neurons without input (their synapse never advances)
     uint8_t *in_mask_host; // N: copy of dn.in_mask
+    uint64_t *h_total;     // pinned landing slot of the synapse total after an index-map rebuild
+    bool total_pending;    // h_total not read yet (the rebuild is still in flight)
+    // on-device recorder (XmlRecorder / XmlGrowthRecorder histories)
+    uint32_t *d_burst, *d_shist;    // spike counts per 1 s and 10 ms bin
+    float *d_rates_hist, *d_radii_hist;
+    int64_t n_burst, n_shist;
+    int32_t rec_epochs, rec_epoch;  // rows (num_epochs + 1) and rows written so far
+    bool rec_on, rec_growth;
+    uint64_t hist_step0;            // first step whose spikes the next compile bins
+    bool hist_range_known;
     DevNet *d_self;        // device copy of dn (dn.self), refreshed when dn changes
     uint32_t *d_ev_off;    // per-step STDP: shard offsets of the event lists (dest, then syn)
     DevNet dn_shadow;      // dn as last copied to d_self
@@ -274,6 +284,14 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
         delete n;
         return set_err(ctx, BGHIP_EDEVICE, "hipStreamCreate failed");
     }
+    if (hipHostMalloc((void **)&n->h_total, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(n->stream);
+        (void)hipFree(n->arena);
+        delete n;
+        return set_err(ctx, BGHIP_ENOMEM, "hipHostMalloc failed");
+    }
+    *n->h_total = 0;
+    n->hist_range_known = true;
     HIP_TRY(ctx, hipEventCreate(&n->ev0));
     HIP_TRY(ctx, hipEventCreate(&n->ev1));
     HIP_TRY(ctx, hipEventCreateWithFlags(&n->ext_ev, hipEventDisableTiming));
@@ -306,6 +324,11 @@ extern "C" void bghip_net_destroy(bghip_net *n)
     (void)hipStreamDestroy(n->stream);
     free(n->in_idx_host);
     free(n->in_mask_host);
+    (void)hipHostFree(n->h_total);
+    (void)hipFree(n->d_burst);
+    (void)hipFree(n->d_shist);
+    (void)hipFree(n->d_rates_hist);
+    (void)hipFree(n->d_radii_hist);
     (void)hipFree(n->arena);
     delete n;
 }
@@ -350,15 +373,23 @@ extern "C" int bghip_upload_neurons(bghip_net *n, const bghip_neurons *h)
     CHK(h2d(n, dn.sp, h->summation_map, N)); CHK(h2d(n, dn.refr, h->nStepsInRefr, N));
     CHK(h2d(n, dn.cnt, h->spikeCount, N)); CHK(h2d(n, dn.off, h->spikeCountOffset, N));
     CHK(h2d(n, dn.hist, h->spike_history, N * (size_t)dn.max_spikes));
-    // last firing step = the newest history entry (getSpikeHistory(i, -1))
+    // last firing step = the newest history entry (getSpikeHistory(i, -1)); the earliest step among
+    // the counted entries bounds the recorder's bins at the next history update
     std::vector<uint64_t> last(N, BG_NO_SPIKE);
+    uint64_t first_counted = n->step;
     if (h->spike_history && h->spikeCount && h->spikeCountOffset && dn.max_spikes > 0)
         for (size_t i = 0; i < N; i++) {
             const int ms = dn.max_spikes;
             const int k = (int)(((int64_t)h->spikeCount[i] + h->spikeCountOffset[i] + ms - 1) % ms);
             last[i] = h->spike_history[i * (size_t)ms + (size_t)k];
+            for (int q = 0, idx = h->spikeCountOffset[i]; q < h->spikeCount[i] && q < ms; q++, idx++) {
+                if (idx >= ms) idx = 0;
+                first_counted = std::min<uint64_t>(first_counted, h->spike_history[i * (size_t)ms + (size_t)idx]);
+            }
         }
     CHK(h2d(n, dn.fired_step, last.data(), N));
+    n->hist_step0 = first_counted;
+    n->hist_range_known = true;
     if (izh) {
         CHK(h2d(n, dn.A, h->Aconst, N)); CHK(h2d(n, dn.B, h->Bconst, N)); CHK(h2d(n, dn.C, h->Cconst, N));
         CHK(h2d(n, dn.D, h->Dconst, N)); CHK(h2d(n, dn.u, h->u, N)); CHK(h2d(n, dn.C3, h->C3, N));
@@ -374,10 +405,22 @@ static int rebuild_incoming(bghip_net *n)
     unsigned blocks = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
     hipLaunchKernelGGL(k_fill_incoming, dim3(blocks), dim3(BG_BLOCK), 0, n->stream, dn);
     HIP_TRY(n->ctx, hipGetLastError());
-    HIP_TRY(n->ctx, hipMemcpyAsync(&n->total, n->d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, n->stream));
-    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    HIP_TRY(n->ctx, hipMemcpyAsync(n->h_total, n->d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, n->stream));
+    n->total_pending = true;
     n->out_valid = false;
     return BGHIP_OK;
+}
+
+// The synapse total, waiting for an index-map rebuild still in flight (growth updates do not wait
+// for it, so the updates of many networks overlap on their streams).
+static uint64_t net_total(bghip_net *n)
+{
+    if (n->total_pending) {
+        (void)hipStreamSynchronize(n->stream);
+        n->total = *n->h_total;
+        n->total_pending = false;
+    }
+    return n->total;
 }
 
 // The outgoing map (Core/SynapseIndexMap.h: outgoingSynapseIndexMap/Begin/Count) from the incoming
@@ -387,7 +430,7 @@ static int rebuild_incoming(bghip_net *n)
 static int build_outgoing(bghip_net *n)
 {
     DevNet &dn = n->dn;
-    const size_t N = dn.N, T = n->total;
+    const size_t N = dn.N, T = net_total(n);
     if (n->out_valid) return BGHIP_OK;
     HIP_TRY(n->ctx, hipMemsetAsync(n->d_out_count, 0, 4 * N, n->stream));
     HIP_TRY(n->ctx, hipMemsetAsync(n->d_cursor, 0, 4 * N, n->stream));
@@ -487,7 +530,9 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
     n->step = step;
     n->pending = false;
     n->kmax = block_kmax(n);
-    return rebuild_incoming(n);
+    CHK(rebuild_incoming(n));
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));   // the staging vectors above go out of scope
+    return BGHIP_OK;
 }
 
 extern "C" int bghip_upload_layout(bghip_net *n, const float *xloc, const float *yloc, const int32_t *ntype)
@@ -536,7 +581,7 @@ extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
     DevNet &dn = n->dn;
     const int nk = n->desc.neuron_model, sk = n->desc.synapse_model;
     const bool per_step = n->kmax <= 1 || sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
-    if (n->total == 0) {
+    if (net_total(n) == 0) {
         // no synapses: neurons only, many steps per launch
         flush_pending(n);
         for (uint64_t done = 0; done < n_steps;) {
@@ -649,8 +694,9 @@ extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
     CHK(rebuild_incoming(n));
     if (stats) {
         uint32_t st[4];
-        HIP_TRY(n->ctx, hipMemcpy(st, n->d_stats, 16, hipMemcpyDeviceToHost));
-        stats->total_synapses = (int64_t)n->total;
+        HIP_TRY(n->ctx, hipMemcpyAsync(st, n->d_stats, 16, hipMemcpyDeviceToHost, n->stream));
+        stats->total_synapses = (int64_t)net_total(n);   // waits for the stream
+        HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
         stats->created = st[0];
         stats->erased = st[1];
     }
@@ -662,10 +708,95 @@ extern "C" int bghip_update_history(bghip_net *n)
     if (!n) return BGHIP_EINVAL;
     net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
-    unsigned nb = (unsigned)((n->dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    const unsigned nb = (unsigned)((n->dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    float *rates_row = nullptr, *radii_row = nullptr;
+    if (n->rec_on) {
+        // bins the epoch's spikes can fall in: steps [hist_step0, step)
+        int64_t b_lo = 0, s_lo = 0;
+        int nb_loc = 0, ns_loc = 0;
+        if (n->hist_range_known && n->step > n->hist_step0) {
+            const double dt = (double)n->dn.dt;
+            const double x0 = (double)n->hist_step0 * dt, x1 = (double)(n->step - 1) * dt;
+            b_lo = (int64_t)(int)x0;
+            s_lo = (int64_t)(int)(x0 * 100);
+            const int64_t bw = (int64_t)(int)x1 - b_lo + 1, sw = (int64_t)(int)(x1 * 100) - s_lo + 1;
+            if (bw > 0 && sw > 0 && bw + sw <= 8192) { nb_loc = (int)bw; ns_loc = (int)sw; }
+        }
+        hipLaunchKernelGGL(k_record_spikes, dim3(nb), dim3(BG_BLOCK), sizeof(uint32_t) * (nb_loc + ns_loc), n->stream,
+                           n->dn, n->d_burst, n->n_burst, n->d_shist, n->n_shist, b_lo, nb_loc, s_lo, ns_loc);
+        const int e = n->rec_epoch + 1;
+        if (n->rec_growth && e <= n->rec_epochs) {
+            rates_row = n->d_rates_hist + (size_t)e * n->dn.N;
+            radii_row = n->d_radii_hist + (size_t)e * n->dn.N;
+        }
+        n->rec_epoch = e;
+    }
     hipLaunchKernelGGL(k_update_history, dim3(nb), dim3(BG_BLOCK), 0, n->stream, n->dn, n->growth.minRadius,
-                       n->has_growth ? 1 : 0);
+                       n->has_growth ? 1 : 0, rates_row, radii_row);
     HIP_TRY(n->ctx, hipGetLastError());
+    n->hist_step0 = n->step;
+    n->hist_range_known = true;
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_recorder_enable(bghip_net *n, int64_t n_burst, int64_t n_spikes_hist, int32_t num_epochs,
+                                     int32_t with_growth, const float *rates0, const float *radii0)
+{
+    if (!n || n_burst < 0 || n_spikes_hist < 0 || num_epochs < 0) return BGHIP_EINVAL;
+    net_begin(n);
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    (void)hipFree(n->d_burst); (void)hipFree(n->d_shist);
+    (void)hipFree(n->d_rates_hist); (void)hipFree(n->d_radii_hist);
+    n->d_burst = n->d_shist = nullptr;
+    n->d_rates_hist = n->d_radii_hist = nullptr;
+    const size_t N = n->dn.N, rows = (size_t)num_epochs + 1;
+    if (hipMalloc((void **)&n->d_burst, sizeof(uint32_t) * (size_t)std::max<int64_t>(n_burst, 1)) != hipSuccess ||
+        hipMalloc((void **)&n->d_shist, sizeof(uint32_t) * (size_t)std::max<int64_t>(n_spikes_hist, 1)) != hipSuccess)
+        return set_err(n->ctx, BGHIP_ENOMEM, "recorder: hipMalloc failed");
+    HIP_TRY(n->ctx, hipMemsetAsync(n->d_burst, 0, sizeof(uint32_t) * (size_t)n_burst, n->stream));
+    HIP_TRY(n->ctx, hipMemsetAsync(n->d_shist, 0, sizeof(uint32_t) * (size_t)n_spikes_hist, n->stream));
+    if (with_growth) {
+        if (hipMalloc((void **)&n->d_rates_hist, sizeof(float) * rows * N) != hipSuccess ||
+            hipMalloc((void **)&n->d_radii_hist, sizeof(float) * rows * N) != hipSuccess)
+            return set_err(n->ctx, BGHIP_ENOMEM, "recorder: hipMalloc failed");
+        HIP_TRY(n->ctx, hipMemsetAsync(n->d_rates_hist, 0, sizeof(float) * rows * N, n->stream));
+        HIP_TRY(n->ctx, hipMemsetAsync(n->d_radii_hist, 0, sizeof(float) * rows * N, n->stream));
+        CHK(h2d(n, n->d_rates_hist, rates0, N));
+        CHK(h2d(n, n->d_radii_hist, radii0, N));
+    }
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    n->n_burst = n_burst;
+    n->n_shist = n_spikes_hist;
+    n->rec_epochs = num_epochs;
+    n->rec_epoch = 0;
+    n->rec_growth = with_growth != 0;
+    n->rec_on = true;
+    return BGHIP_OK;
+}
+
+static void counts_to_float(const std::vector<uint32_t> &c, float *out)
+{
+    for (size_t k = 0; k < c.size(); k++) out[k] = (float)std::min<uint32_t>(c[k], 1u << 24);
+}
+
+extern "C" int bghip_recorder_download(bghip_net *n, float *burstinessHist, float *spikesHistory,
+                                       float *ratesHistory, float *radiiHistory)
+{
+    if (!n || !n->rec_on) return BGHIP_EINVAL;
+    net_begin(n);
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    std::vector<uint32_t> b((size_t)n->n_burst), sh((size_t)n->n_shist);
+    CHK(d2h(n, b.data(), n->d_burst, b.size()));
+    CHK(d2h(n, sh.data(), n->d_shist, sh.size()));
+    const size_t cells = ((size_t)n->rec_epochs + 1) * n->dn.N;
+    if (n->rec_growth) {
+        CHK(d2h(n, ratesHistory, n->d_rates_hist, cells));
+        CHK(d2h(n, radiiHistory, n->d_radii_hist, cells));
+    }
+    CHK(bghip_sync(n));
+    if (burstinessHist) counts_to_float(b, burstinessHist);
+    if (spikesHistory) counts_to_float(sh, spikesHistory);
     return BGHIP_OK;
 }
 
@@ -689,7 +820,7 @@ extern "C" int bghip_download_neurons(bghip_net *n, bghip_neurons *h)
     return bghip_sync(n);
 }
 
-extern "C" int64_t bghip_total_synapses(bghip_net *n) { return n ? (int64_t)n->total : -1; }
+extern "C" int64_t bghip_total_synapses(bghip_net *n) { return n ? (int64_t)net_total(n) : -1; }
 
 extern "C" int bghip_download_synapses(bghip_net *n, bghip_synapses *h)
 {
@@ -757,7 +888,7 @@ extern "C" int bghip_download_index_map(bghip_net *n, bghip_index_map *h)
     net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     DevNet &dn = n->dn;
-    size_t N = dn.N, T = n->total;
+    size_t N = dn.N, T = net_total(n);
     CHK(build_outgoing(n));
     std::vector<uint32_t> cnt(N);
     CHK(d2h(n, h->incomingSynapseIndexMap, dn.in_map, T)); CHK(d2h(n, h->incomingSynapseBegin, dn.in_begin, N));
@@ -921,7 +1052,7 @@ extern "C" int bghip_download_input(bghip_net *n, bghip_input_state *st)
 enum { BG_MODE_QUIET = 0, BG_MODE_FUSED = 1, BG_MODE_STEP = 2 };
 static int net_mode(const bghip_net *x)
 {
-    if (x->total == 0) return BG_MODE_QUIET;
+    if (net_total(const_cast<bghip_net *>(x)) == 0) return BG_MODE_QUIET;
     const int sk = x->desc.synapse_model;
     return (x->kmax <= 1 || sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP) ? BG_MODE_STEP : BG_MODE_FUSED;
 }
@@ -940,6 +1071,13 @@ struct BatchGroup {
     uint2 *d_evs = nullptr;        // their synapse list (sum of the members' slots)
     uint32_t total = 0;
     int kmax = 1;
+    // pinned staging of the per-call tables, so an advance call does not wait for the device;
+    // `staged` marks the copies of the previous call, which must finish before the next rewrite
+    BatchNet *h_nets = nullptr;
+    uint64_t *h_tprev = nullptr;
+    int *h_sel = nullptr;
+    hipEvent_t staged = nullptr;
+    bool staged_pending = false;
 };
 
 struct bghip_batch {
@@ -961,6 +1099,10 @@ static void free_groups(bghip_batch *b)
         (void)hipFree(g.d_sel);
         (void)hipFree(g.d_ev);
         (void)hipFree(g.d_evs);
+        (void)hipHostFree(g.h_nets);
+        (void)hipHostFree(g.h_tprev);
+        (void)hipHostFree(g.h_sel);
+        if (g.staged) (void)hipEventDestroy(g.staged);
     }
     b->groups.clear();
 }
@@ -1037,6 +1179,10 @@ static int regroup(bghip_batch *b)
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sbase, sizeof(uint64_t) * (nn + 1)));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_tprev, sizeof(uint64_t) * nn));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sel, sizeof(int) * nn));
+        HIP_TRY(b->ctx, hipHostMalloc((void **)&g.h_nets, sizeof(BatchNet) * nn, hipHostMallocDefault));
+        HIP_TRY(b->ctx, hipHostMalloc((void **)&g.h_tprev, sizeof(uint64_t) * nn, hipHostMallocDefault));
+        HIP_TRY(b->ctx, hipHostMalloc((void **)&g.h_sel, sizeof(int) * nn, hipHostMallocDefault));
+        HIP_TRY(b->ctx, hipEventCreateWithFlags(&g.staged, hipEventDisableTiming));
         if (g.mode == BG_MODE_STEP && (g.sk == BGHIP_SYN_STDP || g.sk == BGHIP_SYN_DYNSTDP)) {
             std::vector<EvPart> parts(nn);
             for (int k = 0; k < nn; k++) {
@@ -1096,7 +1242,11 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
         }
     for (auto &g : b->groups) {
         const int nn = (int)g.members.size();
-        std::vector<BatchNet> tab(nn);
+        if (g.staged_pending) {   // the previous call's copies out of the staging buffers
+            HIP_TRY(b->ctx, hipEventSynchronize(g.staged));
+            g.staged_pending = false;
+        }
+        BatchNet *tab = g.h_nets;
         for (int k = 0; k < nn; k++) {
             bghip_net *x = b->nets[g.members[k]];
             tab[k].n = x->dn;
@@ -1105,9 +1255,7 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
             tab[k].sel = x->sel;
             tab[k].pull = x->pending ? 1 : 0;
         }
-        HIP_TRY(b->ctx, hipMemcpyAsync(g.d_nets, tab.data(), sizeof(BatchNet) * nn, hipMemcpyHostToDevice, b->stream));
-        // the table copy reads host memory that is reused by the next call: wait for it
-        HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
+        HIP_TRY(b->ctx, hipMemcpyAsync(g.d_nets, tab, sizeof(BatchNet) * nn, hipMemcpyHostToDevice, b->stream));
         if (g.mode == BG_MODE_QUIET) {
             for (uint64_t done = 0; done < n_steps;) {
                 const int k = (int)std::min<uint64_t>(BG_QUIET_STEPS, n_steps - done);
@@ -1135,8 +1283,8 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
                 launch_fused_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, g.sk, g.w32, g.kmax, K, Kfull,
                                    (int)l);
             }
-            std::vector<uint64_t> tprev(nn);
-            std::vector<int> sel(nn);
+            uint64_t *tprev = g.h_tprev;
+            int *sel = g.h_sel;
             for (int k = 0; k < nn; k++) {
                 bghip_net *x = b->nets[g.members[k]];
                 tprev[k] = x->step + (L - 1) * (uint64_t)Kfull;
@@ -1146,9 +1294,8 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
                 x->step += n_steps;
                 x->pending = false;
             }
-            HIP_TRY(b->ctx, hipMemcpyAsync(g.d_tprev, tprev.data(), sizeof(uint64_t) * nn, hipMemcpyHostToDevice,
-                                           b->stream));
-            HIP_TRY(b->ctx, hipMemcpyAsync(g.d_sel, sel.data(), sizeof(int) * nn, hipMemcpyHostToDevice, b->stream));
+            HIP_TRY(b->ctx, hipMemcpyAsync(g.d_tprev, tprev, sizeof(uint64_t) * nn, hipMemcpyHostToDevice, b->stream));
+            HIP_TRY(b->ctx, hipMemcpyAsync(g.d_sel, sel, sizeof(int) * nn, hipMemcpyHostToDevice, b->stream));
             uint64_t slots = 0;
             for (int k = 0; k < nn; k++) slots += b->nets[g.members[k]]->dn.S;
             if (slots) {
@@ -1156,8 +1303,9 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
                 hipLaunchKernelGGL(k_flush_batch, dim3(blocks), dim3(BG_BLOCK), 0, b->stream, g.d_nets, g.d_sbase, nn,
                                    g.d_tprev, g.d_sel);
             }
-            HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
         }
+        HIP_TRY(b->ctx, hipEventRecord(g.staged, b->stream));
+        g.staged_pending = true;
     }
     HIP_TRY(b->ctx, hipGetLastError());
     HIP_TRY(b->ctx, hipEventRecord(b->done, b->stream));

@@ -226,13 +226,58 @@ __global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c,
 
 // Recorder clamp + clearSpikeCounts (Recorders/XmlGrowthRecorder.cpp:89-90,
 // Neurons/AllSpikingNeurons.cpp:79-87).
-__global__ void k_update_history(DevNet n, float minRadius, int clamp)
+// End of epoch: XmlGrowthRecorder's rates/radii rows with the minRadius clamp
+// (Recorders/XmlGrowthRecorder.cpp:82-96; rows only when the recorder keeps them on the device)
+// and AllSpikingNeurons::clearSpikeCounts (Neurons/AllSpikingNeurons.cpp:79-87).
+__global__ void k_update_history(DevNet n, float minRadius, int clamp, float *rates_row, float *radii_row)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n.N) return;
+    if (rates_row) rates_row[i] = n.rates[i];
     if (clamp && n.radii[i] < minRadius) n.radii[i] = minRadius;
+    if (radii_row) radii_row[i] = n.radii[i];
     n.off[i] = (n.cnt[i] + n.off[i]) % n.max_spikes;
     n.cnt[i] = 0;
+}
+
+// XmlRecorder::compileHistories (Recorders/XmlRecorder.cpp:71-104) on the device: every spike
+// recorded since the last clear is counted in 1 s bin int(s*dt) and 10 ms bin int(s*dt*100).
+// Integer counts make the result independent of the order of the atomic adds; the reference's
+// float bins equal min(count, 2^24) (1.0 added to 2^24 in float rounds back to 2^24), which is
+// how counts are converted at download. Bins [b_lo, b_lo + nb_loc) and [s_lo, s_lo + ns_loc)
+// (the epoch's step range) are counted in LDS first, one global atomic per block and bin.
+__global__ __launch_bounds__(BG_BLOCK) void k_record_spikes(DevNet n, uint32_t *burst, int64_t nb, uint32_t *shist,
+                                                            int64_t nsh, int64_t b_lo, int nb_loc, int64_t s_lo,
+                                                            int ns_loc)
+{
+    extern __shared__ uint32_t lds_bins[];
+    for (int k = threadIdx.x; k < nb_loc + ns_loc; k += blockDim.x) lds_bins[k] = 0u;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n.N) {
+        const int ms = n.max_spikes, cnt = n.cnt[i];
+        int idx = n.off[i];
+        for (int q = 0; q < cnt; q++, idx++) {
+            if (idx >= ms) idx = 0;
+            const double x = (double)n.hist[(uint64_t)i * ms + idx] * (double)n.dt;
+            const int i1 = (int)x, i2 = (int)(x * 100);
+            if (i1 >= 0 && i1 < nb) {
+                const int64_t r = (int64_t)i1 - b_lo;
+                if (r >= 0 && r < nb_loc) atomicAdd(&lds_bins[r], 1u);
+                else atomicAdd(&burst[i1], 1u);
+            }
+            if (i2 >= 0 && i2 < nsh) {
+                const int64_t r = (int64_t)i2 - s_lo;
+                if (r >= 0 && r < ns_loc) atomicAdd(&lds_bins[nb_loc + r], 1u);
+                else atomicAdd(&shist[i2], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nb_loc; k += blockDim.x)
+        if (lds_bins[k]) atomicAdd(&burst[b_lo + k], lds_bins[k]);
+    for (int k = threadIdx.x; k < ns_loc; k += blockDim.x)
+        if (lds_bins[nb_loc + k]) atomicAdd(&shist[s_lo + k], lds_bins[nb_loc + k]);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -191,7 +191,11 @@ class Model:
     def advance_epoch(self):
         self.ctx.check(load().bghost_model_advance_epoch(self.h), "advance_epoch")
 
-    def update_connections(self):
+    def update_connections(self, stats: bool = True):
+        """The growth update; with stats=False it does not wait for the device."""
+        if not stats:
+            self.ctx.check(load().bghost_model_update_connections(self.h, None), "update_connections")
+            return None
         st = B.GrowthStats()
         self.ctx.check(load().bghost_model_update_connections(self.h, ctypes.byref(st)), "update_connections")
         return st

@@ -90,6 +90,7 @@ class HipSpikingModel {
         return bghip_update_connections(net_, st);
     }
     int updateHistory();
+    int syncRecorder();
     int saveData(const char *path);
     void cleanupSim()
     {
@@ -98,6 +99,8 @@ class HipSpikingModel {
     }
     int setRadii(float r)
     {
+        int rc = syncRecorder();
+        if (rc) return rc;
         for (auto &x : radii_) x = r;
         return bghip_upload_growth(net_, &growth_, radii_.data(), rates_.data());
     }
@@ -115,6 +118,7 @@ class HipSpikingModel {
     MTRand rng_;
     bghip_net *net_ = nullptr;
     int N_ = 0, max_spikes_ = 0, epoch_ = 0;
+    bool recorder_stale_ = false;   // device histories newer than the host copies
     std::vector<float> xloc_, yloc_;
     std::vector<int32_t> ntype_;
     std::vector<uint8_t> starter_;
@@ -284,41 +288,32 @@ int HipSpikingModel::setupSim()
     }
     if ((rc = bghip_upload_neurons(net_, &h))) return rc;
     if ((rc = bghip_upload_layout(net_, xloc_.data(), yloc_.data(), ntype_.data()))) return rc;
-    return bghip_upload_growth(net_, &growth_, radii_.data(), rates_.data());
+    if ((rc = bghip_upload_growth(net_, &growth_, radii_.data(), rates_.data()))) return rc;
+    // the recorder's histories live on the device (XmlRecorder::initDefaultValues and
+    // XmlGrowthRecorder::initDefaultValues give row 0 and zero bins)
+    const bool growth = p_.conn_kind == BGHOST_CONN_GROWTH;
+    return bghip_recorder_enable(net_, (int64_t)burst_.size(), (int64_t)spikesHist_.size(), E, growth ? 1 : 0,
+                                 ratesHistory_.data(), radiiHistory_.data());
 }
 
 int HipSpikingModel::updateHistory()
-{
-    bghip_neurons h;
-    memset(&h, 0, sizeof h);
-    h.spikeCount = spikeCount_.data();
-    h.spikeCountOffset = spikeOffset_.data();
-    h.spike_history = hist_.data();
-    int rc = bghip_download_neurons(net_, &h);
+{   // XmlGrowthRecorder::compileHistories (Recorders/XmlGrowthRecorder.cpp:72-97) and
+    // clearSpikeCounts, both on the device; the host copies are refreshed when read
+    int rc = bghip_update_history(net_);
     if (rc) return rc;
-    const float dt = p_.dt;
-    for (int i = 0; i < N_; i++) {   // Recorders/XmlRecorder.cpp:77-100
-        for (int q = 0, idx = spikeOffset_[i]; q < spikeCount_[i]; q++, idx++) {
-            if (idx >= max_spikes_) idx = 0;
-            uint64_t s = hist_[(size_t)i * max_spikes_ + idx];
-            int i1 = (int)((double)s * (double)dt), i2 = (int)((double)s * (double)dt * 100);
-            if (i1 >= 0 && (size_t)i1 < burst_.size()) burst_[i1] = (float)((double)burst_[i1] + 1.0);
-            if (i2 >= 0 && (size_t)i2 < spikesHist_.size()) spikesHist_[i2] = (float)((double)spikesHist_[i2] + 1.0);
-        }
-    }
     epoch_++;
-    if (p_.conn_kind == BGHOST_CONN_GROWTH) {
-        if ((rc = bghip_download_growth(net_, radii_.data(), rates_.data()))) return rc;
-        if (epoch_ <= (p_.num_epochs > 0 ? p_.num_epochs : 1)) {   // Recorders/XmlGrowthRecorder.cpp:82-96
-            for (int i = 0; i < N_; i++) {
-                ratesHistory_[(size_t)epoch_ * N_ + i] = rates_[i];
-                float r = radii_[i] < p_.min_radius ? p_.min_radius : radii_[i];
-                radiiHistory_[(size_t)epoch_ * N_ + i] = r;
-            }
-        }
-    }
-    if ((rc = bghip_update_history(net_))) return rc;
-    if (p_.conn_kind == BGHOST_CONN_GROWTH) rc = bghip_download_growth(net_, radii_.data(), rates_.data());
+    recorder_stale_ = true;
+    return BGHIP_OK;
+}
+
+int HipSpikingModel::syncRecorder()
+{
+    if (!recorder_stale_) return BGHIP_OK;
+    const bool growth = p_.conn_kind == BGHOST_CONN_GROWTH;
+    int rc = bghip_recorder_download(net_, burst_.data(), spikesHist_.data(), growth ? ratesHistory_.data() : nullptr,
+                                     growth ? radiiHistory_.data() : nullptr);
+    if (!rc && growth) rc = bghip_download_growth(net_, radii_.data(), rates_.data());
+    if (!rc) recorder_stale_ = false;
     return rc;
 }
 
@@ -341,6 +336,8 @@ static void put_vector(FILE *f, const char *name, const float *v, int n)
 
 int HipSpikingModel::saveData(const char *path)
 {   // XmlGrowthRecorder::saveSimData (Recorders/XmlGrowthRecorder.cpp:104-154)
+    int rc = syncRecorder();
+    if (rc) return rc;
     FILE *f = fopen(path, "w");
     if (!f) return BGHIP_EINVAL;
     const int E = p_.num_epochs > 0 ? p_.num_epochs : 1;
@@ -373,6 +370,7 @@ int HipSpikingModel::saveData(const char *path)
 
 const void *HipSpikingModel::array(const char *name, int64_t *count)
 {
+    if (syncRecorder()) return nullptr;
     struct Item { const char *n; const void *p; size_t c; };
     const Item items[] = {
         {"xloc", xloc_.data(), xloc_.size()}, {"yloc", yloc_.data(), yloc_.size()},

@@ -17,6 +17,7 @@ policy is testable without a GPU.
 from __future__ import annotations
 
 import heapq
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -65,13 +66,16 @@ def served_requests(n: int = 2000, rate: float = 20.0, seed: int = 7, side_lo: i
 
 
 class DeviceEngine:
-    """Networks on the GPU through the C ABI."""
+    """Networks on the GPU through the C ABI. A job's result is its recorder output (spike-count
+    histograms, final radii); with fingerprint=True it also carries a digest of the whole final
+    network state (a full download, for comparisons in tests)."""
 
-    def __init__(self, ctx):
+    def __init__(self, ctx, fingerprint: bool = False):
         from . import bghip as B
         self.ctx = ctx
         self.batch = B.Batch(ctx)
         self._members = None
+        self.fingerprint = fingerprint
 
     def total_bytes(self) -> int:
         return self.ctx.mem_info()[1]
@@ -94,12 +98,18 @@ class DeviceEngine:
 
     def end_epoch(self, h):
         h.end_epoch()
+
+    def check(self, h):
         bits = h.net.error_bits()
         if bits:
             raise RuntimeError(f"device error bits 0x{bits:x}")
 
     def result(self, h) -> dict:
-        return {"fingerprint": h.fingerprint(), "synapses": h.net.total_synapses, "step": h.net.step}
+        out = h.summary()
+        out["step"] = h.net.step
+        if self.fingerprint:
+            out["fingerprint"] = h.fingerprint()
+        return out
 
     def release(self, h):
         h.close()
@@ -124,7 +134,8 @@ class Server:
     t0: float = 0.0
     occupancy: list = field(default_factory=list)  # (clock, used bytes) per tick
     ticks: int = 0
-    phase_s: dict = field(default_factory=lambda: {"admit": 0.0, "advance": 0.0, "epoch": 0.0})
+    phase_s: dict = field(default_factory=lambda: {"admit": 0.0, "advance": 0.0, "epoch_issue": 0.0,
+                                                   "epoch_wait": 0.0, "retire": 0.0})
 
     # --- requests -----------------------------------------------------------------------------
     def submit(self, job: Job):
@@ -158,6 +169,23 @@ class Server:
         if self.log:
             self.log(f"job {job.id} {state}: {err}")
 
+    def prepare(self, limit: int = 8):
+        """Plans (host-side setup of) up to `limit` queued jobs in arrival order."""
+        done = 0
+        for arrival, jid in sorted(self.queue)[: 4 * limit]:
+            job = self.jobs[jid]
+            if job.state != QUEUED or job.plan is not None:
+                continue
+            try:
+                job.plan = self.engine.plan(job.spec)
+                job.footprint = int(self.engine.footprint(job.plan))
+            except Exception as e:
+                job.plan = None
+                job.error = f"{type(e).__name__}: {e}"
+            done += 1
+            if done >= limit:
+                break
+
     def admit(self):
         """Admits queued jobs in arrival order while they fit; up to `lookahead` jobs that do not
         fit yet are passed over (and later ones admitted) before the round ends."""
@@ -172,6 +200,9 @@ class Server:
             if head_blocked:
                 considered += 1
                 kept.append((arrival, jid))
+                continue
+            if job.error is not None:      # planning failed earlier
+                self._fail(job, FAILED, job.error)
                 continue
             try:
                 if job.plan is None:
@@ -206,20 +237,39 @@ class Server:
             heapq.heappush(self.queue, item)
 
     def tick(self):
-        """Admit, advance every resident network by one epoch, do the epoch work, retire."""
+        """Advance every resident network by one epoch; while the device runs it, prepare and admit
+        queued jobs (they join from the next tick on); then do the epoch work and retire."""
         c0 = self.clock()
-        self.admit()
-        c1 = self.clock()
-        self.phase_s["admit"] += c1 - c0
-        self.occupancy.append((c1, self.used))
-        if not self.active:
-            return
-        self.engine.advance([j.handle for j in self.active], self.epoch_steps)
-        c2 = self.clock()
-        self.phase_s["advance"] += c2 - c1
-        for job in list(self.active):
+        advanced = list(self.active)
+        if advanced:
+            # the advance call returns only when the device has taken most of the epoch's launches,
+            # so the host-side planning of queued jobs runs beside it on a worker thread (the C
+            # calls and numpy release the interpreter lock)
+            worker = threading.Thread(target=self.prepare)
+            worker.start()
             try:
+                self.engine.advance([j.handle for j in advanced], self.epoch_steps)
+            finally:
+                worker.join()
+        c1 = self.clock()
+        self.admit()
+        c2 = self.clock()
+        self.phase_s["advance"] += c1 - c0
+        self.phase_s["admit"] += c2 - c1
+        self.occupancy.append((c2, self.used))
+        if not advanced:
+            return
+        # every network's epoch work is queued first, so the networks' streams overlap, then checked
+        for job in advanced:
+            if job.state == ACTIVE:
                 self.engine.end_epoch(job.handle)
+        c3 = self.clock()
+        finished = []
+        for job in advanced:
+            if job.state != ACTIVE:
+                continue
+            try:
+                self.engine.check(job.handle)
             except Exception as e:
                 self.active.remove(job)
                 self._release(job)
@@ -227,12 +277,18 @@ class Server:
                 continue
             job.epochs_done += 1
             if job.epochs_done >= job.epochs:
-                job.result = self.engine.result(job.handle)
-                self.active.remove(job)
-                self._release(job)
-                job.state = DONE
-                job.finished = self.clock()
-        self.phase_s["epoch"] += self.clock() - c2
+                finished.append(job)
+        c4 = self.clock()
+        for job in finished:
+            job.result = self.engine.result(job.handle)
+            self.active.remove(job)
+            self._release(job)
+            job.state = DONE
+            job.finished = self.clock()
+        c5 = self.clock()
+        self.phase_s["epoch_issue"] += c3 - c2
+        self.phase_s["epoch_wait"] += c4 - c3
+        self.phase_s["retire"] += c5 - c4
         self.ticks += 1
 
     # --- open-loop driver ---------------------------------------------------------------------

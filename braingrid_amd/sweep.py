@@ -118,11 +118,25 @@ class SweepNet:
             self.net.upload_input(B.INPUT_POISSON, neuron_type=meta["ntype"], fr_mean=POISSON_HZ,
                                   weight=POISSON_WEIGHT)
             self.max_spikes = meta["max_spikes"]
+            # the spike-count histograms of XmlRecorder, kept on the device
+            self.net.recorder_enable(int(spec["sim_s"]), int(spec["sim_s"] * 100), self.epochs)
         self.synapses = self.net.total_synapses
 
-    def end_epoch(self):
+    def summary(self) -> dict:
+        """The job's output: XmlRecorder's spike-count histograms (1 s and 10 ms bins), and for
+        growth networks the final radii; a few KB at most."""
         if self.model is not None:
-            self.model.update_connections()
+            out = {"burstinessHist": self.model.array("burstinessHist"),
+                   "spikesHistory": self.model.array("spikesHistory"), "radii": self.model.array("radii")}
+        else:
+            out = self.net.recorder_download()
+        out["synapses"] = self.net.total_synapses
+        return out
+
+    def end_epoch(self):
+        """Growth update and history step, queued on the network's stream (not waited for)."""
+        if self.model is not None:
+            self.model.update_connections(stats=False)
             self.model.update_history()
         else:
             self.net.update_history()

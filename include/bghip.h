@@ -202,6 +202,17 @@ int bghip_update_connections(bghip_net *net, bghip_growth_stats *stats);
  * recorder's minRadius clamp (Recorders/XmlGrowthRecorder.cpp:89-90) and clears spike counts
  * (Neurons/AllSpikingNeurons.cpp:79-87). Download spikes and rates/radii before calling it. */
 int bghip_update_history(bghip_net *net);
+/* On-device growth recorder: once enabled, bghip_update_history also bins the epoch's spikes
+ * into burstinessHist (1 s bins) and spikesHistory (10 ms bins) of XmlRecorder::compileHistories
+ * (Recorders/XmlRecorder.cpp:71-104) and, with_growth, stores row e of ratesHistory and of the
+ * clamped radiiHistory of XmlGrowthRecorder::compileHistories (Recorders/XmlGrowthRecorder.cpp:72-97),
+ * all on the device, so no spike history crosses PCIe per epoch. rates0/radii0 are row 0
+ * (XmlGrowthRecorder::initDefaultValues, :27-37), N floats each. Histories are (num_epochs+1) x N. */
+int bghip_recorder_enable(bghip_net *net, int64_t n_burst, int64_t n_spikes_hist, int32_t num_epochs,
+                          int32_t with_growth, const float *rates0, const float *radii0);
+/* Copies the recorder's histories to the host (NULL pointers are skipped); blocking. */
+int bghip_recorder_download(bghip_net *net, float *burstinessHist, float *spikesHistory, float *ratesHistory,
+                            float *radiiHistory);
 /* waits for the network's stream; returns BGHIP_ESTATE if a device error bit is set */
 int bghip_sync(bghip_net *net);
 uint32_t bghip_net_error_bits(bghip_net *net);

@@ -13,7 +13,7 @@ def test_served_jobs_equal_alone(hip_ctx):
     jobs = S.served_requests(10, rate=50.0, seed=11, side_lo=10, side_hi=28, frac_lif=0.5, dur_lo=1, dur_hi=3)
     jobs.append(S.Job(id=100, spec=dict(id=100, side=0, kind="izh_stdp", sim_s=1.0, seed=1), arrival=0.01))
     print(f"\n[served {len(jobs)} jobs]", flush=True)
-    eng = S.DeviceEngine(hip_ctx)
+    eng = S.DeviceEngine(hip_ctx, fingerprint=True)
     fps = [SW.footprint(SW.plan(j.spec)) for j in jobs if j.spec["side"] > 0]
     budget = 3 * max(fps)
     srv = S.Server(engine=eng, budget=budget)
@@ -31,5 +31,9 @@ def test_served_jobs_equal_alone(hip_ctx):
             alone.net.advance(alone.steps_per_epoch)
             alone.end_epoch()
         fp = alone.fingerprint()
+        summ = alone.summary()
         alone.close()
         assert fp == j.result["fingerprint"], (j.id, j.spec)
+        for k in ("burstinessHist", "spikesHistory"):
+            assert (summ[k] == j.result[k]).all(), (j.id, k)
+        assert summ[k].sum() > 0

@@ -41,6 +41,8 @@ class FakeEngine:
 
     def end_epoch(self, h):
         h["epochs"] += 1
+
+    def check(self, h):
         if h["spec"].get("kind") == "boom" and h["epochs"] == 2:
             raise RuntimeError("device error bits 0x2")
 
