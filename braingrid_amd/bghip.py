@@ -145,6 +145,12 @@ def load():
         "bghip_recorder_enable": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                                  _fp, _fp]),
         "bghip_recorder_download": (ctypes.c_int, [vp, _fp, _fp, _fp, _fp]),
+        "bghip_net_save": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "bghip_net_get_desc": (ctypes.c_int, [vp, ctypes.POINTER(NetDesc)]),
+        "bghip_recorder_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                               ctypes.POINTER(ctypes.c_int32)]),
+        "bghip_net_load": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(vp)]),
         "bghip_download_input": (ctypes.c_int, [vp, ctypes.POINTER(InputState)]),
         "bghip_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
         "bghip_batch_destroy": (None, [vp]),
@@ -167,7 +173,8 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_download_neurons", "bghip_download_synapses", "bghip_total_synapses",
             "bghip_download_index_map", "bghip_download_growth", "bghip_kernel_timing", "bghip_batch_create",
             "bghip_batch_destroy", "bghip_batch_set", "bghip_batch_advance", "bghip_batch_sync",
-            "bghip_upload_input", "bghip_download_input", "bghip_recorder_enable", "bghip_recorder_download")
+            "bghip_upload_input", "bghip_download_input", "bghip_recorder_enable", "bghip_recorder_download",
+            "bghip_net_save", "bghip_net_load", "bghip_net_get_desc", "bghip_recorder_info")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -231,6 +238,37 @@ class Net:
 
     def _c(self, rc, what):
         self.ctx.check(rc, what)
+
+    # ---- checkpoints
+    def save(self, path: str):
+        """bghip_net_save: the whole network state as a versioned binary file."""
+        self._c(load().bghip_net_save(self.h, os.fsencode(path)), "net_save")
+
+    @classmethod
+    def load(cls, ctx: Context, path: str) -> "Net":
+        """bghip_net_load: a new network holding a checkpoint's state."""
+        h = ctypes.c_void_p()
+        ctx.check(load().bghip_net_load(ctx.h, os.fsencode(path), ctypes.byref(h)), "net_load")
+        return cls.wrap(ctx, h.value)
+
+    @classmethod
+    def wrap(cls, ctx: Context, handle) -> "Net":
+        """A Net object over an existing handle; close() destroys the handle."""
+        net = cls.__new__(cls)
+        net.ctx, net.h = ctx, ctypes.c_void_p(handle)
+        d = NetDesc()
+        ctx.check(load().bghip_net_get_desc(net.h, ctypes.byref(d)), "net_get_desc")
+        net.desc, net.N, net.M, net.W32 = d, d.num_neurons, d.max_synapses_per_neuron, d.ring_bits // 32
+        return net
+
+    def recorder_info(self):
+        """(n_burst, n_spikes_hist, num_epochs, with_growth, epoch) of the on-device recorder, or
+        None when it is not enabled."""
+        nb, nsh = ctypes.c_int64(), ctypes.c_int64()
+        E, g, e = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._c(load().bghip_recorder_info(self.h, ctypes.byref(nb), ctypes.byref(nsh), ctypes.byref(E),
+                                           ctypes.byref(g), ctypes.byref(e)), "recorder_info")
+        return None if e.value < 0 else (nb.value, nsh.value, E.value, bool(g.value), e.value)
 
     # ---- uploads
     def upload_neurons(self, arrs: dict):
@@ -363,10 +401,12 @@ class Net:
         q0 = None if radii0 is None else np.ascontiguousarray(radii0, np.float32)
         self._c(load().bghip_recorder_enable(self.h, n_burst, n_spikes_hist, num_epochs, 1 if with_growth else 0,
                                              _ptr(r0, _fp), _ptr(q0, _fp)), "recorder_enable")
-        self._rec = (n_burst, n_spikes_hist, num_epochs, with_growth)
 
     def recorder_download(self) -> dict:
-        nb, nsh, E, growth = self._rec
+        info = self.recorder_info()
+        if info is None:
+            raise BgHipError("recorder_download: the recorder is not enabled")
+        nb, nsh, E, growth, _ = info
         out = {"burstinessHist": np.zeros(nb, np.float32), "spikesHistory": np.zeros(nsh, np.float32)}
         if growth:
             out["ratesHistory"] = np.zeros((E + 1) * self.N, np.float32)

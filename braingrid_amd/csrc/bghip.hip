@@ -334,6 +334,24 @@ extern "C" void bghip_net_destroy(bghip_net *n)
 }
 
 extern "C" uint64_t bghip_net_step(const bghip_net *n) { return n ? n->step : 0; }
+extern "C" int bghip_net_get_desc(const bghip_net *n, bghip_net_desc *out)
+{
+    if (!n || !out) return BGHIP_EINVAL;
+    *out = n->desc;
+    return BGHIP_OK;
+}
+extern "C" int bghip_recorder_info(const bghip_net *n, int64_t *n_burst, int64_t *n_spikes_hist, int32_t *num_epochs,
+                                   int32_t *with_growth, int32_t *epoch)
+{
+    if (!n) return BGHIP_EINVAL;
+    const bool on = n->rec_on;
+    if (n_burst) *n_burst = on ? n->n_burst : 0;
+    if (n_spikes_hist) *n_spikes_hist = on ? n->n_shist : 0;
+    if (num_epochs) *num_epochs = on ? n->rec_epochs : 0;
+    if (with_growth) *with_growth = on && n->rec_growth ? 1 : 0;
+    if (epoch) *epoch = on ? n->rec_epoch : -1;
+    return BGHIP_OK;
+}
 
 template <typename T>
 static int h2d(bghip_net *n, T *dst, const T *src, size_t count)
@@ -1043,6 +1061,158 @@ extern "C" int bghip_download_input(bghip_net *n, bghip_input_state *st)
             st->delayQueue[i] = on && phase ? (ring[i] >> phase) | (ring[i] << (32u - phase)) : ring[i];
         if (st->delayIdx) st->delayIdx[i] = on ? di : n->in_idx_host[i];
     }
+    return BGHIP_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Checkpoints: a versioned binary image of one network. The arena is copied as raw bytes (its
+// layout is a function of the description alone, so the loading library re-derives every device
+// pointer); the host-side state that is not in the arena follows as explicit little-endian fields.
+// Philox noise and Poisson draws are functions of (seed, neuron, step), so the step counter is the
+// whole random-number state.
+static const char kCkptMagic[8] = {'B', 'G', 'H', 'I', 'P', 'N', 'E', 'T'};
+static const uint32_t kCkptVersion = 1;
+
+struct CkptHost {   // fixed-size record of the host-side state
+    uint64_t step, last_t0, total, hist_step0, in_cyc_off;
+    int64_t n_burst, n_shist;
+    int32_t kmax, sel, min_delay_uploaded, ds_typed, in_kind, in_dur, in_cycle, rec_epochs, rec_epoch;
+    uint32_t in_phase;
+    uint8_t pending, has_layout, has_growth, rec_on, rec_growth, hist_range_known, has_in_host, pad;
+    bghip_growth growth;
+    float in_W, in_lambda, in_decay[2], in_U[2], in_D[2], in_F[2];
+    int32_t in_delay[2];
+};
+
+template <typename T>
+static bool put(FILE *f, const T *p, size_t n) { return n == 0 || fwrite(p, sizeof(T), n, f) == n; }
+template <typename T>
+static bool get(FILE *f, T *p, size_t n) { return n == 0 || fread(p, sizeof(T), n, f) == n; }
+
+extern "C" int bghip_net_save(bghip_net *n, const char *path)
+{
+    if (!n || !path) return BGHIP_EINVAL;
+    net_begin(n);
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    CHK(bghip_sync(n));
+    flush_pending(n);   // a pending block's spikes go into the rings first
+    const DevNet &dn = n->dn;
+    CkptHost h;
+    memset(&h, 0, sizeof h);
+    h.step = n->step; h.last_t0 = n->last_t0; h.total = net_total(n); h.hist_step0 = n->hist_step0;
+    h.in_cyc_off = dn.in_cyc_off; h.n_burst = n->n_burst; h.n_shist = n->n_shist;
+    h.kmax = n->kmax; h.sel = n->sel; h.min_delay_uploaded = n->min_delay_uploaded; h.ds_typed = dn.ds_typed;
+    h.in_kind = dn.in_kind; h.in_dur = dn.in_dur; h.in_cycle = dn.in_cycle; h.rec_epochs = n->rec_epochs;
+    h.rec_epoch = n->rec_epoch; h.in_phase = n->in_phase; h.pending = n->pending; h.has_layout = n->has_layout;
+    h.has_growth = n->has_growth; h.rec_on = n->rec_on; h.rec_growth = n->rec_growth;
+    h.hist_range_known = n->hist_range_known; h.has_in_host = n->in_idx_host != nullptr;
+    h.growth = n->growth; h.in_W = dn.in_W; h.in_lambda = dn.in_lambda;
+    for (int k = 0; k < 2; k++) {
+        h.in_decay[k] = dn.in_decay[k]; h.in_U[k] = dn.in_U[k]; h.in_D[k] = dn.in_D[k]; h.in_F[k] = dn.in_F[k];
+        h.in_delay[k] = dn.in_delay[k];
+    }
+    std::vector<char> arena(n->arena_bytes);
+    HIP_TRY(n->ctx, hipMemcpyAsync(arena.data(), n->arena, n->arena_bytes, hipMemcpyDeviceToHost, n->stream));
+    const size_t N = dn.N, rows = (size_t)n->rec_epochs + 1;
+    std::vector<uint32_t> burst(n->rec_on ? (size_t)n->n_burst : 0), shist(n->rec_on ? (size_t)n->n_shist : 0);
+    std::vector<float> rh(n->rec_on && n->rec_growth ? rows * N : 0), qh(rh.size());
+    if (n->rec_on) {
+        CHK(d2h(n, burst.data(), n->d_burst, burst.size()));
+        CHK(d2h(n, shist.data(), n->d_shist, shist.size()));
+        CHK(d2h(n, rh.data(), n->d_rates_hist, rh.size()));
+        CHK(d2h(n, qh.data(), n->d_radii_hist, qh.size()));
+    }
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    FILE *f = fopen(path, "wb");
+    if (!f) return set_err(n->ctx, BGHIP_EINVAL, "net_save: cannot open %s", path);
+    const uint32_t sizes[3] = {(uint32_t)sizeof(bghip_net_desc), (uint32_t)sizeof(CkptHost), 0u};
+    const uint64_t ab = n->arena_bytes;
+    bool ok = put(f, kCkptMagic, 8) && put(f, &kCkptVersion, 1) && put(f, sizes, 3) && put(f, &n->desc, 1) &&
+              put(f, &h, 1) && put(f, &ab, 1) && put(f, arena.data(), arena.size());
+    if (ok && h.has_in_host) ok = put(f, n->in_idx_host, N) && put(f, n->in_mask_host, N);
+    if (ok && n->rec_on)
+        ok = put(f, burst.data(), burst.size()) && put(f, shist.data(), shist.size()) && put(f, rh.data(), rh.size()) &&
+             put(f, qh.data(), qh.size());
+    ok = (fclose(f) == 0) && ok;
+    return ok ? BGHIP_OK : set_err(n->ctx, BGHIP_EINVAL, "net_save: write to %s failed", path);
+}
+
+extern "C" int bghip_net_load(bghip_ctx *ctx, const char *path, bghip_net **out)
+{
+    if (!ctx || !path || !out) return BGHIP_EINVAL;
+    *out = nullptr;
+    FILE *f = fopen(path, "rb");
+    if (!f) return set_err(ctx, BGHIP_EINVAL, "net_load: cannot open %s", path);
+    char magic[8];
+    uint32_t version = 0, sizes[3] = {0, 0, 0};
+    bghip_net_desc desc;
+    CkptHost h;
+    uint64_t ab = 0;
+    bool ok = get(f, magic, 8) && memcmp(magic, kCkptMagic, 8) == 0 && get(f, &version, 1) &&
+              version == kCkptVersion && get(f, sizes, 3) && sizes[0] == sizeof(bghip_net_desc) &&
+              sizes[1] == sizeof(CkptHost) && get(f, &desc, 1) && get(f, &h, 1) && get(f, &ab, 1);
+    if (!ok) {
+        fclose(f);
+        return set_err(ctx, BGHIP_EINVAL, "net_load: %s is not a network checkpoint of this version", path);
+    }
+    bghip_net *n = nullptr;
+    int rc = bghip_net_create(ctx, &desc, &n);
+    if (rc) { fclose(f); return rc; }
+    const size_t N = n->dn.N;
+    std::vector<char> arena(ab);
+    ok = ab == n->arena_bytes && get(f, arena.data(), arena.size());
+    std::vector<int32_t> idx(h.has_in_host ? N : 0);
+    std::vector<uint8_t> msk(h.has_in_host ? N : 0);
+    if (ok && h.has_in_host) ok = get(f, idx.data(), N) && get(f, msk.data(), N);
+    const size_t rows = (size_t)h.rec_epochs + 1;
+    std::vector<uint32_t> burst(h.rec_on ? (size_t)h.n_burst : 0), shist(h.rec_on ? (size_t)h.n_shist : 0);
+    std::vector<float> rh(h.rec_on && h.rec_growth ? rows * N : 0), qh(rh.size());
+    if (ok && h.rec_on)
+        ok = get(f, burst.data(), burst.size()) && get(f, shist.data(), shist.size()) && get(f, rh.data(), rh.size()) &&
+             get(f, qh.data(), qh.size());
+    ok = ok && fgetc(f) == EOF;
+    fclose(f);
+    if (!ok) {
+        bghip_net_destroy(n);
+        return set_err(ctx, BGHIP_EINVAL, "net_load: %s is truncated or does not match its description", path);
+    }
+    DevNet &dn = n->dn;
+    if (hipMemcpy(n->arena, arena.data(), ab, hipMemcpyHostToDevice) != hipSuccess) {
+        bghip_net_destroy(n);
+        return set_err(ctx, BGHIP_EDEVICE, "net_load: upload failed");
+    }
+    n->step = h.step; n->last_t0 = h.last_t0; n->total = h.total; n->total_pending = false;
+    n->hist_step0 = h.hist_step0; n->kmax = h.kmax; n->sel = h.sel; n->min_delay_uploaded = h.min_delay_uploaded;
+    n->pending = h.pending; n->has_layout = h.has_layout; n->has_growth = h.has_growth; n->growth = h.growth;
+    n->hist_range_known = h.hist_range_known; n->in_phase = h.in_phase; n->out_valid = false;
+    dn.ds_typed = h.ds_typed; dn.in_kind = h.in_kind; dn.in_dur = h.in_dur; dn.in_cycle = h.in_cycle;
+    dn.in_cyc_off = h.in_cyc_off; dn.in_W = h.in_W; dn.in_lambda = h.in_lambda;
+    for (int k = 0; k < 2; k++) {
+        dn.in_decay[k] = h.in_decay[k]; dn.in_U[k] = h.in_U[k]; dn.in_D[k] = h.in_D[k]; dn.in_F[k] = h.in_F[k];
+        dn.in_delay[k] = h.in_delay[k];
+    }
+    if (h.has_in_host) {
+        n->in_idx_host = (int32_t *)malloc(sizeof(int32_t) * N);
+        n->in_mask_host = (uint8_t *)malloc(N);
+        if (!n->in_idx_host || !n->in_mask_host) {
+            bghip_net_destroy(n);
+            return set_err(ctx, BGHIP_ENOMEM, "net_load: host allocation");
+        }
+        memcpy(n->in_idx_host, idx.data(), sizeof(int32_t) * N);
+        memcpy(n->in_mask_host, msk.data(), N);
+    }
+    if (h.rec_on) {
+        rc = bghip_recorder_enable(n, h.n_burst, h.n_shist, h.rec_epochs, h.rec_growth, nullptr, nullptr);
+        if (rc) { bghip_net_destroy(n); return rc; }
+        n->rec_epoch = h.rec_epoch;
+        if (h2d(n, n->d_burst, burst.data(), burst.size()) || h2d(n, n->d_shist, shist.data(), shist.size()) ||
+            h2d(n, n->d_rates_hist, rh.data(), rh.size()) || h2d(n, n->d_radii_hist, qh.data(), qh.size()) ||
+            hipStreamSynchronize(n->stream) != hipSuccess) {
+            bghip_net_destroy(n);
+            return set_err(ctx, BGHIP_EDEVICE, "net_load: recorder upload failed");
+        }
+    }
+    *out = n;
     return BGHIP_OK;
 }
 

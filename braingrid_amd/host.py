@@ -122,7 +122,9 @@ def load():
                 ("bghost_model_save_xml", ctypes.c_int, [vp, ctypes.c_char_p]),
                 ("bghost_model_set_radii", ctypes.c_int, [vp, ctypes.c_float]),
                 ("bghost_model_array", vp, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
-                ("bghost_model_epoch", ctypes.c_int32, [vp])):
+                ("bghost_model_epoch", ctypes.c_int32, [vp]),
+                ("bghost_model_serialize", ctypes.c_int, [vp, ctypes.c_char_p]),
+                ("bghost_model_deserialize", ctypes.c_int, [vp, ctypes.c_char_p])):
             f = getattr(L, name)
             f.restype, f.argtypes = res, args
         _lib = L
@@ -132,7 +134,7 @@ def load():
 EXPORTED = ("bghost_model_create", "bghost_model_destroy", "bghost_model_net", "bghost_model_advance",
             "bghost_model_advance_epoch", "bghost_model_update_connections", "bghost_model_update_history",
             "bghost_model_run_epoch", "bghost_model_save_xml", "bghost_model_set_radii", "bghost_model_array",
-            "bghost_model_epoch")
+            "bghost_model_epoch", "bghost_model_serialize", "bghost_model_deserialize")
 
 _ARRAY_DTYPES = {"neuron_type_map": np.int32, "starter_map": np.uint8}
 
@@ -171,13 +173,7 @@ class Model:
         ctx.check(L.bghost_model_create(ctx.h, ctypes.byref(self._p), ctypes.byref(h)), "bghost_model_create")
         self.h = h
         self.N = cfg["width"] * cfg["height"]
-        self.net = B.Net.__new__(B.Net)
-        self.net.ctx, self.net.h, self.net.N = ctx, ctypes.c_void_p(L.bghost_model_net(h)), self.N
-        self.net.M = cfg["max_synapses_per_neuron"]
-        self.net.W32 = cfg["ring_bits"] // 32
-        d = B.NetDesc()
-        d.neuron_model, d.synapse_model = self._p.neuron_model, self._p.synapse_model
-        self.net.desc = d
+        self.net = B.Net.wrap(ctx, L.bghost_model_net(h))
 
     def close(self):
         if self.h:
@@ -208,6 +204,15 @@ class Model:
 
     def save_xml(self, path):
         self.ctx.check(load().bghost_model_save_xml(self.h, path.encode()), "save_xml")
+
+    def serialize(self, path: str):
+        """IModel::serialize: the model's network and recorder as a checkpoint file."""
+        self.ctx.check(load().bghost_model_serialize(self.h, os.fsencode(path)), "serialize")
+
+    def deserialize(self, path: str):
+        """IModel::deserialize: continue from a checkpoint of a model with this configuration."""
+        self.ctx.check(load().bghost_model_deserialize(self.h, os.fsencode(path)), "deserialize")
+        self.net.h = ctypes.c_void_p(load().bghost_model_net(self.h))
 
     def set_radii(self, r):
         self.ctx.check(load().bghost_model_set_radii(self.h, r), "set_radii")

@@ -91,6 +91,8 @@ class HipSpikingModel {
     }
     int updateHistory();
     int syncRecorder();
+    int serialize(const char *path);
+    int deserialize(const char *path);
     int saveData(const char *path);
     void cleanupSim()
     {
@@ -317,6 +319,36 @@ int HipSpikingModel::syncRecorder()
     return rc;
 }
 
+int HipSpikingModel::serialize(const char *path)
+{   // Model::serialize (Core/Model.cpp:85-98): the whole device network, recorder included
+    return bghip_net_save(net_, path);
+}
+
+int HipSpikingModel::deserialize(const char *path)
+{   // Model::deserialize (Core/Model.cpp:62-76), called after setupSim (Core/BGDriver.cpp:88-95):
+    // the host arrays setupSim drew stay, the device network is replaced by the checkpoint's,
+    // which must have been saved by a model of the same configuration.
+    bghip_net *nn = nullptr;
+    int rc = bghip_net_load(ctx_, path, &nn);
+    if (rc) return rc;
+    bghip_net_desc a, b;
+    bghip_net_get_desc(net_, &a);
+    bghip_net_get_desc(nn, &b);
+    int32_t e = -1;
+    bghip_recorder_info(nn, nullptr, nullptr, nullptr, nullptr, &e);
+    if (a.num_neurons != b.num_neurons || a.max_synapses_per_neuron != b.max_synapses_per_neuron ||
+        a.neuron_model != b.neuron_model || a.synapse_model != b.synapse_model || a.max_spikes != b.max_spikes ||
+        a.dt != b.dt || a.noise_seed != b.noise_seed || a.ring_bits != b.ring_bits || e < 0) {
+        bghip_net_destroy(nn);
+        return BGHIP_EINVAL;
+    }
+    bghip_net_destroy(net_);
+    net_ = nn;
+    epoch_ = e;
+    recorder_stale_ = true;
+    return syncRecorder();
+}
+
 static void put_matrix(FILE *f, const char *name, const float *v, int rows, int cols)
 {   // CompleteMatrix::toXML / Print (Matrix/CompleteMatrix.cpp:225-249): "%g" is ostream's default
     fprintf(f, "   <Matrix name=\"%s\" type=\"complete\" rows=\"%d\" columns=\"%d\" multiplier=\"1.0\">\n   ", name,
@@ -453,6 +485,14 @@ extern "C" int bghost_model_run_epoch(bghost_model *w)
 extern "C" int bghost_model_save_xml(bghost_model *w, const char *path)
 {
     return w && path ? w->m->saveData(path) : BGHIP_EINVAL;
+}
+extern "C" int bghost_model_serialize(bghost_model *w, const char *path)
+{
+    return w && path ? w->m->serialize(path) : BGHIP_EINVAL;
+}
+extern "C" int bghost_model_deserialize(bghost_model *w, const char *path)
+{
+    return w && path ? w->m->deserialize(path) : BGHIP_EINVAL;
 }
 extern "C" int bghost_model_set_radii(bghost_model *w, float r) { return w ? w->m->setRadii(r) : BGHIP_EINVAL; }
 extern "C" const void *bghost_model_array(bghost_model *w, const char *name, int64_t *count)

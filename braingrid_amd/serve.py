@@ -11,12 +11,21 @@ whose specification is invalid, or that raises a device error, fails on its own;
 cancelled while queued or running. A network's results do not depend on which other networks
 shared its launches.
 
+With eviction on, a queued job that has been passed over too often does not wait for residents to
+finish: at the end of a tick (an epoch boundary) the most recently admitted residents are saved to
+checkpoint files (bghip_net_save) and released until it fits; they rejoin the queue behind it and
+continue from their checkpoints (bghip_net_load) bit for bit when readmitted. A job is evicted at
+most `max_evictions` times.
+
 The scheduling logic talks to an engine (DeviceEngine here; tests substitute a CPU one), so the
 policy is testable without a GPU.
 """
 from __future__ import annotations
 
 import heapq
+import os
+import shutil
+import tempfile
 import threading
 import time
 from dataclasses import dataclass, field
@@ -44,6 +53,8 @@ class Job:
     skipped: int = 0                # times a later job was admitted ahead of it
     error: str | None = None
     result: dict | None = None
+    checkpoint: str | None = None   # file of an evicted job's state, until it is readmitted
+    evictions: int = 0
 
 
 def served_requests(n: int = 2000, rate: float = 20.0, seed: int = 7, side_lo: int = 30, side_hi: int = 200,
@@ -89,6 +100,12 @@ class DeviceEngine:
     def create(self, spec, plan):
         return SW.SweepNet(self.ctx, spec, plan)
 
+    def save(self, h, path: str):
+        h.save(path)
+
+    def restore(self, spec, plan, path: str):
+        return SW.SweepNet.restore(self.ctx, spec, path)
+
     def advance(self, handles, steps: int):
         key = tuple(id(h) for h in handles)
         if key != self._members:
@@ -125,6 +142,9 @@ class Server:
     epoch_steps: int = int(round(SW.EPOCH_S / SW.DT))
     lookahead: int = 16                  # queued jobs considered per admission round
     max_skips: int = 32                  # after this many bypasses the oldest job blocks the queue
+    evict: bool = False                  # make room for a blocked job by checkpointing residents
+    max_evictions: int = 1
+    spill_dir: str | None = None         # where checkpoints of evicted jobs go (a temporary folder)
     clock: object = time.perf_counter
     log: object = None
     jobs: dict = field(default_factory=dict)
@@ -134,8 +154,12 @@ class Server:
     t0: float = 0.0
     occupancy: list = field(default_factory=list)  # (clock, used bytes) per tick
     ticks: int = 0
+    blocked: Job | None = None                     # the job eviction makes room for
+    evicted: int = 0
+    restored: int = 0
+    spill_bytes: int = 0
     phase_s: dict = field(default_factory=lambda: {"admit": 0.0, "advance": 0.0, "epoch_issue": 0.0,
-                                                   "epoch_wait": 0.0, "retire": 0.0})
+                                                   "epoch_wait": 0.0, "retire": 0.0, "evict": 0.0})
 
     # --- requests -----------------------------------------------------------------------------
     def submit(self, job: Job):
@@ -150,6 +174,7 @@ class Server:
         if job.state == ACTIVE:
             self._release(job)
             self.active.remove(job)
+        self._drop_checkpoint(job)
         job.state = CANCELLED
         job.finished = self.clock()
         return True
@@ -162,7 +187,16 @@ class Server:
         self.used -= job.footprint
         job.plan = None
 
+    def _drop_checkpoint(self, job: Job):
+        if job.checkpoint is not None:
+            try:
+                os.remove(job.checkpoint)
+            except OSError:
+                pass
+            job.checkpoint = None
+
     def _fail(self, job: Job, state: str, err: str):
+        self._drop_checkpoint(job)
         job.state = state
         job.error = err
         job.finished = self.clock()
@@ -222,19 +256,81 @@ class Server:
                 # an old job that keeps being bypassed stops later ones from overtaking it
                 if job.skipped > self.max_skips:
                     head_blocked = True
+                    if self.blocked is None:
+                        self.blocked = job
                 continue
             try:
-                job.handle = self.engine.create(job.spec, job.plan)
+                if job.checkpoint is not None:
+                    job.handle = self.engine.restore(job.spec, job.plan, job.checkpoint)
+                    self._drop_checkpoint(job)
+                    self.restored += 1
+                else:
+                    job.handle = self.engine.create(job.spec, job.plan)
             except Exception as e:
                 job.plan = None
                 self._fail(job, FAILED, f"{type(e).__name__}: {e}")
                 continue
+            if self.blocked is job:
+                self.blocked = None
             self.used += job.footprint
             job.state = ACTIVE
-            job.admitted = self.clock()
+            if job.admitted is None:
+                job.admitted = self.clock()
             self.active.append(job)
         for item in kept:
             heapq.heappush(self.queue, item)
+
+    def preempt(self):
+        """Evicts the most recently admitted residents (each at most max_evictions times) until the
+        blocked job fits; evicts nothing when that cannot free enough. Runs between ticks."""
+        job = self.blocked
+        if not self.evict or job is None:
+            return
+        if job.state != QUEUED:
+            self.blocked = None
+            return
+        need = self.used + job.footprint - self.budget
+        if need <= 0:
+            return
+        victims, freed = [], 0
+        for r in reversed(self.active):
+            if freed >= need:
+                break
+            if r.evictions < self.max_evictions:
+                victims.append(r)
+                freed += r.footprint
+        if freed < need:
+            return
+        if self.spill_dir is None:
+            self.spill_dir = tempfile.mkdtemp(prefix="bg_spill_")
+        key = max(self.clock() - self.t0, job.arrival)   # behind the blocked job in the queue
+        for r in victims:
+            path = os.path.join(self.spill_dir, f"job{r.id}.ckpt")
+            try:
+                self.engine.save(r.handle, path)
+            except Exception as e:
+                self.active.remove(r)
+                self._release(r)
+                self._fail(r, FAILED, f"eviction: {type(e).__name__}: {e}")
+                continue
+            self.spill_bytes += os.path.getsize(path)
+            self.engine.release(r.handle)
+            r.handle = None
+            self.used -= r.footprint
+            self.active.remove(r)
+            r.checkpoint = path
+            r.evictions += 1
+            r.state = QUEUED
+            self.evicted += 1
+            heapq.heappush(self.queue, (key, r.id))
+            if self.log:
+                self.log(f"job {r.id} evicted for job {job.id} after {r.epochs_done} epochs")
+
+    def close(self):
+        """Removes the spill folder of evicted jobs' checkpoints."""
+        if self.spill_dir is not None:
+            shutil.rmtree(self.spill_dir, ignore_errors=True)
+            self.spill_dir = None
 
     def tick(self):
         """Advance every resident network by one epoch; while the device runs it, prepare and admit
@@ -286,9 +382,12 @@ class Server:
             job.state = DONE
             job.finished = self.clock()
         c5 = self.clock()
+        self.preempt()
+        c6 = self.clock()
         self.phase_s["epoch_issue"] += c3 - c2
         self.phase_s["epoch_wait"] += c4 - c3
         self.phase_s["retire"] += c5 - c4
+        self.phase_s["evict"] += c6 - c5
         self.ticks += 1
 
     # --- open-loop driver ---------------------------------------------------------------------
@@ -336,6 +435,7 @@ class Server:
                "latency_mean_s": float(lat.mean()) if lat.size else None,
                "hbm_resident_mean_bytes": float(occ.mean()), "hbm_resident_max_bytes": float(occ.max()),
                "ticks": self.ticks, "budget_bytes": self.budget,
+               "evicted": self.evicted, "restored": self.restored, "spill_bytes": self.spill_bytes,
                "host_phase_s": {k: round(v, 3) for k, v in self.phase_s.items()}}
         if total_bytes:
             out["hbm_occupancy_mean"] = float(occ.mean()) / total_bytes

@@ -122,6 +122,35 @@ class SweepNet:
             self.net.recorder_enable(int(spec["sim_s"]), int(spec["sim_s"] * 100), self.epochs)
         self.synapses = self.net.total_synapses
 
+    def save(self, path: str):
+        """Checkpoint of the network (and, for growth networks, its recorder) at an epoch boundary."""
+        if self.model is not None:
+            self.model.serialize(path)
+        else:
+            self.net.save(path)
+
+    @classmethod
+    def restore(cls, ctx: B.Context, spec: dict, path: str) -> "SweepNet":
+        """The network of `spec` continued from a checkpoint written by save()."""
+        x = cls.__new__(cls)
+        x.spec, x.kind, x.N = spec, spec["kind"], spec["side"] ** 2
+        x.epochs = int(round(spec["sim_s"] / EPOCH_S))
+        x.steps_per_epoch = int(round(EPOCH_S / DT))
+        x.model = None
+        if x.kind == "lif_growth":
+            x.model = host.Model(ctx, lif_config(spec))
+            try:
+                x.model.deserialize(path)
+            except Exception:
+                x.model.close()
+                raise
+            x.net = x.model.net
+        else:
+            x.net = B.Net.load(ctx, path)
+        x.max_spikes = x.net.desc.max_spikes
+        x.synapses = x.net.total_synapses
+        return x
+
     def summary(self) -> dict:
         """The job's output: XmlRecorder's spike-count histograms (1 s and 10 ms bins), and for
         growth networks the final radii; a few KB at most."""

@@ -190,6 +190,14 @@ int bghip_upload_growth(bghip_net *net, const bghip_growth *g, const float *radi
 int bghip_upload_input(bghip_net *net, const bghip_input *in, const bghip_input_state *state);
 int bghip_download_input(bghip_net *net, bghip_input_state *state);
 
+/* Checkpoint of the whole network state, device and host side, as a versioned binary file
+ * (Model::serialize / deserialize, Core/Model.cpp:62-98, whose round trip the reference breaks:
+ * Synapses/AllSynapses.cpp:136-144 vs 170-177). A loaded network continues bit for bit as the saved
+ * one would have; the noise and input streams are counter-based, so the step is their state.
+ * save blocks; load creates a new network on ctx's device. */
+int bghip_net_save(bghip_net *net, const char *path);
+int bghip_net_load(bghip_ctx *ctx, const char *path, bghip_net **out);
+
 /* ---- simulation ------------------------------------------------------------------------- */
 /* n_steps calls of GPUSpikingModel::advance (Core/GPUSpikingModel.cu:188-225), starting at the
  * network's current step; stream-ordered. */
@@ -217,6 +225,14 @@ int bghip_recorder_download(bghip_net *net, float *burstinessHist, float *spikes
 int bghip_sync(bghip_net *net);
 uint32_t bghip_net_error_bits(bghip_net *net);
 uint64_t bghip_net_step(const bghip_net *net);
+/* The description the network was created with (or loaded from a checkpoint). */
+int bghip_net_get_desc(const bghip_net *net, bghip_net_desc *out);
+/* The on-device recorder's sizes as enabled, and `epoch`: the epochs it has stored (history rows
+ * after row 0), which is Simulator::simulate's epoch count, currentStep (Core/Simulator.cpp:106-116),
+ * for a network loaded from a checkpoint. NULL pointers are skipped; when the recorder is not
+ * enabled the sizes are 0 and epoch is -1. */
+int bghip_recorder_info(const bghip_net *net, int64_t *n_burst, int64_t *n_spikes_hist, int32_t *num_epochs,
+                        int32_t *with_growth, int32_t *epoch);
 
 /* ---- downloads (blocking) --------------------------------------------------------------- */
 int bghip_download_neurons(bghip_net *net, bghip_neurons *h);

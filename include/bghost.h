@@ -53,13 +53,21 @@ int bghost_model_advance(bghost_model *m, uint64_t n_steps);
 int bghost_model_advance_epoch(bghost_model *m);
 /* IModel::updateConnections (on the GPU). */
 int bghost_model_update_connections(bghost_model *m, bghip_growth_stats *stats);
-/* IModel::updateHistory: recorder histories on the host from downloaded spikes/rates/radii
- * (Recorders/XmlRecorder.cpp:71-104, XmlGrowthRecorder.cpp:72-97) + device clamp/clear. */
+/* IModel::updateHistory: recorder histories (Recorders/XmlRecorder.cpp:71-104,
+ * XmlGrowthRecorder.cpp:72-97), minRadius clamp and spike-count clear, all on the device; the host
+ * copies returned by bghost_model_array are refreshed when read. */
 int bghost_model_update_history(bghost_model *m);
 /* One full epoch: advance_epoch + update_connections + update_history. */
 int bghost_model_run_epoch(bghost_model *m);
 /* XmlGrowthRecorder::saveSimData (Recorders/XmlGrowthRecorder.cpp:104-154), same bytes. */
 int bghost_model_save_xml(bghost_model *m, const char *path);
+/* IModel::serialize (Core/Model.cpp:85-98, written by BGDriver -w, Core/BGDriver.cpp:109-116):
+ * the device network with its recorder as a bghip_net_save checkpoint. */
+int bghost_model_serialize(bghost_model *m, const char *path);
+/* IModel::deserialize (Core/Model.cpp:62-76, BGDriver -r after setupSim, Core/BGDriver.cpp:88-95):
+ * replaces the model's network by a checkpoint saved from a model of the same configuration and
+ * resumes its epoch count; BGHIP_EINVAL for a checkpoint of another configuration. */
+int bghost_model_deserialize(bghost_model *m, const char *path);
 /* Overwrite all radii on host and device (pre-grown measurement states). */
 int bghost_model_set_radii(bghost_model *m, float r);
 

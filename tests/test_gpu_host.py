@@ -137,3 +137,51 @@ def test_rates_agree_with_reference_noise_stream(hip_ctx):
     print(f"[stats] spikes gpu {gpu_spikes} reference-noise oracle {ref_spikes}", flush=True)
     assert ref_spikes > 1000
     assert abs(gpu_spikes - ref_spikes) / ref_spikes < 0.05
+
+
+@pytest.mark.parametrize("name", ["conn-lif-ds-dynamic", "conn-izh-stdp"])
+def test_host_model_serialize_deserialize_resumes_bit_for_bit(hip_ctx, tmp_path, name):
+    """IModel::serialize after some epochs, IModel::deserialize into a freshly set-up model of the
+    same configuration (the -w / -r flags of BrainGrid's driver): the resumed model's remaining
+    epochs and XML output equal the uninterrupted model's byte for byte, and the oracle's."""
+    print(f"[host] serialize {name}", flush=True)
+    cfg = _host_cfg(name)
+    a = host.Model(hip_ctx, cfg)
+    b = None
+    try:
+        half = cfg["num_epochs"] // 2
+        for _ in range(half):
+            a.run_epoch()
+        a.advance(123)    # a checkpoint need not fall on an epoch boundary
+        path = os.path.join(tmp_path, "model.ckpt")
+        a.serialize(path)
+        b = host.Model(hip_ctx, cfg)
+        b.deserialize(path)
+        assert b.epoch == a.epoch == half and b.net.step == a.net.step
+        for m in (a, b):
+            m.advance(H.steps_per_epoch(H.load_config(name)) - 123)
+            m.update_connections()
+            m.update_history()
+            for _ in range(half + 1, cfg["num_epochs"]):
+                m.run_epoch()
+        outs = []
+        for m in (a, b):
+            p = os.path.join(tmp_path, f"out{len(outs)}.xml")
+            m.save_xml(p)
+            outs.append(open(p).read())
+        assert outs[0] == outs[1]
+        ocfg = H.load_config(name)
+        ora = O.OracleNet(ocfg)
+        for _ in range(cfg["num_epochs"]):
+            ora.run_epoch()
+        assert outs[1] == xmlout.oracle_growth_xml(ora, ocfg)
+        ora.close()
+        other = dict(cfg, max_synapses_per_neuron=cfg["max_synapses_per_neuron"] + 1)
+        c = host.Model(hip_ctx, other)
+        with pytest.raises(RuntimeError, match="deserialize"):
+            c.deserialize(path)
+        c.close()
+    finally:
+        a.close()
+        if b is not None:
+            b.close()

@@ -1,5 +1,8 @@
 """Scheduling of the served form (braingrid_amd/serve.py) on the CPU, with an engine that only
 keeps accounts: admission by footprint, rejection, failures, cancellation, fairness, statistics."""
+import json
+import os
+
 import numpy as np
 
 from braingrid_amd import serve as S
@@ -15,6 +18,7 @@ class FakeEngine:
         self.max_resident = 0
         self.released = []
         self.advanced = []
+        self.saved = []
 
     def total_bytes(self):
         return 10**9
@@ -52,6 +56,19 @@ class FakeEngine:
     def release(self, h):
         self.released.append(h["spec"]["id"])
         del self.resident[id(h)]
+
+    def save(self, h, path):
+        with open(path, "w") as f:
+            json.dump({"id": h["spec"]["id"], "epochs": h["epochs"]}, f)
+        self.saved.append(h["spec"]["id"])
+
+    def restore(self, spec, plan, path):
+        with open(path) as f:
+            st = json.load(f)
+        assert st["id"] == spec["id"]
+        h = self.create(spec, plan)
+        h["epochs"] = st["epochs"]
+        return h
 
 
 class Clock:
@@ -150,3 +167,56 @@ def test_request_stream_shape():
     assert abs(frac - 0.7) < 0.04
     d = np.array([j.spec["sim_s"] for j in jobs])
     assert d.min() == 1 and d.max() == 20 and np.all(d == np.round(d))
+
+
+def test_eviction_makes_room_for_a_blocked_job():
+    """Long residents fill the budget; a large job passed over max_skips times gets room at an epoch
+    boundary by checkpointing the newest residents, which later continue from their checkpoints."""
+    srv, eng, clock = make(budget=10_000, max_skips=2, evict=True)
+    residents = [job(i, 3, sim_s=30.0, arrival=0.001 * i) for i in range(3)]      # 9000 B for 30 epochs
+    big = job(10, 6, sim_s=2.0, arrival=0.01)
+    try:
+        srv.run(residents + [big], sleep=clock.sleep)
+        assert srv.counts() == {S.DONE: 4}
+        assert srv.evicted >= 1 and srv.restored == srv.evicted and sorted(eng.saved) == [1, 2]
+        assert big.finished < min(r.finished for r in residents[1:])
+        for r in residents:
+            assert r.result["epochs"] == 30 and r.epochs_done == 30
+        assert residents[0].evictions == 0 and residents[2].evictions == 1
+        assert eng.max_resident <= 10_000 and srv.used == 0
+        assert all(r.checkpoint is None for r in residents) and not os.listdir(srv.spill_dir)
+        st = srv.stats()
+        assert st["evicted"] == srv.evicted and st["spill_bytes"] > 0
+    finally:
+        srv.close()
+
+
+def test_no_eviction_when_it_cannot_free_enough():
+    srv, eng, clock = make(budget=10_000, max_skips=1, evict=True, max_evictions=0)
+    jobs = [job(0, 5, sim_s=6.0), job(1, 4, sim_s=6.0, arrival=0.001), job(2, 6, sim_s=1.0, arrival=0.002)]
+    try:
+        srv.run(jobs, sleep=clock.sleep)
+        assert srv.evicted == 0 and not eng.saved and srv.counts() == {S.DONE: 3}
+        assert jobs[2].admitted >= jobs[0].finished and jobs[2].skipped > 1
+    finally:
+        srv.close()
+
+
+def test_cancel_of_an_evicted_job_removes_its_checkpoint():
+    srv, eng, clock = make(budget=6_000, max_skips=0, evict=True)
+    a, b = job(0, 3, sim_s=10.0), job(1, 3, sim_s=10.0, arrival=0.001)
+    big = job(2, 5, sim_s=3.0, arrival=0.002)
+    try:
+        for j in (a, b, big):
+            srv.submit(j)
+        srv.tick()            # a and b admitted; big blocked at once (max_skips 0)
+        srv.tick()            # epoch boundary: both residents evicted to make room for big
+        assert a.state == S.QUEUED and b.state == S.QUEUED and a.checkpoint and b.checkpoint
+        path = b.checkpoint
+        assert os.path.exists(path) and srv.cancel(b.id)
+        assert not os.path.exists(path) and b.state == S.CANCELLED
+        while srv.active or srv.queue:
+            srv.tick()
+        assert big.state == S.DONE and a.state == S.DONE and a.result["epochs"] == 10
+    finally:
+        srv.close()
