@@ -146,6 +146,8 @@ def load():
                                                  _fp, _fp]),
         "bghip_recorder_download": (ctypes.c_int, [vp, _fp, _fp, _fp, _fp]),
         "bghip_net_save": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "bghip_add_synapses": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.POINTER(ctypes.c_int32), _fp]),
         "bghip_net_get_desc": (ctypes.c_int, [vp, ctypes.POINTER(NetDesc)]),
         "bghip_recorder_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
@@ -174,7 +176,8 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_download_index_map", "bghip_download_growth", "bghip_kernel_timing", "bghip_batch_create",
             "bghip_batch_destroy", "bghip_batch_set", "bghip_batch_advance", "bghip_batch_sync",
             "bghip_upload_input", "bghip_download_input", "bghip_recorder_enable", "bghip_recorder_download",
-            "bghip_net_save", "bghip_net_load", "bghip_net_get_desc", "bghip_recorder_info")
+            "bghip_net_save", "bghip_net_load", "bghip_net_get_desc", "bghip_recorder_info",
+            "bghip_add_synapses")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -238,6 +241,16 @@ class Net:
 
     def _c(self, rc, what):
         self.ctx.check(rc, what)
+
+    def add_synapses(self, src, dest, W):
+        """bghip_add_synapses: addSynapse for each src[i] -> dest[i] in order, then W[iSyn] = W[i]."""
+        s = np.ascontiguousarray(src, np.int32)
+        d = np.ascontiguousarray(dest, np.int32)
+        w = np.ascontiguousarray(W, np.float32)
+        if not (s.shape == d.shape == w.shape and s.ndim == 1):
+            raise ValueError("src, dest and W must be 1-D arrays of one length")
+        _i32 = ctypes.POINTER(ctypes.c_int32)
+        self._c(load().bghip_add_synapses(self.h, s.size, _ptr(s, _i32), _ptr(d, _i32), _ptr(w, _fp)), "add_synapses")
 
     # ---- checkpoints
     def save(self, path: str):

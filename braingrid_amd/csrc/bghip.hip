@@ -677,19 +677,12 @@ extern "C" int bghip_sync(bghip_net *n)
     return BGHIP_OK;
 }
 
-extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// createSynapse's per-type constants (Synapses/AllSpikingSynapses.cpp, AllDSSynapses.cpp,
+// AllSTDPSynapses.cpp createSynapse) for synapses created at the network's current step.
+static CreateCtx create_ctx(const bghip_net *n)
 {
-    if (!n) return BGHIP_EINVAL;
-    net_begin(n);
-    if (!n->has_layout || !n->has_growth) return set_err(n->ctx, BGHIP_EINVAL, "layout/growth state not uploaded");
-    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
-    DevNet &dn = n->dn;
-    flush_pending(n);
-    const bghip_growth &g = n->growth;
-    unsigned nb = (unsigned)((dn.N + BG_BLOCK - 1) / BG_BLOCK);
-    float c = g.epochDuration * g.rho;
-    hipLaunchKernelGGL(k_growth_radii, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, g.epochDuration, g.epsilon,
-                       g.maxRate, g.beta, c);
     CreateCtx cc;
     memset(&cc, 0, sizeof cc);
     const bghip_synapse_types &ty = n->desc.types;
@@ -703,6 +696,82 @@ extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
     cc.useFD = ty.useFroemkeDanSTDP;
     cc.syn_kind = n->desc.synapse_model;
     cc.step = n->step;
+    return cc;
+}
+
+extern "C" int bghip_add_synapses(bghip_net *n, int64_t count, const int32_t *src, const int32_t *dest,
+                                  const float *W)
+{
+    if (!n || count < 0 || (count > 0 && (!src || !dest || !W))) return BGHIP_EINVAL;
+    net_begin(n);
+    if (!n->has_layout) return set_err(n->ctx, BGHIP_EINVAL, "add_synapses: layout not uploaded");
+    const int N = n->dn.N;
+    if (count == 0) return BGHIP_OK;
+    // group by destination, keeping call order within each destination (counting sort)
+    std::vector<int64_t> per(N + 1, 0);
+    for (int64_t i = 0; i < count; i++) {
+        if (src[i] < 0 || src[i] >= N || dest[i] < 0 || dest[i] >= N)
+            return set_err(n->ctx, BGHIP_EINVAL, "add_synapses: neuron index out of range at entry %lld",
+                           (long long)i);
+        per[dest[i] + 1]++;
+    }
+    std::vector<int32_t> dests;
+    std::vector<int64_t> begin;
+    for (int d = 0; d < N; d++) {
+        if (per[d + 1]) { dests.push_back(d); begin.push_back(per[d]); }
+        per[d + 1] += per[d];
+    }
+    begin.push_back(count);
+    std::vector<int32_t> s2(count);
+    std::vector<float> w2(count);
+    std::vector<int64_t> fill(per.begin(), per.end() - 1);
+    for (int64_t i = 0; i < count; i++) {
+        const int64_t k = fill[dest[i]]++;
+        s2[k] = src[i];
+        w2[k] = W[i];
+    }
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    flush_pending(n);
+    const int nd = (int)dests.size();
+    char *buf = nullptr;
+    const size_t o_dest = 0, o_begin = align256(sizeof(int32_t) * nd),
+                 o_src = o_begin + align256(sizeof(int64_t) * (nd + 1)), o_w = o_src + align256(sizeof(int32_t) * count),
+                 bytes = o_w + sizeof(float) * count;
+    if (hipMalloc((void **)&buf, bytes) != hipSuccess) return set_err(n->ctx, BGHIP_ENOMEM, "add_synapses: staging");
+    int rc = BGHIP_OK;
+    if (hipMemcpyAsync(buf + o_dest, dests.data(), sizeof(int32_t) * nd, hipMemcpyHostToDevice, n->stream) ||
+        hipMemcpyAsync(buf + o_begin, begin.data(), sizeof(int64_t) * (nd + 1), hipMemcpyHostToDevice, n->stream) ||
+        hipMemcpyAsync(buf + o_src, s2.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, n->stream) ||
+        hipMemcpyAsync(buf + o_w, w2.data(), sizeof(float) * count, hipMemcpyHostToDevice, n->stream)) {
+        rc = set_err(n->ctx, BGHIP_EDEVICE, "add_synapses: upload");
+    } else {
+        const unsigned grid = (unsigned)(((int64_t)nd * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
+        hipLaunchKernelGGL(k_add_synapses, dim3(grid), dim3(BG_BLOCK), 0, n->stream, n->dn, create_ctx(n),
+                           (const int32_t *)(buf + o_dest), (const int64_t *)(buf + o_begin),
+                           (const int32_t *)(buf + o_src), (const float *)(buf + o_w), nd);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(n->stream) != hipSuccess)
+            rc = set_err(n->ctx, BGHIP_EDEVICE, "add_synapses: kernel");
+    }
+    hipFree(buf);
+    if (rc) return rc;
+    CHK(rebuild_incoming(n));
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
+{
+    if (!n) return BGHIP_EINVAL;
+    net_begin(n);
+    if (!n->has_layout || !n->has_growth) return set_err(n->ctx, BGHIP_EINVAL, "layout/growth state not uploaded");
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    DevNet &dn = n->dn;
+    flush_pending(n);
+    const bghip_growth &g = n->growth;
+    unsigned nb = (unsigned)((dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    float c = g.epochDuration * g.rho;
+    hipLaunchKernelGGL(k_growth_radii, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, g.epochDuration, g.epsilon,
+                       g.maxRate, g.beta, c);
+    const CreateCtx cc = create_ctx(n);
     HIP_TRY(n->ctx, hipMemsetAsync(n->d_stats, 0, 16, n->stream));
     if (dn.M > 0) {
         unsigned grid = (unsigned)std::min(dn.N, 4096);

@@ -224,6 +224,47 @@ __global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c,
     }
 }
 
+// AllSynapses::addSynapse (Synapses/AllSynapses.cpp:357-377) for a list of new synapses grouped by
+// destination (entries of destination dests[q] are [begin[q], begin[q+1]) in call order), one
+// wavefront per destination. The reference puts each new synapse into the first free slot left
+// by the earlier ones, so entry j of a destination lands in its j-th free slot: the wave walks the
+// slots 64 at a time, ballots the free ones, and lane L of a chunk takes entry
+// (entries placed so far) + (free slots below L). Entries beyond the free slots are dropped with
+// the slot-overflow error bit.
+__global__ __launch_bounds__(BG_BLOCK) void k_add_synapses(DevNet n, CreateCtx c, const int32_t *dests,
+                                                          const int64_t *begin, const int32_t *src,
+                                                          const float *w, int nd)
+{
+    const int lane = threadIdx.x & 63;
+    const int q = (int)((blockIdx.x * (uint64_t)BG_BLOCK + threadIdx.x) >> 6);
+    if (q >= nd) return;
+    const int d = dests[q];
+    const int64_t b = begin[q], cnt = begin[q + 1] - b;
+    const int M = n.M, td = n.ntype[d];
+    const uint64_t base = (uint64_t)d * M;
+    int64_t placed = 0;
+    uint32_t top = 0;   // one past the highest slot this lane filled
+    for (int k0 = 0; k0 < M && placed < cnt; k0 += 64) {
+        const int k = k0 + lane;
+        const bool free_slot = k < M && !(n.meta[base + k] & BG_META_IN_USE);
+        const uint64_t ball = __ballot(free_slot);
+        const int64_t j = placed + (int64_t)__popcll(ball & ((1ull << lane) - 1ull));
+        if (free_slot && j < cnt) {
+            const int s = src[b + j];
+            create_synapse(n, c, base + k, s, syn_type_of(n.ntype[s], td), w[b + j]);
+            top = (uint32_t)k + 1u;
+        }
+        placed += (int64_t)__popcll(ball);
+    }
+    for (int off = 32; off > 0; off >>= 1) top = max(top, (uint32_t)__shfl_xor((int)top, off));
+    if (lane == 0) {
+        const int64_t made = placed < cnt ? placed : cnt;
+        if (made < cnt) dev_error(n, 4u);
+        n.syn_count[d] += (uint32_t)made;
+        if (top > n.slot_hi[d]) n.slot_hi[d] = top;
+    }
+}
+
 // Recorder clamp + clearSpikeCounts (Recorders/XmlGrowthRecorder.cpp:89-90,
 // Neurons/AllSpikingNeurons.cpp:79-87).
 // End of epoch: XmlGrowthRecorder's rates/radii rows with the minRadius clamp

@@ -32,7 +32,10 @@ class Params(ctypes.Structure):
                [(k, _F2) for k in ("Iinject", "Inoise", "Vthresh", "Vresting", "Vreset", "Vinit", "starter_Vthresh",
                                    "starter_Vreset", "excA", "excB", "excC", "excD", "inhA", "inhB", "inhC", "inhD")] + \
                [(k, ctypes.c_float) for k in ("epsilon", "beta", "rho", "target_rate", "min_radius", "start_radius")] + \
-               [("block_steps", ctypes.c_int32)]
+               [("block_steps", ctypes.c_int32), ("static_nconns", ctypes.c_int32), ("static_thresh", ctypes.c_float),
+                ("static_exc_w", _F2), ("static_inh_w", _F2)]
+
+CONN_KINDS = {"ConnGrowth": 1, "ConnStatic": 2}
 
 
 def base_config() -> dict:
@@ -85,6 +88,13 @@ def read_config(path: str, base_dir: str | None = None) -> dict:
         for key, tag in (("epsilon", "epsilon"), ("beta", "beta"), ("rho", "rho"), ("target_rate", "targetRate"),
                          ("min_radius", "minRadius"), ("start_radius", "startRadius")):
             c[key] = float(_t(g.find(tag)))
+    sp = conn.find("StaticConnectionsParams")   # ConnStatic::readParameters (Connections/ConnStatic.cpp:120-210)
+    if sp is not None:
+        c["static_nconns"] = int(_t(sp.find("nConnsPerNeuron")))
+        c["static_thresh"] = float(_t(sp.find("threshConnsRadius")))
+        sw = conn.find("StaticConnectionsWeight")
+        c["static_exc_w"] = (float(_t(sw.find("minExc"))), float(_t(sw.find("maxExc"))))
+        c["static_inh_w"] = (float(_t(sw.find("minInh"))), float(_t(sw.find("maxInh"))))
     lay = m.find("LayoutParams")
     c["layout"] = lay.get("class")
     files = lay.find("LayoutFiles")
@@ -144,7 +154,11 @@ def make_params(c: dict):
     p.width, p.height = c["width"], c["height"]
     p.neuron_model, p.synapse_model = NEURON_CLASSES[c["neuron"]], SYN_CLASSES[c["synapse"]]
     p.layout_kind = 0 if c["layout"] == "FixedLayout" else 1
-    p.conn_kind = 1 if c["conn"] == "ConnGrowth" else 0
+    p.conn_kind = CONN_KINDS.get(c["conn"], 0)
+    if p.conn_kind == CONN_KINDS["ConnStatic"]:
+        p.static_nconns, p.static_thresh = c["static_nconns"], c["static_thresh"]
+        p.static_exc_w[0], p.static_exc_w[1] = c["static_exc_w"]
+        p.static_inh_w[0], p.static_inh_w[1] = c["static_inh_w"]
     p.max_synapses_per_neuron, p.max_firing_rate = c["max_synapses_per_neuron"], c["max_firing_rate"]
     p.num_epochs, p.ring_bits, p.dt, p.epoch_duration = c["num_epochs"], c["ring_bits"], c["dt"], c["epoch_duration"]
     p.seed = c["seed"] & 0xFFFFFFFF

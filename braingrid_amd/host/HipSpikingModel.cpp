@@ -15,8 +15,10 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <algorithm>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 #include "../../include/bghost.h"
 
@@ -113,6 +115,7 @@ class HipSpikingModel {
   private:
     void setupLayout();
     void createAllNeurons();
+    int setupStaticConnections();
     bghip_synapse_types synapseTypes() const;
 
     bghip_ctx *ctx_;
@@ -163,6 +166,62 @@ void HipSpikingModel::setupLayout()
             }
         }
     }
+}
+
+namespace {
+struct DistDest {   // the record ConnStatic sorts: ordered by distance alone (Connections/ConnStatic.h:138-146)
+    float dist;
+    int dest;
+    bool operator<(const DistDest &o) const { return dist < o.dist; }
+};
+}  // namespace
+
+int HipSpikingModel::setupStaticConnections()
+{   // ConnStatic::setupConnections (Connections/ConnStatic.cpp:33-88). For each source in ascending
+    // order: the other neurons within threshConnsRadius, std::sort by distance (so equal distances
+    // end in the standard library's order, as in the reference), the first nConnsPerNeuron of them
+    // connected, each weight drawn from the global rng right after its synapse is added. The
+    // candidate lists of different sources are independent, so they are built and sorted on worker
+    // threads; the weight draws then run in the reference's order. Distances are Layout::dist: the
+    // float square root of the float squared distance (Layouts/Layout.cpp:54-69).
+    const int N = N_, k = p_.static_nconns;
+    const float thresh = p_.static_thresh;
+    if (k < 0 || !(thresh >= 0)) return BGHIP_EINVAL;
+    std::vector<std::vector<DistDest>> chosen(N);
+    auto work = [&](int t, int nt) {
+        std::vector<DistDest> cand;
+        cand.reserve(N);
+        for (int s = t; s < N; s += nt) {
+            cand.clear();
+            for (int d = 0; d < N; d++) {
+                if (d == s) continue;
+                const float dx = xloc_[s] - xloc_[d], dy = yloc_[s] - yloc_[d];
+                const float r = sqrtf(dx * dx + dy * dy);
+                if (r <= thresh) cand.push_back(DistDest{r, d});
+            }
+            std::sort(cand.begin(), cand.end());
+            chosen[s].assign(cand.begin(), cand.begin() + std::min<size_t>(cand.size(), (size_t)k));
+        }
+    };
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work, t, nt);
+    work(0, nt);
+    for (auto &th : pool) th.join();
+    std::vector<int32_t> src, dst, indeg(N, 0);
+    std::vector<float> w;
+    for (int s = 0; s < N; s++)
+        for (const DistDest &c : chosen[s]) {
+            const int d = c.dest;
+            const bool exc_src = ntype_[s] == EXC;   // synSign(synType(s, d)) > 0: EE and EI
+            const float *r = exc_src ? p_.static_exc_w : p_.static_inh_w;
+            src.push_back(s);
+            dst.push_back(d);
+            w.push_back(rng_.inRange(r[0], r[1]));
+            // createSynapseImap asserts synapse_counts < maxSynapsesPerNeuron (Synapses/AllSynapses.cpp:231)
+            if (++indeg[d] >= p_.max_synapses_per_neuron) return BGHIP_EINVAL;
+        }
+    return bghip_add_synapses(net_, (int64_t)src.size(), src.data(), dst.data(), w.data());
 }
 
 void HipSpikingModel::createAllNeurons()
@@ -291,6 +350,8 @@ int HipSpikingModel::setupSim()
     if ((rc = bghip_upload_neurons(net_, &h))) return rc;
     if ((rc = bghip_upload_layout(net_, xloc_.data(), yloc_.data(), ntype_.data()))) return rc;
     if ((rc = bghip_upload_growth(net_, &growth_, radii_.data(), rates_.data()))) return rc;
+    // m_conns->setupConnections (Core/Model.cpp:154): ConnGrowth starts without synapses
+    if (p_.conn_kind == BGHOST_CONN_STATIC && (rc = setupStaticConnections())) return rc;
     // the recorder's histories live on the device (XmlRecorder::initDefaultValues and
     // XmlGrowthRecorder::initDefaultValues give row 0 and zero bins)
     const bool growth = p_.conn_kind == BGHOST_CONN_GROWTH;

@@ -190,6 +190,15 @@ int bghip_upload_growth(bghip_net *net, const bghip_growth *g, const float *radi
 int bghip_upload_input(bghip_net *net, const bghip_input *in, const bghip_input_state *state);
 int bghip_download_input(bghip_net *net, bghip_input_state *state);
 
+/* AllSynapses::addSynapse (Synapses/AllSynapses.cpp:357-377) for `count` new synapses
+ * src[i] -> dest[i], each followed by W[iSyn] = W[i] as ConnStatic::setupConnections does
+ * (Connections/ConnStatic.cpp:66-80): every synapse goes to the first slot of its destination left
+ * free by the earlier entries, with createSynapse's per-type constants and the synapse type from
+ * the uploaded layout; the incoming index map is rebuilt (createSynapseImap). Entries past a full
+ * destination are dropped and set BGHIP_ERR_SLOT_OVERFLOW. Needs bghip_upload_layout first;
+ * blocks. */
+int bghip_add_synapses(bghip_net *net, int64_t count, const int32_t *src, const int32_t *dest, const float *W);
+
 /* Checkpoint of the whole network state, device and host side, as a versioned binary file
  * (Model::serialize / deserialize, Core/Model.cpp:62-98, whose round trip the reference breaks:
  * Synapses/AllSynapses.cpp:136-144 vs 170-177). A loaded network continues bit for bit as the saved

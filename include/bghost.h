@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 enum { BGHOST_LAYOUT_FIXED = 0, BGHOST_LAYOUT_DYNAMIC = 1 };
-enum { BGHOST_CONN_NONE = 0, BGHOST_CONN_GROWTH = 1 };
+enum { BGHOST_CONN_NONE = 0, BGHOST_CONN_GROWTH = 1, BGHOST_CONN_STATIC = 2 };
 
 /* The simulation parameters of a BrainGrid XML config (Core/SimulationInfo.h:89-155, neuron
  * ranges of Neurons/AllIFNeurons.h and AllIZHNeurons.h, ConnGrowth's m_growth, layout lists). */
@@ -38,6 +38,10 @@ typedef struct bghost_params {
     float excA[2], excB[2], excC[2], excD[2], inhA[2], inhB[2], inhC[2], inhD[2];
     float epsilon, beta, rho, target_rate, min_radius, start_radius;
     int32_t block_steps;                     /* 0 = automatic */
+    /* ConnStatic (Connections/ConnStatic.cpp:120-210): nConnsPerNeuron, threshConnsRadius and the
+     * weight ranges [minExc, maxExc], [minInh, maxInh] */
+    int32_t static_nconns;
+    float static_thresh, static_exc_w[2], static_inh_w[2];
 } bghost_params;
 
 typedef struct bghost_model bghost_model;

@@ -283,6 +283,8 @@ void bgo_destroy(bgo_net *n)
     free(n);
 }
 
+static int conn_static_setup(bgo_net *n);
+
 bgo_net *bgo_create(const bgo_params *p, char *err, int errlen)
 {
     bgo_net *n = calloc(1, sizeof *n);
@@ -350,6 +352,12 @@ bgo_net *bgo_create(const bgo_params *p, char *err, int errlen)
         n->rates[i] = 0.0f;
         n->radiiHistory[i] = p->start_radius;   /* XmlGrowthRecorder::initDefaultValues (:27-37) */
         n->ratesHistory[i] = 0.0f;
+    }
+    /* m_conns->setupConnections, then createSynapseImap (Core/Model.cpp:154-157) */
+    if (p->conn_kind == BGO_CONN_STATIC && (conn_static_setup(n) < 0 || bgo_rebuild_index_map(n) < 0)) {
+        if (err) snprintf(err, errlen, "%s", n->err);
+        bgo_destroy(n);
+        return NULL;
     }
     norm_seed(&n->norm, p->seed);
     input_init(n);   /* Simulator::setup (Core/Simulator.cpp:40-43) */
@@ -450,6 +458,46 @@ int64_t bgo_add_synapse(bgo_net *n, int32_t src, int32_t dest, float w)
     int64_t i = add_synapse(n, src, dest, syn_type(n, src, dest));
     if (i >= 0) n->W[i] = w;
     return i;
+}
+
+/* ConnStatic::setupConnections (Connections/ConnStatic.cpp:33-88): for every source in ascending
+ * order, the other neurons within threshConnsRadius sorted by distance (std::sort, equal distances
+ * in the standard library's order), the first nConnsPerNeuron of them connected through addSynapse
+ * and each weight drawn from the global rng right after its synapse: excitatory types from
+ * [minExc, maxExc], inhibitory ones from [minInh, maxInh]. Distances are Layout::dist, the float
+ * square root of the float squared distance (Layouts/Layout.cpp:54-69). A destination without a
+ * free slot makes the reference write the weight through an unset index; here it fails, and a
+ * destination that reaches maxSynapsesPerNeuron fails in createSynapseImap as the reference's
+ * assert does (Synapses/AllSynapses.cpp:231). */
+static int conn_static_setup(bgo_net *n)
+{
+    const int N = n->N;
+    const bgo_params *p = &n->p;
+    float *dist = malloc(sizeof(float) * (size_t)N);
+    int32_t *dest = malloc(sizeof(int32_t) * (size_t)N);
+    if (!dist || !dest) { free(dist); free(dest); return fail(n, "out of memory"); }
+    int rc = 0;
+    for (int s = 0; s < N && rc == 0; s++) {
+        int c = 0;
+        for (int d = 0; d < N; d++) {
+            if (d == s) continue;
+            float dx = n->xloc[s] - n->xloc[d], dy = n->yloc[s] - n->yloc[d];
+            float d2 = dx * dx + dy * dy;
+            float r = sqrtf(d2);
+            if (r <= p->static_thresh) { dist[c] = r; dest[c] = d; c++; }
+        }
+        bgo_sort_by_dist(dist, dest, c);
+        for (int i = 0; i < c && i < p->static_nconns; i++) {
+            int d = dest[i], t = syn_type(n, s, d);
+            int64_t slot = add_synapse(n, s, d, t);
+            if (slot < 0) { rc = -1; break; }
+            n->W[slot] = syn_sign(t) > 0 ? mt_inrange(&n->rng, p->static_exc_w[0], p->static_exc_w[1])
+                                         : mt_inrange(&n->rng, p->static_inh_w[0], p->static_inh_w[1]);
+        }
+    }
+    free(dist);
+    free(dest);
+    return rc;
 }
 
 /* AllSynapses::createSynapseImap (Synapses/AllSynapses.cpp:223-293). */

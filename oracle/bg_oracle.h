@@ -31,7 +31,7 @@ extern "C" {
 enum { BGO_LIF = 0, BGO_IZH = 1 };
 enum { BGO_SPIKING = 0, BGO_DS = 1, BGO_STDP = 2, BGO_DYNSTDP = 3 };
 enum { BGO_LAYOUT_FIXED = 0, BGO_LAYOUT_DYNAMIC = 1 };
-enum { BGO_CONN_NONE = 0, BGO_CONN_GROWTH = 1 };
+enum { BGO_CONN_NONE = 0, BGO_CONN_GROWTH = 1, BGO_CONN_STATIC = 2 };
 
 typedef struct bgo_params {
     int32_t width, height;
@@ -68,6 +68,9 @@ typedef struct bgo_params {
     float input_duration, input_interval; /* SInputRegular IntParams (seconds) */
     int32_t input_sync;                   /* SInputRegular sync: 0 "yes", 1 "no", 2 "wave" */
     float input_values[100];              /* SInputRegular <Values><I>: the 10x10 table */
+    /* ConnStatic (Connections/ConnStatic.cpp:120-210): nConnsPerNeuron, threshConnsRadius, weight ranges */
+    int32_t static_nconns;
+    float static_thresh, static_exc_w[2], static_inh_w[2];
 } bgo_params;
 
 enum { BGO_INPUT_NONE = 0, BGO_INPUT_POISSON = 1, BGO_INPUT_REGULAR = 2 };
@@ -108,6 +111,11 @@ void bgo_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float bgo_noise(uint64_t seed, uint32_t neuron, uint64_t step);
 float bgo_norm_stream(uint32_t seed, int n, float *out);   /* first n Norm draws */
 double bgo_detmath(int fn, double x, double y);            /* 0 exp 1 log 2 sin 3 cos 4 acos 5 pow 6 cos2pi 7 expf */
+/* ConnStatic's candidate order: std::sort of (dist, dest) pairs compared by dist alone
+ * (Connections/ConnStatic.cpp:62, ConnStatic.h:138-146). The tie order is the standard library's,
+ * so this one function is C++ (oracle/bg_oracle_sort.cpp), compiled against the same libstdc++ as
+ * the reference build. */
+void bgo_sort_by_dist(float *dist, int32_t *dest, int32_t n);
 /* Poisson input inter-spike interval in steps drawn from the counter-based stream. */
 int32_t bgo_poisson_isi(uint64_t seed, uint32_t neuron, uint64_t step, float lambda_ms, float dt);
 

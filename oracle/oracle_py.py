@@ -21,7 +21,7 @@ REF_DIR = os.path.join(HERE, "_ref")
 LIF, IZH = 0, 1
 SPIKING, DS, STDP, DYNSTDP = 0, 1, 2, 3
 LAYOUT_FIXED, LAYOUT_DYNAMIC = 0, 1
-CONN_NONE, CONN_GROWTH = 0, 1
+CONN_NONE, CONN_GROWTH, CONN_STATIC = 0, 1, 2
 
 SYN_CLASSES = {"AllSpikingSynapses": SPIKING, "AllDSSynapses": DS, "AllSTDPSynapses": STDP,
                "AllDynamicSTDPSynapses": DYNSTDP}
@@ -60,6 +60,8 @@ class Params(ctypes.Structure):
         ("input_mask", ctypes.POINTER(ctypes.c_int32)), ("n_input_mask", ctypes.c_int32),
         ("input_duration", ctypes.c_float), ("input_interval", ctypes.c_float), ("input_sync", ctypes.c_int32),
         ("input_values", ctypes.c_float * 100),
+        ("static_nconns", ctypes.c_int32), ("static_thresh", ctypes.c_float),
+        ("static_exc_w", _F2), ("static_inh_w", _F2),
     ]
 
 
@@ -172,6 +174,13 @@ def parse_config_xml(path: str, base_dir: str) -> dict:
         for key, tag in (("epsilon", "epsilon"), ("beta", "beta"), ("rho", "rho"), ("target_rate", "targetRate"),
                          ("min_radius", "minRadius"), ("start_radius", "startRadius")):
             cfg[key] = float(_text(g.find(tag)))
+    sp = conn.find("StaticConnectionsParams")   # Connections/ConnStatic.cpp:120-210
+    if sp is not None:
+        cfg["static_nconns"] = int(_text(sp.find("nConnsPerNeuron")))
+        cfg["static_thresh"] = float(_text(sp.find("threshConnsRadius")))
+        sw = conn.find("StaticConnectionsWeight")
+        cfg["static_exc_w"] = (float(_text(sw.find("minExc"))), float(_text(sw.find("maxExc"))))
+        cfg["static_inh_w"] = (float(_text(sw.find("minInh"))), float(_text(sw.find("maxInh"))))
     lay = model.find("LayoutParams")
     cfg["layout"] = lay.get("class")
     files = lay.find("LayoutFiles")
@@ -220,7 +229,11 @@ def make_params(cfg: dict) -> tuple[Params, list]:
     p.neuron_kind = NEURON_CLASSES[cfg["neuron"]]
     p.syn_kind = SYN_CLASSES[cfg["synapse"]]
     p.layout_kind = LAYOUT_FIXED if cfg["layout"] == "FixedLayout" else LAYOUT_DYNAMIC
-    p.conn_kind = CONN_GROWTH if cfg["conn"] == "ConnGrowth" else CONN_NONE
+    p.conn_kind = {"ConnGrowth": CONN_GROWTH, "ConnStatic": CONN_STATIC}.get(cfg["conn"], CONN_NONE)
+    if p.conn_kind == CONN_STATIC:
+        p.static_nconns, p.static_thresh = cfg["static_nconns"], cfg["static_thresh"]
+        p.static_exc_w[0], p.static_exc_w[1] = cfg["static_exc_w"]
+        p.static_inh_w[0], p.static_inh_w[1] = cfg["static_inh_w"]
     p.max_synapses_per_neuron = cfg["max_synapses_per_neuron"]
     p.max_firing_rate = cfg["max_firing_rate"]
     p.num_epochs = cfg["num_epochs"]

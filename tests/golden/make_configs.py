@@ -60,7 +60,41 @@ def dynamic_layout(fexc, fst):
 """
 
 
-def config(name, x, y, tsim, nsims, maxsyn, neurons, synapse, layout, start_radius, seed=1, max_rate=200):
+def growth_conn(start_radius):
+    return f"""    <ConnectionsParams class="ConnGrowth" name="ConnectionsParams">
+      <GrowthParams name="GrowthParams">
+        <epsilon name="epsilon">0.60</epsilon>
+        <beta name="beta">0.10</beta>
+        <rho name="rho">0.0001</rho>
+        <targetRate name="targetRate">1.9</targetRate>
+        <minRadius name="minRadius">0.1</minRadius>
+        <startRadius name="startRadius">{start_radius}</startRadius>
+      </GrowthParams>
+    </ConnectionsParams>
+"""
+
+
+def static_conn(nconns, thresh, exc=(0, 0.5e-7), inh=(-0.5e-7, 0)):
+    """ConnStatic (the block of configfiles/static_izh_1000.xml)."""
+    return f"""    <ConnectionsParams class="ConnStatic" name="ConnectionsParams">
+      <StaticConnectionsParams name="StaticConnectionsParams">
+        <nConnsPerNeuron name="nConnsPerNeuron">{nconns}</nConnsPerNeuron>
+        <threshConnsRadius name="threshConnsRadius">{thresh}</threshConnsRadius>
+        <pRewiring name="pRewiring">0</pRewiring>
+      </StaticConnectionsParams>
+      <StaticConnectionsWeight name="StaticConnectionsWeight">
+        <minExc name="minExc">{exc[0]}</minExc>
+        <maxExc name="maxExc">{exc[1]}</maxExc>
+        <minInh name="minInh">{inh[0]}</minInh>
+        <maxInh name="maxInh">{inh[1]}</maxInh>
+      </StaticConnectionsWeight>
+    </ConnectionsParams>
+"""
+
+
+def config(name, x, y, tsim, nsims, maxsyn, neurons, synapse, layout, start_radius=None, seed=1, max_rate=200,
+           conn=None):
+    conn = conn if conn is not None else growth_conn(start_radius)
     return f"""<?xml version="1.0" encoding="UTF-8" standalone="no"?>
 <BGSimParams>
   <SimInfoParams name="SimInfoParams">
@@ -73,17 +107,7 @@ def config(name, x, y, tsim, nsims, maxsyn, neurons, synapse, layout, start_radi
   <ModelParams>
 {neurons}    <SynapsesParams class="{synapse}" name="SynapsesParams">
     </SynapsesParams>
-    <ConnectionsParams class="ConnGrowth" name="ConnectionsParams">
-      <GrowthParams name="GrowthParams">
-        <epsilon name="epsilon">0.60</epsilon>
-        <beta name="beta">0.10</beta>
-        <rho name="rho">0.0001</rho>
-        <targetRate name="targetRate">1.9</targetRate>
-        <minRadius name="minRadius">0.1</minRadius>
-        <startRadius name="startRadius">{start_radius}</startRadius>
-      </GrowthParams>
-    </ConnectionsParams>
-{layout}  </ModelParams>
+{conn}{layout}  </ModelParams>
 </BGSimParams>
 """
 
@@ -119,6 +143,23 @@ CONFIGS = {
     "conn-lif-ds-dynamic": dict(x=8, y=8, tsim=0.5, nsims=3, maxsyn=40, neurons=LIF, synapse="AllDSSynapses",
                                 layout=fixed_layout("", ""), start_radius=1.2, seed=7),
 }
+
+# ConnStatic nearest-k (Connections/ConnStatic.cpp:33-88) on the 10x10 grid: interior neurons have
+# 4 neighbours at distance 1, 4 at sqrt 2, 4 at 2 and 8 at sqrt 5, so k = 10 and k = 14 cut through
+# a group of equal distances and the standard library's tie order decides which neighbours connect
+# and which weight draw each one gets; the all-to-all case is configfiles/static_izh_1000.xml's
+# (every other neuron within the radius, k = N - 1) on 8x8.
+STATIC_CONFIGS = {
+    "static-izh-spiking": dict(x=10, y=10, tsim=0.5, nsims=2, maxsyn=40, neurons=IZH, synapse="AllSpikingSynapses",
+                               layout=fixed_layout("NList/A10x10.xml", "NList/I10x10.xml"), max_rate=1000,
+                               conn=static_conn(10, 2.5)),
+    "static-lif-ds": dict(x=10, y=10, tsim=0.5, nsims=2, maxsyn=40, neurons=LIF, synapse="AllDSSynapses",
+                          layout=fixed_layout("NList/A10x10.xml", "NList/I10x10.xml"), seed=3,
+                          conn=static_conn(14, 3.0, exc=(0, 2.0e-7), inh=(-1.0e-7, 0))),
+    "static-izh-all": dict(x=8, y=8, tsim=0.5, nsims=2, maxsyn=64, neurons=IZH, synapse="AllSpikingSynapses",
+                           layout=fixed_layout("", ""), max_rate=1000, seed=777, conn=static_conn(63, 50)),
+}
+CONFIGS.update(STATIC_CONFIGS)
 
 
 # Stimulus inputs (Inputs/SInputPoisson.cpp, Inputs/SInputRegular.cpp). The reference's CPU Poisson

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 from oracle import oracle_py as O  # noqa: E402
-from make_configs import INPUT_CONFIGS  # noqa: E402
+from make_configs import INPUT_CONFIGS, STATIC_CONFIGS  # noqa: E402
 
 CONFIGS = {
     # name: (config path relative to base dir, base dir, extra harness args)
@@ -31,6 +31,8 @@ for _n in ("conn-lif-ds", "conn-lif-ds-dense", "conn-lif-spiking", "conn-lif-std
            "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"):
     CONFIGS[_n] = (_n + ".xml", os.path.join(HERE, "configs"),
                    ["--dynamic-layout", "0.9", "0.1"] if "dynamic" in _n else [])
+for _n in STATIC_CONFIGS:
+    CONFIGS[_n] = (_n + ".xml", os.path.join(HERE, "configs"), [])
 for _n, (_kw, _stim) in INPUT_CONFIGS.items():
     CONFIGS[_n] = (_n + ".xml", os.path.join(HERE, "configs"), ["--stim", _stim])
 

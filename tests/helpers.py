@@ -19,6 +19,8 @@ INPUT_STIM = {"input-lif-poisson": "stim/poisson-50hz-w25-mask.xml", "input-izh-
 
 GROWTH_CONFIGS = ["conn-lif-ds", "conn-lif-ds-dense", "conn-lif-spiking", "conn-lif-stdp", "conn-lif-dynstdp",
                   "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"]
+# ConnStatic nearest-k configs (tests/golden/make_configs.py)
+STATIC_CONFIGS = ["static-izh-spiking", "static-lif-ds", "static-izh-all"]
 
 NEURON_UPLOAD = ("Vm", "Vthresh", "Vreset", "I0", "Inoise", "C1", "C2", "Trefract", "summation_map", "nStepsInRefr",
                  "spikeCount", "spikeCountOffset", "spike_history")
@@ -170,7 +172,8 @@ def izh_static_oracle(neurons: dict, syn: dict, meta: dict, seed: int, epoch_s: 
     from braingrid_amd import networks as NW
     N = meta["N"]
     cfg = O.default_config()
-    cfg.update(width=N, height=1, neuron="AllIZHNeurons", synapse="AllDynamicSTDPSynapses", conn="ConnStatic",
+    # connections come from the builder's list, so the oracle's own generators stay off
+    cfg.update(width=N, height=1, neuron="AllIZHNeurons", synapse="AllDynamicSTDPSynapses", conn="none",
                layout="FixedLayout", inh_list=[int(i) for i in np.flatnonzero(meta["ntype"] == 1)], active_list=[],
                max_synapses_per_neuron=meta["M"], max_firing_rate=max_rate, num_epochs=1, epoch_duration=epoch_s,
                seed=seed, ring_bits=meta["ring_bits"], stdp=NW.STDP_CFG, noise_mode=1, math_mode=1, sum_mode=1,

@@ -185,3 +185,34 @@ def test_host_model_serialize_deserialize_resumes_bit_for_bit(hip_ctx, tmp_path,
         a.close()
         if b is not None:
             b.close()
+
+
+@pytest.mark.parametrize("name", H.STATIC_CONFIGS)
+def test_host_static_connections_match_oracle(hip_ctx, name):
+    """The host model's ConnStatic (candidate lists sorted on worker threads, weights drawn in the
+    reference's order, synapses created on the device by bghip_add_synapses) builds exactly the
+    oracle's synapses, pinned to the reference's own tie order; the run and its XmlRecorder output
+    then match the oracle."""
+    print(f"[host] static {name}", flush=True)
+    cfg = _host_cfg(name)
+    ocfg = H.load_config(name)
+    m = host.Model(hip_ctx, cfg)
+    ora = O.OracleNet(ocfg)
+    try:
+        names = SETUP_ARRAYS + (IZH_ARRAYS if "izh" in name else ())
+        bad = [k for k in names if not np.array_equal(m.array(k).view(np.uint8), ora.array(k).view(np.uint8))]
+        bad += H.compare_state(ora, m.net, ocfg, "setup")
+        assert not bad, bad
+        assert m.net.total_synapses == ora.total_synapses > 0
+        for ep in range(cfg["num_epochs"]):
+            m.run_epoch()
+            ora.run_epoch()
+            bad = H.compare_state(ora, m.net, ocfg, f"epoch{ep}")
+            assert not bad, bad
+        assert ora.array("spikeCount").sum() + ora.array("spikeCountOffset").sum() > 0
+        p = os.path.join(tempfile.mkdtemp(), "out.xml")
+        m.save_xml(p)
+        assert open(p).read() == xmlout.oracle_growth_xml(ora, ocfg)
+    finally:
+        m.close()
+        ora.close()

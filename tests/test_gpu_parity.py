@@ -30,7 +30,8 @@ def _run(hip_ctx, name, block_steps=0, epochs=None):
         spikes += int(ora.array("spikeCount").sum())
         bad += H.compare_state(ora, net, cfg, f"ep{ep}-advanced")
         ora.update_connections()
-        net.update_connections()
+        if cfg["conn"] == "ConnGrowth":   # Model::updateConnections changes nothing for ConnStatic
+            net.update_connections()
         bad += H.compare_state(ora, net, cfg, f"ep{ep}-grown")
         ora.update_history()
         net.update_history()
@@ -49,6 +50,16 @@ def test_growth_configs_bit_exact(hip_ctx, name):
     assert not bad, bad[:10]
     assert spikes > 0
     assert total > 0
+
+
+@pytest.mark.parametrize("name", H.STATIC_CONFIGS)
+def test_static_configs_bit_exact(hip_ctx, name):
+    """ConnStatic networks (nearest-k, Spiking and DS synapses, up to all-to-all): the oracle's
+    synapses uploaded, every epoch bit for bit."""
+    print(f"[parity] {name}", flush=True)
+    bad, spikes, total = _run(hip_ctx, name)
+    assert not bad, bad[:10]
+    assert spikes > 0 and total > 0
 
 
 @pytest.mark.parametrize("block_steps", [1, 2, 5])

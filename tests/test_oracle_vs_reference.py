@@ -36,7 +36,8 @@ CFG_BASE = os.path.join(GOLDEN, "configs")
 
 CONFIGS = {"test-tiny": (os.path.join(REF_BASE, "configfiles/test-tiny.xml"), REF_BASE)}
 for _n in ("conn-lif-ds", "conn-lif-ds-dense", "conn-lif-spiking", "conn-lif-stdp", "conn-lif-dynstdp",
-           "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic") + tuple(INPUT_STIM):
+           "conn-izh-ds", "conn-izh-stdp", "conn-lif-ds-dynamic", "static-izh-spiking", "static-lif-ds",
+           "static-izh-all") + tuple(INPUT_STIM):
     CONFIGS[_n] = (os.path.join(CFG_BASE, _n + ".xml"), CFG_BASE)
 
 
@@ -131,7 +132,7 @@ def test_oracle_output_equals_2016_golden_xml(name):
 
 @pytest.mark.skipif(not O.ref_available() or not os.path.isdir("/root/reference"),
                     reason="compiled reference not present (built only where the reference sources are)")
-@pytest.mark.parametrize("name", ["conn-lif-ds", "conn-izh-stdp", "conn-lif-ds-dynamic"])
+@pytest.mark.parametrize("name", ["conn-lif-ds", "conn-izh-stdp", "conn-lif-ds-dynamic", "static-lif-ds"])
 def test_oracle_matches_live_reference(name):
     path, base = CONFIGS[name]
     with tempfile.TemporaryDirectory() as td:
@@ -167,3 +168,28 @@ def test_fixture_comparison_is_not_vacuous(name):
         assert arrs["in_use"].any()
     for k, f in plants.items():
         assert _compare(net, {**arrs, k: f(arrs[k])}, "plant") == [f"plant:{k}"], k
+
+
+@pytest.mark.parametrize("name,k,thresh", [("static-izh-spiking", 10, 2.5), ("static-lif-ds", 14, 3.0)])
+def test_static_fixtures_pin_the_tie_order(name, k, thresh):
+    """The ConnStatic fixtures cut through groups of equal distance, so their synapse sets depend on
+    the order std::sort leaves equal distances in: a selection that breaks ties by neuron index
+    (a stable sort) connects a different set, while the distances each source reaches are the same."""
+    recs = _records(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    setup = [a for tag, _, a in recs if tag == "setup"][0]
+    use = setup["in_use"].astype(bool)
+    cfg = _cfg(name)
+    W, N = cfg["width"], cfg["width"] * cfg["height"]
+    M = use.size // N
+    ref = set(zip(setup["sourceNeuronIndex"][use].tolist(), np.repeat(np.arange(N), M)[use].tolist()))
+    x = (np.arange(N) % W).astype(np.float32)
+    y = (np.arange(N) // W).astype(np.float32)
+    stable = set()
+    for s in range(N):
+        dx, dy = x[s] - x, y[s] - y
+        d = np.sqrt(dx * dx + dy * dy, dtype=np.float32)
+        cand = [j for j in np.argsort(d, kind="stable") if j != s and d[j] <= thresh][:k]
+        stable |= {(s, int(j)) for j in cand}
+        reached = sorted(d[j] for (a, j) in ref if a == s)
+        assert reached == sorted(d[cand].tolist())
+    assert len(ref) == len(stable) and len(ref ^ stable) > 50

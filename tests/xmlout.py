@@ -1,6 +1,7 @@
 """Writes recorder output in the byte format of XmlGrowthRecorder::saveSimData
-(Recorders/XmlGrowthRecorder.cpp:104-154, Matrix/CompleteMatrix.cpp:225-249,
-Matrix/VectorMatrix.cpp:173-190): ostream's default float formatting is C's "%g"."""
+(Recorders/XmlGrowthRecorder.cpp:104-154) and, without the radii/rates histories, of
+XmlRecorder::saveSimData (Recorders/XmlRecorder.cpp:111-158), with Matrix/CompleteMatrix.cpp:225-249
+and Matrix/VectorMatrix.cpp:173-190: ostream's default float formatting is C's "%g"."""
 import numpy as np
 
 
@@ -22,7 +23,8 @@ def vector(name, v):
 
 def growth_xml(radii_hist, rates_hist, burst, spikes_hist, xloc, yloc, ntype, starter_map, vthresh, tsim, end_time):
     out = '<?xml version="1.0" standalone="no"?>\n<!-- State output file for the DCT growth modeling-->\n<SimState>\n'
-    out += matrix("radiiHistory", radii_hist) + matrix("ratesHistory", rates_hist)
+    if radii_hist is not None:
+        out += matrix("radiiHistory", radii_hist) + matrix("ratesHistory", rates_hist)
     out += vector("burstinessHist", burst) + vector("spikesHistory", spikes_hist)
     out += vector("xloc", xloc) + vector("yloc", yloc) + vector("neuronTypes", ntype)
     st = np.nonzero(starter_map)[0]
@@ -40,7 +42,9 @@ def oracle_growth_xml(net, cfg) -> str:
     E = cfg["num_epochs"]
     f32 = np.float32
     end = f32(f32(net.step) * f32(cfg["dt"]))
-    return growth_xml(net.array("radiiHistory").reshape(E + 1, N), net.array("ratesHistory").reshape(E + 1, N),
+    growth = cfg["conn"] == "ConnGrowth"
+    return growth_xml(net.array("radiiHistory").reshape(E + 1, N) if growth else None,
+                      net.array("ratesHistory").reshape(E + 1, N) if growth else None,
                       net.array("burstinessHist"), net.array("spikesHistory"), net.array("xloc"),
                       net.array("yloc"), net.array("neuron_type_map"), net.array("starter_map"),
                       net.array("Vthresh"), f32(cfg["epoch_duration"]), end)
