@@ -157,17 +157,20 @@ struct Carve {
 // Segment offsets of the sharded event lists for destinations (work items) [base_i, base_i + N_i)
 // with M_i slots each: shard s holds the work items w = s (mod BG_EV_SHARDS). Returns false when
 // the synapse list would not be addressable with 32-bit offsets.
-struct EvPart { uint64_t base, N, M; };
+// Segment sizes of the sharded event lists. The streaming pass works on runs ("chunks") of
+// BG_STREAM_DESTS destinations and lists a chunk's events in shard (chunk index mod BG_EV_SHARDS);
+// a part is one network whose chunks are numbered from cbase. Each shard gets room for the worst
+// case: every destination of its chunks listed, with all M of its slots.
+struct EvPart { uint64_t cbase, N, M; };
 static bool ev_offsets(const std::vector<EvPart> &parts, std::vector<uint32_t> &dest_off, std::vector<uint32_t> &syn_off)
 {
-    const uint64_t NS = BG_EV_SHARDS;
+    const uint64_t NS = BG_EV_SHARDS, D = BG_STREAM_DESTS;
     std::vector<uint64_t> dc(NS, 0), sc(NS, 0);
-    auto below = [&](uint64_t x, uint64_t r) { return (x + NS - 1 - r) / NS; };   // #{w < x : w = r mod NS}
     for (const EvPart &p : parts)
-        for (uint64_t r = 0; r < NS; r++) {
-            const uint64_t k = below(p.base + p.N, r) - below(p.base, r);
-            dc[r] += k;
-            sc[r] += k * p.M;
+        for (uint64_t k = 0; k * D < p.N; k++) {
+            const uint64_t r = (p.cbase + k) % NS, nd = std::min<uint64_t>(D, p.N - k * D);
+            dc[r] += nd;
+            sc[r] += nd * p.M;
         }
     dest_off.assign(NS + 1, 0);
     syn_off.assign(NS + 1, 0);
@@ -752,7 +755,7 @@ extern "C" int bghip_add_synapses(bghip_net *n, int64_t count, const int32_t *sr
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(n->stream) != hipSuccess)
             rc = set_err(n->ctx, BGHIP_EDEVICE, "add_synapses: kernel");
     }
-    hipFree(buf);
+    (void)hipFree(buf);
     if (rc) return rc;
     CHK(rebuild_incoming(n));
     return BGHIP_OK;
@@ -1302,6 +1305,9 @@ struct BatchGroup {
     std::vector<int> members;      // indices into bghip_batch::nets
     BatchNet *d_nets = nullptr;    // device table (nn entries)
     uint32_t *d_base = nullptr;    // neuron prefix (nn + 1)
+    uint16_t *d_nof = nullptr;     // member of each neuron / destination (total)
+    uint2 *d_chunks = nullptr;     // per-step STDP: runs of BG_STREAM_DESTS destinations
+    uint32_t nchunks = 0;
     uint64_t *d_sbase = nullptr;   // slot prefix (nn + 1)
     uint64_t *d_tprev = nullptr;   // flush parameters (nn)
     int *d_sel = nullptr;
@@ -1333,6 +1339,8 @@ static void free_groups(bghip_batch *b)
     for (auto &g : b->groups) {
         (void)hipFree(g.d_nets);
         (void)hipFree(g.d_base);
+        (void)hipFree(g.d_nof);
+        (void)hipFree(g.d_chunks);
         (void)hipFree(g.d_sbase);
         (void)hipFree(g.d_tprev);
         (void)hipFree(g.d_sel);
@@ -1413,6 +1421,11 @@ static int regroup(bghip_batch *b)
             sbase[k + 1] = sbase[k] + dn.S;
         }
         g.total = base[nn];
+        if (nn > 65535) return set_err(b->ctx, BGHIP_EUNSUPPORTED, "more than 65535 networks of one kind in a batch");
+        std::vector<uint16_t> nof(std::max<uint32_t>(g.total, 1), 0);
+        for (int k = 0; k < nn; k++) std::fill(nof.begin() + base[k], nof.begin() + base[k + 1], (uint16_t)k);
+        HIP_TRY(b->ctx, hipMalloc((void **)&g.d_nof, sizeof(uint16_t) * nof.size()));
+        HIP_TRY(b->ctx, hipMemcpy(g.d_nof, nof.data(), sizeof(uint16_t) * nof.size(), hipMemcpyHostToDevice));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_nets, sizeof(BatchNet) * nn));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_base, sizeof(uint32_t) * (nn + 1)));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sbase, sizeof(uint64_t) * (nn + 1)));
@@ -1424,9 +1437,11 @@ static int regroup(bghip_batch *b)
         HIP_TRY(b->ctx, hipEventCreateWithFlags(&g.staged, hipEventDisableTiming));
         if (g.mode == BG_MODE_STEP && (g.sk == BGHIP_SYN_STDP || g.sk == BGHIP_SYN_DYNSTDP)) {
             std::vector<EvPart> parts(nn);
+            uint64_t cbase = 0;
             for (int k = 0; k < nn; k++) {
                 const DevNet &dn = b->nets[g.members[k]]->dn;
-                parts[k] = {base[k], (uint64_t)dn.N, (uint64_t)dn.M};
+                parts[k] = {cbase, (uint64_t)dn.N, (uint64_t)dn.M};
+                cbase += ((uint64_t)dn.N + BG_STREAM_DESTS - 1) / BG_STREAM_DESTS;
             }
             std::vector<uint32_t> doff, soff;
             if (!ev_offsets(parts, doff, soff))
@@ -1442,6 +1457,12 @@ static int regroup(bghip_batch *b)
             g.ev.syn_off = g.d_ev + nc + BG_EV_SHARDS + 1;
             g.ev.dest = g.d_ev + nc + no;
             g.ev.syn = g.d_evs;
+            std::vector<uint2> chunks;
+            for (int k = 0; k < nn; k++)
+                for (uint32_t d = 0; d < base[k + 1] - base[k]; d += BG_STREAM_DESTS) chunks.push_back(make_uint2(k, d));
+            g.nchunks = (uint32_t)chunks.size();
+            HIP_TRY(b->ctx, hipMalloc((void **)&g.d_chunks, sizeof(uint2) * std::max<size_t>(chunks.size(), 1)));
+            HIP_TRY(b->ctx, hipMemcpy(g.d_chunks, chunks.data(), sizeof(uint2) * chunks.size(), hipMemcpyHostToDevice));
         }
         HIP_TRY(b->ctx, hipMemcpy(g.d_base, base.data(), sizeof(uint32_t) * (nn + 1), hipMemcpyHostToDevice));
         HIP_TRY(b->ctx, hipMemcpy(g.d_sbase, sbase.data(), sizeof(uint64_t) * (nn + 1), hipMemcpyHostToDevice));
@@ -1495,10 +1516,11 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
             tab[k].pull = x->pending ? 1 : 0;
         }
         HIP_TRY(b->ctx, hipMemcpyAsync(g.d_nets, tab, sizeof(BatchNet) * nn, hipMemcpyHostToDevice, b->stream));
+        const BatchMap bm = {g.d_nets, g.d_base, g.d_nof, g.total, nn, g.d_chunks, g.nchunks};
         if (g.mode == BG_MODE_QUIET) {
             for (uint64_t done = 0; done < n_steps;) {
                 const int k = (int)std::min<uint64_t>(BG_QUIET_STEPS, n_steps - done);
-                launch_neuron_only_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, done, k);
+                launch_neuron_only_batch(bm, b->stream, g.nk, done, k);
                 done += (uint64_t)k;
             }
             for (int k = 0; k < nn; k++) {
@@ -1508,7 +1530,7 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
             }
         } else if (g.mode == BG_MODE_STEP) {
             for (uint64_t s = 0; s < n_steps; s++)
-                launch_step_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, g.sk, g.w32, s, g.ev);
+                launch_step_batch(bm, b->stream, g.nk, g.sk, g.w32, s, g.ev);
             for (int k = 0; k < nn; k++) {
                 bghip_net *x = b->nets[g.members[k]];
                 x->step += n_steps;
@@ -1519,8 +1541,7 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
             const uint64_t L = (n_steps + (uint64_t)Kfull - 1) / (uint64_t)Kfull;
             for (uint64_t l = 0; l < L; l++) {
                 const int K = (int)std::min<uint64_t>((uint64_t)Kfull, n_steps - l * (uint64_t)Kfull);
-                launch_fused_batch(g.d_nets, g.d_base, nn, g.total, b->stream, g.nk, g.sk, g.w32, g.kmax, K, Kfull,
-                                   (int)l);
+                launch_fused_batch(bm, b->stream, g.nk, g.sk, g.w32, g.kmax, K, Kfull, (int)l);
             }
             uint64_t *tprev = g.h_tprev;
             int *sel = g.h_sel;

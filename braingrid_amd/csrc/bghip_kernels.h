@@ -1,6 +1,7 @@
 // HIP kernels of the engine (gfx950). All memory-bound integer/FP32 work: no MFMA anywhere.
 #pragma once
 #include "bghip_device.h"
+#include "bghip_launch.h"
 
 #define BG_WAVE 64
 #define BG_BLOCK 256
@@ -190,88 +191,194 @@ __device__ __forceinline__ bool post_walk_learns(const DevNet &n, uint64_t slot,
     return !(delta <= 0 || (double)delta >= 3.0 * (double)n.taupos[slot]);
 }
 
-// Classifies and, when event-free, decays the synapses of ranks qa = q0 + lane and qb = qa + 64
-// whose ring word, meta and psr are given; lists event synapses.
-__device__ __forceinline__ void stdp_stream_pair(const DevNet &n, uint64_t t, uint32_t m, bool post, uint32_t w,
-                                                 uint32_t shard, const EvLists &ev, int lane, bool va, uint32_t qa,
-                                                 uint64_t sa, uint32_t ra, uint32_t ma, float pa, bool vb, uint32_t qb,
-                                                 uint64_t sb, uint32_t rb, uint32_t mb, float pb, float &acc,
-                                                 bool &any)
+// The per-type decay of a synapse from the four held in registers (no dependent load after meta).
+__device__ __forceinline__ float decay_of(const float (&dec)[4], uint32_t meta)
 {
-    const bool ea = va && ((ra & m) != 0 || (post && post_walk_learns(n, sa, ma, t)));
-    const bool eb = vb && ((rb & m) != 0 || (post && post_walk_learns(n, sb, mb, t)));
+    const uint32_t k = (meta >> BG_META_TYPE_SHIFT) & 3u;
+    return (k & 2u) ? ((k & 1u) ? dec[3] : dec[2]) : ((k & 1u) ? dec[1] : dec[0]);
+}
+
+// Classifies the synapses of ranks qa and qb = qa + 64 whose ring word, meta and psr are given:
+// ea / eb mark event synapses (a pre spike arrives, or the destination fired and the post-spike
+// walk can change W); event-free ones are decayed, stored and added to acc.
+__device__ __forceinline__ void stdp_stream_classify(const DevNet &n, const float (&dec)[4], uint64_t t, uint32_t m,
+                                                     bool post, bool va, uint64_t sa, uint32_t ra, uint32_t ma,
+                                                     float pa, bool vb, uint64_t sb, uint32_t rb, uint32_t mb,
+                                                     float pb, float &acc, bool &ea, bool &eb)
+{
+    ea = va && ((ra & m) != 0 || (post && post_walk_learns(n, sa, ma, t)));
+    eb = vb && ((rb & m) != 0 || (post && post_walk_learns(n, sb, mb, t)));
     if (va && !ea) {
-        pa = pa * n.decay[(ma >> BG_META_TYPE_SHIFT) & 3];
+        pa = pa * decay_of(dec, ma);
         n.psr[sa] = pa;
         acc += pa;
     }
     if (vb && !eb) {
-        pb = pb * n.decay[(mb >> BG_META_TYPE_SHIFT) & 3];
+        pb = pb * decay_of(dec, mb);
         n.psr[sb] = pb;
         acc += pb;
     }
-    const uint64_t ema = __ballot(ea), emb = __ballot(eb);
-    if (ema | emb) {
-        any = true;
-        const uint32_t na = (uint32_t)__popcll(ema);
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ev.count[(BG_EV_SHARDS + shard) * BG_EV_PAD], na + (uint32_t)__popcll(emb));
-        base = __shfl(base, 0) + ev.syn_off[shard];
-        const uint64_t below = (1ull << lane) - 1ull;
-        if (ea) ev.syn[base + (uint32_t)__popcll(ema & below)] = make_uint2(w, qa);
-        if (eb) ev.syn[base + na + (uint32_t)__popcll(emb & below)] = make_uint2(w, qb);
-    }
 }
 
-template <int SK, int W32>
-__device__ __forceinline__ bool stdp_stream_body(const DevNet &n, int d, int lane, uint64_t t, uint32_t w,
-                                                 const EvLists &ev)
+// Destinations d0 .. d0+nd-1 (nd <= D) of one network on one wavefront, all in event-list shard
+// `shard`. The pass is bound by memory latency: a destination's chain is header -> row -> store,
+// and a wave with one destination of ~100 synapses spends it waiting. Here the first 128 slots of
+// all D rows are loaded together with the headers (lane j loads destination j's), on the guess
+// that the rows are dense (no holes, so rank = slot offset, the usual case); a row the header
+// shows to have holes is reloaded through the incoming map. The chain is thus one level shorter
+// and paid once for D destinations; the wave's event synapses and event destinations are then
+// listed with one reservation per list. Each destination keeps its own lane-strided accumulation
+// and wave tree, so its sum is the same as with one destination per wave; rows longer than 128
+// ranks continue in a loop of their own.
+template <int SK, int W32, int D>
+__device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int nd, int lane, uint64_t t, uint32_t g0,
+                                                  uint32_t shard, const EvLists &ev)
 {
-    const uint32_t shard = w % BG_EV_SHARDS;
-    const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
-    const bool dense = (cnt == hi);
-    const bool post = n.fired_step[d] == t;
-    const float sp0 = n.sp[d];
+    const float dec[4] = {n.decay[0], n.decay[1], n.decay[2], n.decay[3]};
     const uint32_t tb = (uint32_t)(t & (uint64_t)(n.R - 1));
     const uint64_t wbase = (uint64_t)(tb >> 5) * n.S;
     const uint32_t m = 1u << (tb & 31);
-    const uint64_t row = (uint64_t)d * n.M;
-    const uint32_t *imap = n.in_map + (dense ? 0 : n.in_begin[d]);
-    float acc = 0.0f;
-    bool any = false;
-    // two lane-strided ranks per pass, all their loads issued before either is used
-    for (uint32_t q0 = 0; q0 < cnt; q0 += 2 * BG_WAVE) {
-        const uint32_t qa = q0 + (uint32_t)lane, qb = qa + BG_WAVE;
-        const bool va = qa < cnt, vb = qb < cnt;
-        uint64_t sa = 0, sb = 0;
-        uint32_t ra = 0, rb = 0, ma = 0, mb = 0;
-        float pa = 0.0f, pb = 0.0f;
-        if (va) {
-            sa = dense ? row + qa : (uint64_t)imap[qa];
-            ra = n.ring[wbase + sa]; ma = n.meta[sa]; pa = n.psr[sa];
-        }
-        if (vb) {
-            sb = dense ? row + qb : (uint64_t)imap[qb];
-            rb = n.ring[wbase + sb]; mb = n.meta[sb]; pb = n.psr[sb];
-        }
-        stdp_stream_pair(n, t, m, post, w, shard, ev, lane, va, qa, sa, ra, ma, pa, vb, qb, sb, rb, mb, pb, acc, any);
+    uint32_t h_cnt = 0, h_hi = 0, h_beg = 0;
+    uint64_t h_fs = BG_NO_SPIKE;
+    float h_sp = 0.0f;
+    if (lane < nd) {
+        const int d = d0 + lane;
+        h_cnt = n.syn_count[d];
+        h_hi = n.slot_hi[d];
+        h_beg = n.in_begin[d];
+        h_fs = n.fired_step[d];
+        h_sp = n.sp[d];
     }
-    if (any) {
-        if (lane == 0) ev.dest[ev.dest_off[shard] + atomicAdd(&ev.count[shard * BG_EV_PAD], 1u)] = w;
-        return true;
+    uint32_t cnt[D];
+    const uint32_t *imap[D];
+    uint64_t sa[D], sb[D];
+    uint32_t ra[D], rb[D], ma[D], mb[D];
+    float pa[D], pb[D];
+    const uint32_t qa = (uint32_t)lane, qb = qa + BG_WAVE;
+    const uint32_t M = (uint32_t)n.M;
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        const uint64_t row = (uint64_t)(d0 + j) * n.M;
+        sa[j] = row + qa; sb[j] = row + qb;
+        ra[j] = rb[j] = ma[j] = mb[j] = 0u;
+        pa[j] = pb[j] = 0.0f;
+        if (j < nd && qa < M) { ra[j] = n.ring[wbase + sa[j]]; ma[j] = n.meta[sa[j]]; pa[j] = n.psr[sa[j]]; }
+        if (j < nd && qb < M) { rb[j] = n.ring[wbase + sb[j]]; mb[j] = n.meta[sb[j]]; pb[j] = n.psr[sb[j]]; }
     }
-    acc = wave_sum(acc);
-    if (lane == 0) n.sp[d] = sp0 + acc;
-    return false;
+    const uint64_t dense_mask = __ballot(h_cnt == h_hi), post_mask = __ballot(lane < nd && h_fs == t);
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        cnt[j] = j < nd ? (uint32_t)__builtin_amdgcn_readlane((int)h_cnt, j) : 0u;
+        const bool dense = (dense_mask >> j) & 1ull;
+        imap[j] = dense ? nullptr : n.in_map + (uint32_t)__builtin_amdgcn_readlane((int)h_beg, j);
+        if (!dense) {
+            if (qa < cnt[j]) {
+                sa[j] = (uint64_t)imap[j][qa];
+                ra[j] = n.ring[wbase + sa[j]]; ma[j] = n.meta[sa[j]]; pa[j] = n.psr[sa[j]];
+            }
+            if (qb < cnt[j]) {
+                sb[j] = (uint64_t)imap[j][qb];
+                rb[j] = n.ring[wbase + sb[j]]; mb[j] = n.meta[sb[j]]; pb[j] = n.psr[sb[j]];
+            }
+        }
+    }
+    float acc[D];
+    uint64_t ema[D], emb[D];
+    uint32_t event_dests = 0;   // bit j: destination j has events (and is summed by the event pass)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        acc[j] = 0.0f;
+        ema[j] = emb[j] = 0ull;
+        if (j >= nd) continue;
+        const bool post = (post_mask >> j) & 1ull;
+        bool ea, eb;
+        stdp_stream_classify(n, dec, t, m, post, qa < cnt[j], sa[j], ra[j], ma[j], pa[j], qb < cnt[j], sb[j], rb[j],
+                             mb[j], pb[j], acc[j], ea, eb);
+        ema[j] = __ballot(ea);
+        emb[j] = __ballot(eb);
+        bool more = false;
+        if (cnt[j] > 2u * BG_WAVE) {   // long rows: the remaining ranks, listed pass by pass
+            const uint32_t w = g0 + (uint32_t)j;
+            const bool dense = (dense_mask >> j) & 1ull;
+            const uint64_t row = (uint64_t)(d0 + j) * n.M;
+            for (uint32_t q0 = 2 * BG_WAVE; q0 < cnt[j]; q0 += 2 * BG_WAVE) {
+                const uint32_t xa = q0 + (uint32_t)lane, xb = xa + BG_WAVE;
+                const bool va = xa < cnt[j], vb = xb < cnt[j];
+                uint64_t s1 = 0, s2 = 0;
+                uint32_t r1 = 0, r2 = 0, m1 = 0, m2 = 0;
+                float p1 = 0.0f, p2 = 0.0f;
+                if (va) {
+                    s1 = dense ? row + xa : (uint64_t)imap[j][xa];
+                    r1 = n.ring[wbase + s1]; m1 = n.meta[s1]; p1 = n.psr[s1];
+                }
+                if (vb) {
+                    s2 = dense ? row + xb : (uint64_t)imap[j][xb];
+                    r2 = n.ring[wbase + s2]; m2 = n.meta[s2]; p2 = n.psr[s2];
+                }
+                bool e1, e2;
+                stdp_stream_classify(n, dec, t, m, post, va, s1, r1, m1, p1, vb, s2, r2, m2, p2, acc[j], e1, e2);
+                const uint64_t b1 = __ballot(e1), b2 = __ballot(e2);
+                if (b1 | b2) {
+                    more = true;
+                    const uint32_t n1 = (uint32_t)__popcll(b1);
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&ev.count[(BG_EV_SHARDS + shard) * BG_EV_PAD], n1 + (uint32_t)__popcll(b2));
+                    base = __shfl(base, 0) + ev.syn_off[shard];
+                    const uint64_t below = (1ull << lane) - 1ull;
+                    if (e1) ev.syn[base + (uint32_t)__popcll(b1 & below)] = make_uint2(w, xa);
+                    if (e2) ev.syn[base + n1 + (uint32_t)__popcll(b2 & below)] = make_uint2(w, xb);
+                }
+            }
+        }
+        if (ema[j] | emb[j] | (uint64_t)more) event_dests |= 1u << j;
+    }
+    // the first-pass event synapses and the event destinations of all D rows, one reservation each
+    uint32_t nsyn = 0;
+#pragma unroll
+    for (int j = 0; j < D; j++) nsyn += (uint32_t)(__popcll(ema[j]) + __popcll(emb[j]));
+    if (event_dests) {
+        uint32_t bs = 0, bd = 0;
+        if (lane == 0) {
+            if (nsyn) bs = atomicAdd(&ev.count[(BG_EV_SHARDS + shard) * BG_EV_PAD], nsyn);
+            bd = atomicAdd(&ev.count[shard * BG_EV_PAD], (uint32_t)__popc(event_dests));
+        }
+        bs = (uint32_t)__shfl((int)bs, 0) + ev.syn_off[shard];
+        bd = (uint32_t)__shfl((int)bd, 0) + ev.dest_off[shard];
+        const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            if (!((event_dests >> j) & 1u)) continue;
+            const uint32_t w = g0 + (uint32_t)j;
+            if ((ema[j] >> lane) & 1ull) ev.syn[bs + (uint32_t)__popcll(ema[j] & below)] = make_uint2(w, qa);
+            bs += (uint32_t)__popcll(ema[j]);
+            if ((emb[j] >> lane) & 1ull) ev.syn[bs + (uint32_t)__popcll(emb[j] & below)] = make_uint2(w, qb);
+            bs += (uint32_t)__popcll(emb[j]);
+            if (lane == 0) ev.dest[bd] = w;
+            bd++;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        if (j >= nd || ((event_dests >> j) & 1u)) continue;
+        const float tot = wave_sum(acc[j]);
+        if (lane == j) n.sp[d0 + j] = h_sp + tot;
+    }
 }
 
+// destinations per wavefront of the streaming pass
+#ifndef BG_STREAM_DESTS
+#define BG_STREAM_DESTS 4
+#endif
 template <int SK, int W32>
 __global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream(DevNet n, uint64_t t)
 {
     const int lane = threadIdx.x & (BG_WAVE - 1);
-    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
-    if (d >= n.N) return;
-    stdp_stream_body<SK, W32>(n, d, lane, t, (uint32_t)d, n.ev);
+    const int d0 = __builtin_amdgcn_readfirstlane(
+        (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE) * BG_STREAM_DESTS);
+    if (d0 >= n.N) return;
+    const uint32_t chunk = (uint32_t)d0 / BG_STREAM_DESTS;
+    stdp_stream_dests<SK, W32, BG_STREAM_DESTS>(n, d0, min(BG_STREAM_DESTS, n.N - d0), lane, t, (uint32_t)d0,
+                                                chunk % BG_EV_SHARDS, n.ev);
 }
 
 // One event synapse (destination d, rank q): the body of the reference's advanceSynapse for a
@@ -531,118 +638,86 @@ struct BatchNet {
     int pull;           // 1 if those spikes are still to be inserted into the rings
 };
 
-// Network of thread work item w when the block's first item belongs to network i0 (found once per
-// block with batch_find): a block spans few networks, so a short forward walk suffices.
-__device__ __forceinline__ int batch_walk(const uint32_t *base, int nn, uint32_t w, int i0)
-{
-    int i = i0;
-    while (i + 1 < nn && base[i + 1] <= w) i++;
-    return i;
-}
-
-__device__ __forceinline__ int batch_find(const uint32_t *base, int nn, uint32_t w)
-{
-    int lo = 0, hi = nn;   // base[lo] <= w < base[hi]
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (base[mid] <= w) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 template <int NK, int SK, int W32, int KMAX>
 __global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_batch(
-    const BatchNet *nets, const uint32_t *wbase, int nn, int K, int Kfull, int launch)
+    BatchMap bm, int K, int Kfull, int launch)
 {
     // wave-uniform by construction (blockDim is a multiple of 64); readfirstlane lets the
     // compiler keep the network's descriptor in scalar registers
     const uint32_t w = __builtin_amdgcn_readfirstlane(
         (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
-    if (w >= wbase[nn]) return;
-    const int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, w));
-    const BatchNet &b = nets[i];
+    if (w >= bm.total) return;
+    const int i = __builtin_amdgcn_readfirstlane((int)bm.nof[w]);
+    const BatchNet &b = bm.nets[i];
     const uint64_t t0 = b.t0 + (uint64_t)launch * (uint64_t)Kfull;
     const int sel = b.sel ^ (launch & 1);
     const int pull = launch > 0 ? 1 : b.pull;
     const uint64_t tprev0 = launch > 0 ? t0 - (uint64_t)Kfull : b.tprev0;
-    fused_block_body<NK, SK, W32, KMAX>(b.n, (int)(w - wbase[i]), t0, K, pull, tprev0, sel);
+    fused_block_body<NK, SK, W32, KMAX>(b.n, (int)(w - bm.base[i]), t0, K, pull, tprev0, sel);
 }
 
 template <int NK>
-__global__ __launch_bounds__(BG_BLOCK) void k_neuron_batch(const BatchNet *nets, const uint32_t *nbase, int nn,
-                                                           uint64_t step)
+__global__ __launch_bounds__(BG_BLOCK) void k_neuron_batch(BatchMap bm, uint64_t step)
 {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t w0 = blockIdx.x * blockDim.x;
-    if (w >= nbase[nn]) return;
-    const int i = batch_walk(nbase, nn, w, __builtin_amdgcn_readfirstlane(batch_find(nbase, nn, w0)));
-    const BatchNet &b = nets[i];
-    neuron_step_body<NK>(b.n, (int)(w - nbase[i]), b.t0 + step);
+    if (w >= bm.total) return;
+    const int i = bm.nof[w];
+    const BatchNet &b = bm.nets[i];
+    neuron_step_body<NK>(b.n, (int)(w - bm.base[i]), b.t0 + step);
 }
 
 template <int SK, int W32>
-__global__ __launch_bounds__(BG_BLOCK) void k_synapse_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
-                                                            uint64_t step)
+__global__ __launch_bounds__(BG_BLOCK) void k_synapse_batch(BatchMap bm, uint64_t step)
 {
     const uint32_t w = __builtin_amdgcn_readfirstlane(
         (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
-    if (w >= wbase[nn]) return;
-    const int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, w));
-    const BatchNet &b = nets[i];
-    synapse_step_body<SK, W32>(b.n, (int)(w - wbase[i]), threadIdx.x & (BG_WAVE - 1), b.t0 + step);
+    if (w >= bm.total) return;
+    const int i = __builtin_amdgcn_readfirstlane((int)bm.nof[w]);
+    const BatchNet &b = bm.nets[i];
+    synapse_step_body<SK, W32>(b.n, (int)(w - bm.base[i]), threadIdx.x & (BG_WAVE - 1), b.t0 + step);
 }
 
 // The per-step STDP passes over a batch; event lists hold batch work-item indices.
 template <int NK>
-__global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons_batch(const BatchNet *nets, const uint32_t *nbase, int nn,
-                                                                 uint64_t step, EvLists ev)
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons_batch(BatchMap bm, uint64_t step, EvLists ev)
 {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (BG_WAVE - 1);
     if (w < 2 * BG_EV_SHARDS) ev.count[w * BG_EV_PAD] = 0;   // launches have at least 256 threads
     bool fired = false;
     int ni = 0, li = 0;
-    const int i0 = __builtin_amdgcn_readfirstlane(batch_find(nbase, nn, blockIdx.x * blockDim.x));
-    if (w < nbase[nn]) {
-        ni = batch_walk(nbase, nn, w, i0);
-        li = (int)(w - nbase[ni]);
-        fired = neuron_step_body<NK>(nets[ni].n, li, nets[ni].t0 + step);
+    if (w < bm.total) {
+        ni = bm.nof[w];
+        li = (int)(w - bm.base[ni]);
+        fired = neuron_step_body<NK>(bm.nets[ni].n, li, bm.nets[ni].t0 + step);
     }
     uint64_t fm = __ballot(fired);
     while (fm) {
         const int l = __ffsll((unsigned long long)fm) - 1;
         fm &= fm - 1;
         const int bi = __builtin_amdgcn_readfirstlane(__shfl(ni, l));
-        const BatchNet &b = nets[bi];
+        const BatchNet &b = bm.nets[bi];
         push_spike(b.n, __shfl(li, l), b.t0 + step, lane);
     }
 }
 
-// BG_STREAM_DESTS consecutive destinations per wave (one: the per-destination latency chain is
-// what bounds this pass, and more waves in flight hide it better than fewer, longer ones).
-#define BG_STREAM_DESTS 1
+// chunks[c] = {member, first destination} of the c-th run of BG_STREAM_DESTS destinations (runs
+// never cross networks)
 template <int SK, int W32>
-__global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
-                                                                uint64_t step, EvLists ev)
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream_batch(BatchMap bm, uint64_t step, EvLists ev)
 {
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(
+    const uint32_t c = __builtin_amdgcn_readfirstlane(
         (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
-    const uint32_t g0 = wv * BG_STREAM_DESTS, total = wbase[nn];
-    if (g0 >= total) return;
-    const uint32_t g1 = min(g0 + BG_STREAM_DESTS, total);
-    int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, g0));
-    const int lane = threadIdx.x & (BG_WAVE - 1);
-    for (uint32_t g = g0; g < g1; g++) {
-        while (wbase[i + 1] <= g) i++;
-        const BatchNet &b = nets[i];
-        stdp_stream_body<SK, W32>(b.n, (int)(g - wbase[i]), lane, b.t0 + step, g, ev);
-    }
+    if (c >= bm.nchunks) return;
+    const uint2 e = bm.chunks[c];
+    const int i = __builtin_amdgcn_readfirstlane((int)e.x), d0 = __builtin_amdgcn_readfirstlane((int)e.y);
+    const BatchNet &b = bm.nets[i];
+    stdp_stream_dests<SK, W32, BG_STREAM_DESTS>(b.n, d0, min(BG_STREAM_DESTS, b.n.N - d0), threadIdx.x & (BG_WAVE - 1),
+                                                b.t0 + step, bm.base[i] + (uint32_t)d0, c % BG_EV_SHARDS, ev);
 }
 
 template <int SK>
-__global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn_batch(const BatchNet *nets, const uint32_t *wbase, int nn,
-                                                                   uint64_t step, EvLists ev)
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn_batch(BatchMap bm, uint64_t step, EvLists ev)
 {
     const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
     const uint32_t G = gridDim.x / BG_EV_SHARDS;
@@ -650,14 +725,13 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn_batch(const BatchNe
     const uint2 *list = ev.syn + ev.syn_off[s];
     for (uint32_t k = g * BG_BLOCK + threadIdx.x; k < c; k += G * BG_BLOCK) {
         const uint2 e = list[k];
-        const int i = batch_find(wbase, nn, e.x);
-        const BatchNet &b = nets[i];
-        stdp_event_syn_body<SK>(b.n, (int)(e.x - wbase[i]), e.y, b.t0 + step);
+        const int i = bm.nof[e.x];
+        const BatchNet &b = bm.nets[i];
+        stdp_event_syn_body<SK>(b.n, (int)(e.x - bm.base[i]), e.y, b.t0 + step);
     }
 }
 
-static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum_batch(const BatchNet *nets, const uint32_t *wbase,
-                                                                          int nn, EvLists ev)
+static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum_batch(BatchMap bm, EvLists ev)
 {
     const int lane = threadIdx.x & (BG_WAVE - 1);
     const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
@@ -666,18 +740,17 @@ static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum_batch(const 
     const uint32_t *list = ev.dest + ev.dest_off[s];
     for (uint32_t k = g * wpb + threadIdx.x / BG_WAVE; k < c; k += G * wpb) {
         const uint32_t w = __builtin_amdgcn_readfirstlane(list[k]);
-        const int i = __builtin_amdgcn_readfirstlane(batch_find(wbase, nn, w));
-        stdp_sum_body(nets[i].n, (int)(w - wbase[i]), lane);
+        const int i = __builtin_amdgcn_readfirstlane((int)bm.nof[w]);
+        stdp_sum_body(bm.nets[i].n, (int)(w - bm.base[i]), lane);
     }
 }
 
 template <int NK>
-__global__ __launch_bounds__(BG_BLOCK) void k_neuron_only_batch(const BatchNet *nets, const uint32_t *nbase, int nn,
-                                                                uint64_t step, int steps)
+__global__ __launch_bounds__(BG_BLOCK) void k_neuron_only_batch(BatchMap bm, uint64_t step, int steps)
 {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nbase[nn]) return;
-    const int i = batch_walk(nbase, nn, w, __builtin_amdgcn_readfirstlane(batch_find(nbase, nn, blockIdx.x * blockDim.x)));
-    const BatchNet &b = nets[i];
-    neuron_only_body<NK>(b.n, (int)(w - nbase[i]), b.t0 + step, steps);
+    if (w >= bm.total) return;
+    const int i = bm.nof[w];
+    const BatchNet &b = bm.nets[i];
+    neuron_only_body<NK>(b.n, (int)(w - bm.base[i]), b.t0 + step, steps);
 }

@@ -16,9 +16,19 @@ void launch_synapse_step(const DevNet &dn, hipStream_t s, int sk, uint64_t t);
 void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int nk, int sk, uint64_t t);
 
 struct BatchNet;
-void launch_fused_batch(const BatchNet *nets, const uint32_t *wbase, int nn, uint32_t waves, hipStream_t s, int nk,
-                        int sk, int w32, int kmax, int K, int Kfull, int launch);
-void launch_neuron_only_batch(const BatchNet *nets, const uint32_t *base, int nn, uint32_t total, hipStream_t s,
-                              int nk, uint64_t step, int steps);
-void launch_step_batch(const BatchNet *nets, const uint32_t *base, int nn, uint32_t total, hipStream_t s, int nk,
-                       int sk, int w32, uint64_t step, const EvLists &ev);
+// Work items of a batch (destination waves or neuron threads) are numbered across the member
+// networks: member i owns items [base[i], base[i+1]) and nof[w] is the member of item w, so a
+// kernel finds its network with one load instead of a search.
+struct BatchMap {
+    const BatchNet *nets;
+    const uint32_t *base;   // nn + 1 prefix
+    const uint16_t *nof;    // total entries
+    uint32_t total;
+    int nn;
+    const uint2 *chunks;    // per-step STDP streaming pass: {member, first destination} runs
+    uint32_t nchunks;
+};
+void launch_fused_batch(const BatchMap &bm, hipStream_t s, int nk, int sk, int w32, int kmax, int K, int Kfull,
+                        int launch);
+void launch_neuron_only_batch(const BatchMap &bm, hipStream_t s, int nk, uint64_t step, int steps);
+void launch_step_batch(const BatchMap &bm, hipStream_t s, int nk, int sk, int w32, uint64_t step, const EvLists &ev);

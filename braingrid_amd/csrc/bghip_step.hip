@@ -57,7 +57,8 @@ template <int NK, int SK, int W32>
 static void stdp_go(const DevNet &dn, const DevNet *d_self, hipStream_t s, uint64_t t)
 {
     const unsigned nb = (unsigned)((dn.N + BG_BLOCK - 1) / BG_BLOCK);
-    const unsigned wb = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
+    const uint64_t waves = ((uint64_t)dn.N + BG_STREAM_DESTS - 1) / BG_STREAM_DESTS;
+    const unsigned wb = (unsigned)((waves * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
     hipLaunchKernelGGL(k_stdp_neurons<NK>, dim3(nb), dim3(BG_BLOCK), 0, s, dn, t);
     hipLaunchKernelGGL((k_stdp_stream<SK, W32>), dim3(wb), dim3(BG_BLOCK), 0, s, dn, t);
     hipLaunchKernelGGL(k_stdp_event_syn<SK>, dim3(event_grid(dn.N, 4)), dim3(BG_BLOCK), 0, s, d_self, t);
