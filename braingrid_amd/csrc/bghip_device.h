@@ -35,7 +35,7 @@ struct EvLists {
     const uint32_t *dest_off;   // SHARDS + 1 segment offsets into dest
     const uint32_t *syn_off;    // SHARDS + 1 segment offsets into syn
     uint32_t *dest;             // destinations with events
-    uint2 *syn;                 // event synapses as (destination, rank in its row)
+    uint2 *syn;                 // event synapses as (destination work item, slot)
 };
 
 struct DevNet {

@@ -325,8 +325,8 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
                     if (lane == 0) base = atomicAdd(&ev.count[(BG_EV_SHARDS + shard) * BG_EV_PAD], n1 + (uint32_t)__popcll(b2));
                     base = __shfl(base, 0) + ev.syn_off[shard];
                     const uint64_t below = (1ull << lane) - 1ull;
-                    if (e1) ev.syn[base + (uint32_t)__popcll(b1 & below)] = make_uint2(w, xa);
-                    if (e2) ev.syn[base + n1 + (uint32_t)__popcll(b2 & below)] = make_uint2(w, xb);
+                    if (e1) ev.syn[base + (uint32_t)__popcll(b1 & below)] = make_uint2(w, (uint32_t)s1);
+                    if (e2) ev.syn[base + n1 + (uint32_t)__popcll(b2 & below)] = make_uint2(w, (uint32_t)s2);
                 }
             }
         }
@@ -349,9 +349,9 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
         for (int j = 0; j < D; j++) {
             if (!((event_dests >> j) & 1u)) continue;
             const uint32_t w = g0 + (uint32_t)j;
-            if ((ema[j] >> lane) & 1ull) ev.syn[bs + (uint32_t)__popcll(ema[j] & below)] = make_uint2(w, qa);
+            if ((ema[j] >> lane) & 1ull) ev.syn[bs + (uint32_t)__popcll(ema[j] & below)] = make_uint2(w, (uint32_t)sa[j]);
             bs += (uint32_t)__popcll(ema[j]);
-            if ((emb[j] >> lane) & 1ull) ev.syn[bs + (uint32_t)__popcll(emb[j] & below)] = make_uint2(w, qb);
+            if ((emb[j] >> lane) & 1ull) ev.syn[bs + (uint32_t)__popcll(emb[j] & below)] = make_uint2(w, (uint32_t)sb[j]);
             bs += (uint32_t)__popcll(emb[j]);
             if (lane == 0) ev.dest[bd] = w;
             bd++;
@@ -384,10 +384,8 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream(DevNet n, uint64_t t)
 // One event synapse (destination d, rank q): the body of the reference's advanceSynapse for a
 // synapse with fPre or fPost.
 template <int SK>
-__device__ __forceinline__ void stdp_event_syn_body(const DevNet &n, int d, uint32_t q, uint64_t t)
+__device__ __forceinline__ void stdp_event_syn_body(const DevNet &n, int d, uint64_t slot, uint64_t t)
 {
-    const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
-    const uint64_t slot = cnt == hi ? (uint64_t)d * n.M + q : (uint64_t)n.in_map[n.in_begin[d] + q];
     const uint32_t tb = (uint32_t)(t & (uint64_t)(n.R - 1));
     uint32_t *wp = &n.ring[(uint64_t)(tb >> 5) * n.S + slot];
     const uint32_t m = 1u << (tb & 31);
@@ -421,17 +419,27 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn(const DevNet *np, u
     }
 }
 
+// The summation of a destination that had events: its psr values are final after the event pass.
+// The first 128 slots are loaded together with the header on the guess that the row is dense
+// (rank = slot offset), as in the streaming pass; a row with holes goes through the incoming map.
 __device__ __forceinline__ void stdp_sum_body(const DevNet &n, int d, int lane)
 {
+    const uint64_t row = (uint64_t)d * n.M;
+    const uint32_t M = (uint32_t)n.M, qa = (uint32_t)lane, qb = qa + BG_WAVE;
+    const float sa = qa < M ? n.psr[row + qa] : 0.0f, sb = qb < M ? n.psr[row + qb] : 0.0f;
     const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
-    const bool dense = (cnt == hi);
+    const float sp0 = n.sp[d];
     float acc = 0.0f;
-    for (uint32_t q = lane; q < cnt; q += BG_WAVE) {
-        const uint64_t slot = dense ? (uint64_t)d * n.M + q : (uint64_t)n.in_map[n.in_begin[d] + q];
-        acc += n.psr[slot];
+    if (cnt == hi) {
+        if (qa < cnt) acc += sa;
+        if (qb < cnt) acc += sb;
+        for (uint32_t q = qb + BG_WAVE; q < cnt; q += BG_WAVE) acc += n.psr[row + q];
+    } else {
+        const uint32_t *imap = n.in_map + n.in_begin[d];
+        for (uint32_t q = qa; q < cnt; q += BG_WAVE) acc += n.psr[imap[q]];
     }
     acc = wave_sum(acc);
-    if (lane == 0) n.sp[d] = n.sp[d] + acc;
+    if (lane == 0) n.sp[d] = sp0 + acc;
 }
 
 static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum(DevNet n)
