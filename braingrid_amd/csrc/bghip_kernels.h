@@ -195,7 +195,9 @@ __device__ __forceinline__ bool post_walk_learns(const DevNet &n, uint64_t slot,
 __device__ __forceinline__ float decay_of(const float (&dec)[4], uint32_t meta)
 {
     const uint32_t k = (meta >> BG_META_TYPE_SHIFT) & 3u;
-    return (k & 2u) ? ((k & 1u) ? dec[3] : dec[2]) : ((k & 1u) ? dec[1] : dec[0]);
+    const float d0 = dec[0], d1 = dec[1], d2 = dec[2], d3 = dec[3];
+    const float lo = (k & 1u) ? d1 : d0, hi = (k & 1u) ? d3 : d2;
+    return (k & 2u) ? hi : lo;
 }
 
 // Classifies the synapses of ranks qa and qb = qa + 64 whose ring word, meta and psr are given:
@@ -465,68 +467,220 @@ static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum(DevNet n)
 // loop, so the divergent transcendental work runs once per synapse and block rather than once
 // per step; later spikes in the same block (possible only when a source's refractory period is
 // shorter than the block) are handled inside the loop.
+// One synapse's cold state (W, and for DS r, u, lastSpike), loaded ahead when its delivery is
+// likely (valid) or on demand otherwise.
+struct ColdDS {
+    float W, r, u;
+    uint64_t last;
+    bool valid;
+};
+
+template <int SK>
+__device__ __forceinline__ void cold_load(const DevNet &n, uint64_t slot, ColdDS &c)
+{
+    c.W = n.W[slot];
+    if (SK == SK_DS) {
+        c.r = n.r[slot];
+        c.u = n.us[slot];
+        c.last = n.last[slot];
+    }
+    c.valid = true;
+}
+
+// The per-type constants of a network held in registers for a wavefront: a per-lane type then
+// selects among them instead of indexing an array of the descriptor (a memory access).
+struct TypeRegs {
+    float dec[4], U[4], D[4], F[4];
+};
+
+__device__ __forceinline__ float pick4(const float (&v)[4], int k)
+{   // selects values (not array elements), so the array stays in registers
+    const float v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
+    const float lo = (k & 1) ? v1 : v0, hi = (k & 1) ? v3 : v2;
+    return (k & 2) ? hi : lo;
+}
+
+__device__ __forceinline__ void type_regs(const DevNet &n, TypeRegs &tr)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        tr.dec[k] = n.decay[k];
+        tr.U[k] = n.tU[k];
+        tr.D[k] = n.tD[k];
+        tr.F[k] = n.tF[k];
+    }
+}
+
+// AllDSSynapses::changePSR's increment (Synapses/AllDSSynapses.cpp:227-247) on register state:
+// the same operations in the same order as psr_increment, with r, u and lastSpike carried in c.
+template <int SK>
+__device__ __forceinline__ float psr_increment_regs(const DevNet &n, const TypeRegs &tr, uint64_t slot, int type,
+                                                    uint64_t t, float decay, ColdDS &c, bool &ru_changed)
+{
+    if (SK == SK_DS) {
+        if (c.last != BG_NO_SPIKE) {
+            const float isi = (float)(t - c.last) * n.dt;
+            float D, F, U;
+            if (n.ds_typed) {
+                D = pick4(tr.D, type);
+                F = pick4(tr.F, type);
+                U = pick4(tr.U, type);
+            } else {
+                D = n.Dd[slot];
+                F = n.F[slot];
+                U = n.U[slot];
+            }
+            c.r = 1.0f + (c.r * (1.0f - c.u) - 1.0f) * bg_expf(-isi / D);
+            c.u = U + c.u * (1.0f - U) * bg_expf(-isi / F);
+            ru_changed = true;
+        }
+        c.last = t;
+        return ((c.W / decay) * c.u) * c.r;
+    } else {
+        return c.W / decay;
+    }
+}
+
+// A synapse through the K steps of a block: pull the spikes its source fired in the previous block
+// (fire mask fm) into the ring, take the block's deliveries out of it, and run the PSR with the
+// canonical per-step accumulation. The first delivery's increment is evaluated before the step
+// loop, so the divergent transcendental work runs once per synapse and block. With a 32-bit ring
+// the pulled spikes are one rotated copy of fm and the block's deliveries one rotated window of the
+// ring word; wider rings take them bit by bit.
+// Occupancy target of the fused kernels (waves per SIMD): bounds their registers.
+#ifndef BG_FUSED_WAVES
+#define BG_FUSED_WAVES 5
+#endif
+
 template <int SK, int W32, int KMAX>
-__device__ __forceinline__ void fused_synapse(const DevNet &n, uint64_t slot, uint32_t meta, uint32_t (&rw)[W32],
-                                              float &psr, uint64_t t0, int K, bool pull, uint64_t tprev0,
-                                              const uint32_t *mprev, float (&acc)[KMAX])
+__device__ __forceinline__ void fused_synapse(const DevNet &n, const TypeRegs &tr, uint64_t slot, uint32_t meta,
+                                              uint32_t (&rw)[W32], float &psr, uint32_t fm, ColdDS &c, uint64_t t0,
+                                              int K, uint64_t tprev0, float (&acc)[KMAX], uint32_t &hit)
 {
     const uint32_t Rm = (uint32_t)n.R - 1;
     const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
+    const uint32_t kmask = K >= 32 ? 0xFFFFFFFFu : ((1u << K) - 1u);
     bool changed = false;
-    if (pull) {
-        uint32_t fm = mprev[meta & BG_META_SRC_MASK];
-        const uint32_t delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
-        while (fm) {
-            const int b0 = __ffs(fm) - 1;
-            fm &= fm - 1;
-            const uint32_t b = (uint32_t)((tprev0 + (uint64_t)b0 + delay) & Rm);
-            const uint32_t m = 1u << (b & 31);
-#pragma unroll
-            for (int w = 0; w < W32; w++)
-                if ((int)(b >> 5) == w) {
-                    if (rw[w] & m) dev_error(n, 1u);
-                    rw[w] |= m;
-                }
+    uint32_t hm = 0;
+    if (W32 == 1) {
+        if (fm) {
+            // spike j of the previous block is delivered at bit (tprev0 + j + delay) mod 32
+            const uint32_t delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
+            const uint32_t sh = (uint32_t)((tprev0 + (uint64_t)delay) & 31u);
+            const uint32_t ins = (fm << sh) | (sh ? fm >> (32u - sh) : 0u);
+            if (rw[0] & ins) dev_error(n, 1u);
+            rw[0] |= ins;
             changed = true;
         }
-    }
-    uint32_t hm = 0;
+        const uint32_t b0 = (uint32_t)(t0 & 31u);
+        hm = __builtin_amdgcn_alignbit(rw[0], rw[0], b0) & kmask;   // bit k = ring bit (t0 + k) mod 32
+        const uint32_t clear = (kmask << b0) | (b0 ? kmask >> (32u - b0) : 0u);
+        rw[0] &= ~clear;
+    } else {
+        if (fm) {
+            const uint32_t delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
+            while (fm) {
+                const int b0 = __ffs(fm) - 1;
+                fm &= fm - 1;
+                const uint32_t b = (uint32_t)((tprev0 + (uint64_t)b0 + delay) & Rm);
+                const uint32_t m = 1u << (b & 31);
 #pragma unroll
-    for (int k = 0; k < KMAX; k++) {
-        if (k < K) {
-            const uint32_t b = (uint32_t)((t0 + (uint64_t)k) & Rm);
-            const uint32_t m = 1u << (b & 31);
+                for (int w = 0; w < W32; w++)
+                    if ((int)(b >> 5) == w) {
+                        if (rw[w] & m) dev_error(n, 1u);
+                        rw[w] |= m;
+                    }
+                changed = true;
+            }
+        }
 #pragma unroll
-            for (int w = 0; w < W32; w++)
-                if ((int)(b >> 5) == w && (rw[w] & m)) {
-                    rw[w] &= ~m;
-                    hm |= 1u << k;
-                }
+        for (int k = 0; k < KMAX; k++) {
+            if (k < K) {
+                const uint32_t b = (uint32_t)((t0 + (uint64_t)k) & Rm);
+                const uint32_t m = 1u << (b & 31);
+#pragma unroll
+                for (int w = 0; w < W32; w++)
+                    if ((int)(b >> 5) == w && (rw[w] & m)) {
+                        rw[w] &= ~m;
+                        hm |= 1u << k;
+                    }
+            }
         }
     }
-    const float decay = n.decay[type];
+    hit = hm;
+    const float decay = pick4(tr.dec, type);
     int k1 = -1;
     float c1 = 0.0f;
+    bool ru_changed = false;
     if (hm) {
         changed = true;
+        if (!c.valid) cold_load<SK>(n, slot, c);
         k1 = __ffs(hm) - 1;
-        c1 = psr_increment<SK>(n, slot, type, t0 + (uint64_t)k1, decay);
+        c1 = psr_increment_regs<SK>(n, tr, slot, type, t0 + (uint64_t)k1, decay, c, ru_changed);
         hm &= hm - 1;
     }
     float p = psr;
+    if (__ballot(hm != 0) == 0ull) {
+        // no lane has a second delivery in this block (a source's refractory period is longer
+        // than the block in every network so far): the step loop without the per-step test
 #pragma unroll
-    for (int k = 0; k < KMAX; k++) {
-        if (k < K) {
-            p = (k == k1) ? p + c1 : p;
-            if ((hm >> k) & 1u) p = p + psr_increment<SK>(n, slot, type, t0 + (uint64_t)k, decay);
-            p = p * decay;
-            acc[k] += p;
+        for (int k = 0; k < KMAX; k++) {
+            if (k < K) {
+                p = (k == k1) ? p + c1 : p;
+                p = p * decay;
+                acc[k] += p;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            if (k < K) {
+                p = (k == k1) ? p + c1 : p;
+                if ((hm >> k) & 1u)
+                    p = p + psr_increment_regs<SK>(n, tr, slot, type, t0 + (uint64_t)k, decay, c, ru_changed);
+                p = p * decay;
+                acc[k] += p;
+            }
         }
     }
     psr = p;
+    if (k1 >= 0 && SK == SK_DS) {
+        if (ru_changed) {
+            n.r[slot] = c.r;
+            n.us[slot] = c.u;
+        }
+        n.last[slot] = c.last;
+    }
     if (changed) {
 #pragma unroll
         for (int w = 0; w < W32; w++) n.ring[(uint64_t)w * n.S + slot] = rw[w];
+    }
+}
+
+// The hot state of one rank in the pipeline of fused_block_body.
+template <int W32>
+struct HotRank {
+    uint64_t slot;
+    uint32_t meta;
+    uint32_t rw[W32];
+    float psr;
+};
+
+template <int W32>
+__device__ __forceinline__ void hot_load(const DevNet &n, bool dense, uint64_t row, const uint32_t *imap, uint32_t q,
+                                         uint32_t cnt, HotRank<W32> &h)
+{
+    h.slot = 0;
+    h.meta = 0;
+#pragma unroll
+    for (int w = 0; w < W32; w++) h.rw[w] = 0;
+    h.psr = 0.0f;
+    if (q < cnt) {
+        h.slot = dense ? row + q : (uint64_t)imap[q];
+        h.meta = n.meta[h.slot];
+#pragma unroll
+        for (int w = 0; w < W32; w++) h.rw[w] = n.ring[(uint64_t)w * n.S + h.slot];
+        h.psr = n.psr[h.slot];
     }
 }
 
@@ -549,41 +703,49 @@ __device__ __forceinline__ void fused_block_body(const DevNet &n, int d, uint64_
 #pragma unroll
     for (int k = 0; k < KMAX; k++) acc[k] = 0.0f;
 
-    // lane-strided ranks (the canonical summation order), with the next rank's meta/ring/psr
-    // loads issued before the current rank is processed
+    // Lane-strided ranks (the canonical summation order) in a three-stage pipeline: while rank q is
+    // processed, rank q+64 has its source's fire mask and (when deliveries are frequent) its cold
+    // state in flight, and rank q+128 its meta/ring/psr. Cold state is fetched ahead only while the
+    // previous pass had at least 4 of 64 lanes with a delivery: then nearly every cache line of it
+    // is needed anyway, and otherwise the few delivering synapses load it on demand.
+    const bool do_pull = pull != 0;
+    TypeRegs tr;
+    type_regs(n, tr);
+    HotRank<W32> x, y, z;
+    ColdDS cx, cy, cz;
+    uint32_t fx = 0, fy = 0, fz = 0;
+    bool spec = false;
     uint32_t q = lane;
-    uint64_t slot = 0;
-    uint32_t meta = 0;
-    uint32_t rw[W32];
-    float psr = 0.0f;
-    if (q < cnt) {
-        slot = dense ? row + q : (uint64_t)imap[q];
-        meta = n.meta[slot];
-#pragma unroll
-        for (int w = 0; w < W32; w++) rw[w] = n.ring[(uint64_t)w * n.S + slot];
-        psr = n.psr[slot];
-    }
-    while (q < cnt) {
-        const uint32_t qn = q + BG_WAVE;
-        uint64_t slot_n = slot;
-        uint32_t meta_n = 0;
-        uint32_t rw_n[W32];
-        float psr_n = 0.0f;
-        if (qn < cnt) {
-            slot_n = dense ? row + qn : (uint64_t)imap[qn];
-            meta_n = n.meta[slot_n];
-#pragma unroll
-            for (int w = 0; w < W32; w++) rw_n[w] = n.ring[(uint64_t)w * n.S + slot_n];
-            psr_n = n.psr[slot_n];
+    hot_load<W32>(n, dense, row, imap, q, cnt, x);
+    hot_load<W32>(n, dense, row, imap, q + BG_WAVE, cnt, y);
+    cx.valid = false;
+    if (do_pull && q < cnt) fx = mprev[x.meta & BG_META_SRC_MASK];
+    // one pass: process `cur`, issue the fire-mask gather (and cold state) of `nxt`, whose hot state
+    // has arrived, and the hot loads of `far`. The loop is unrolled by three so that the three
+    // stages trade roles instead of being copied: a copy of registers whose loads are still in
+    // flight would wait for them and drain the pipeline.
+    auto pass = [&](HotRank<W32> &cur, ColdDS &ccur, uint32_t &fcur, HotRank<W32> &nxt, ColdDS &cnxt, uint32_t &fnxt,
+                    HotRank<W32> &far) {
+        hot_load<W32>(n, dense, row, imap, q + 2 * BG_WAVE, cnt, far);
+        cnxt.valid = false;
+        fnxt = 0;
+        if (q + BG_WAVE < cnt) {
+            if (do_pull) fnxt = mprev[nxt.meta & BG_META_SRC_MASK];
+            if (spec) cold_load<SK>(n, nxt.slot, cnxt);
         }
-        fused_synapse<SK, W32, KMAX>(n, slot, meta, rw, psr, t0, K, pull != 0, tprev0, mprev, acc);
-        n.psr[slot] = psr;
-        q = qn;
-        slot = slot_n;
-        meta = meta_n;
-#pragma unroll
-        for (int w = 0; w < W32; w++) rw[w] = rw_n[w];
-        psr = psr_n;
+        uint32_t hit = 0;
+        fused_synapse<SK, W32, KMAX>(n, tr, cur.slot, cur.meta, cur.rw, cur.psr, fcur, ccur, t0, K, tprev0, acc, hit);
+        n.psr[cur.slot] = cur.psr;
+        spec = __popcll(__ballot(hit != 0)) >= 4;
+        q += BG_WAVE;
+    };
+    while (true) {
+        if (q >= cnt) break;
+        pass(x, cx, fx, y, cy, fy, z);
+        if (q >= cnt) break;
+        pass(y, cy, fy, z, cz, fz, x);
+        if (q >= cnt) break;
+        pass(z, cz, fz, x, cx, fx, y);
     }
 #pragma unroll
     for (int k = 0; k < KMAX; k++)
@@ -625,7 +787,7 @@ __device__ __forceinline__ void fused_block_body(const DevNet &n, int d, uint64_
 }
 
 template <int NK, int SK, int W32, int KMAX>
-__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_block(
+__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_FUSED_WAVES))) void k_fused_block(
     DevNet n, uint64_t t0, int K, int pull, uint64_t tprev0, int sel_in)
 {
     const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
@@ -647,7 +809,7 @@ struct BatchNet {
 };
 
 template <int NK, int SK, int W32, int KMAX>
-__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_batch(
+__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_FUSED_WAVES))) void k_fused_batch(
     BatchMap bm, int K, int Kfull, int launch)
 {
     // wave-uniform by construction (blockDim is a multiple of 64); readfirstlane lets the
