@@ -10,7 +10,10 @@ static unsigned thread_blocks(uint32_t threads) { return (unsigned)(((uint64_t)t
 template <int NK, int SK, int W32>
 static void fused(const BatchMap &bm, hipStream_t s, int kmax, int K, int Kfull, int launch)
 {
-    if (kmax <= 16)
+    if (W32 == 1 && K == BG_K_COMMON)
+        hipLaunchKernelGGL((k_fused_batch<NK, SK, W32, BG_K_COMMON>), dim3(wave_blocks(bm.total)), dim3(BG_BLOCK), 0, s,
+                           bm, K, Kfull, launch);
+    else if (kmax <= 16)
         hipLaunchKernelGGL((k_fused_batch<NK, SK, W32, 16>), dim3(wave_blocks(bm.total)), dim3(BG_BLOCK), 0, s, bm, K,
                            Kfull, launch);
     else

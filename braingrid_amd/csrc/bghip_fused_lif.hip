@@ -6,7 +6,10 @@ template <int SK, int W32>
 static void go(const DevNet &dn, hipStream_t s, int kmax, uint64_t t0, int K, int pull, uint64_t tprev0, int sel)
 {
     unsigned blocks = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
-    if (kmax <= 16)
+    if (W32 == 1 && K == BG_K_COMMON)   // the usual full block of a 32-bit-ring network
+        hipLaunchKernelGGL((k_fused_block<NK_LIF, SK, W32, BG_K_COMMON>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t0, K,
+                           pull, tprev0, sel);
+    else if (kmax <= 16)
         hipLaunchKernelGGL((k_fused_block<NK_LIF, SK, W32, 16>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t0, K, pull,
                            tprev0, sel);
     else

@@ -519,7 +519,10 @@ __device__ __forceinline__ float psr_increment_regs(const DevNet &n, const TypeR
 {
     if (SK == SK_DS) {
         if (c.last != BG_NO_SPIKE) {
-            const float isi = (float)(t - c.last) * n.dt;
+            // steps since the last delivery, converted from 32 bits when they fit (the same value,
+            // a shorter conversion than from 64 bits)
+            const uint64_t gap = t - c.last;
+            const float isi = ((gap >> 32) ? (float)gap : (float)(uint32_t)gap) * n.dt;
             float D, F, U;
             if (n.ds_typed) {
                 D = pick4(tr.D, type);
@@ -547,6 +550,13 @@ __device__ __forceinline__ float psr_increment_regs(const DevNet &n, const TypeR
 // loop, so the divergent transcendental work runs once per synapse and block. With a 32-bit ring
 // the pulled spikes are one rotated copy of fm and the block's deliveries one rotated window of the
 // ring word; wider rings take them bit by bit.
+// Block length of the fused kernels: KMAX 16 and 32 take any K <= KMAX at run time (the step loops
+// are unrolled to KMAX with a guard on k < K, which the compiler turns into selects on every
+// step); any other KMAX is an instantiation for exactly KMAX steps, with no guard at all.
+#define BG_K_EXACT(KMAX) ((KMAX) != 16 && (KMAX) != 32)
+// the exact block length instantiated: the minimum reference delay at dt = 0.1 ms, int(0.8/0.1)+1
+#define BG_K_COMMON 9
+
 // Occupancy target of the fused kernels (waves per SIMD): bounds their registers.
 #ifndef BG_FUSED_WAVES
 #define BG_FUSED_WAVES 5
@@ -559,7 +569,8 @@ __device__ __forceinline__ void fused_synapse(const DevNet &n, const TypeRegs &t
 {
     const uint32_t Rm = (uint32_t)n.R - 1;
     const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
-    const uint32_t kmask = K >= 32 ? 0xFFFFFFFFu : ((1u << K) - 1u);
+    const int Kb = BG_K_EXACT(KMAX) ? KMAX : K;
+    const uint32_t kmask = Kb >= 32 ? 0xFFFFFFFFu : ((1u << Kb) - 1u);
     bool changed = false;
     uint32_t hm = 0;
     if (W32 == 1) {
@@ -595,7 +606,7 @@ __device__ __forceinline__ void fused_synapse(const DevNet &n, const TypeRegs &t
         }
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            if (k < K) {
+            if (BG_K_EXACT(KMAX) || k < K) {
                 const uint32_t b = (uint32_t)((t0 + (uint64_t)k) & Rm);
                 const uint32_t m = 1u << (b & 31);
 #pragma unroll
@@ -625,7 +636,7 @@ __device__ __forceinline__ void fused_synapse(const DevNet &n, const TypeRegs &t
         // than the block in every network so far): the step loop without the per-step test
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            if (k < K) {
+            if (BG_K_EXACT(KMAX) || k < K) {
                 p = (k == k1) ? p + c1 : p;
                 p = p * decay;
                 acc[k] += p;
@@ -634,7 +645,7 @@ __device__ __forceinline__ void fused_synapse(const DevNet &n, const TypeRegs &t
     } else {
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            if (k < K) {
+            if (BG_K_EXACT(KMAX) || k < K) {
                 p = (k == k1) ? p + c1 : p;
                 if ((hm >> k) & 1u)
                     p = p + psr_increment_regs<SK>(n, tr, slot, type, t0 + (uint64_t)k, decay, c, ru_changed);
@@ -749,7 +760,7 @@ __device__ __forceinline__ void fused_block_body(const DevNet &n, int d, uint64_
     }
 #pragma unroll
     for (int k = 0; k < KMAX; k++)
-        if (k < K) acc[k] = wave_sum(acc[k]);
+        if (BG_K_EXACT(KMAX) || k < K) acc[k] = wave_sum(acc[k]);
 
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -773,7 +784,7 @@ __device__ __forceinline__ void fused_block_body(const DevNet &n, int d, uint64_
         uint32_t mask = 0;
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            if (k < K) {
+            if (BG_K_EXACT(KMAX) || k < K) {
                 if ((inbits >> k) & 1u) sp = sp + s_in[wv][k];
                 if (neuron_advance<NK>(n, d, t0 + (uint64_t)k, sp, s_noise[wv][k], s)) mask |= 1u << k;
                 sp = 0.0f + acc[k];

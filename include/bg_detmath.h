@@ -314,8 +314,10 @@ static inline BG_HD double bg_exp_for_float(double x)
 
 /* exp in single precision for the per-spike synapse updates: x = k ln2 + r with a 16-bit ln2_hi
  * (k * ln2_hi exact for |k| < 256), |r| <= ln2/2, Taylor series to r^7 (truncation below 5e-9
- * relative), scaled by 2^k through the exponent bits. About one ulp from the correctly rounded
- * value; every operation is a single IEEE float operation, so host and device agree bit for bit. */
+ * relative) in Horner form with fused multiply-adds, scaled by 2^k through the exponent bits.
+ * About one ulp from the correctly rounded value; every operation is a single correctly rounded
+ * IEEE float operation (fmaf included: one rounding on the host's C library and on the device's
+ * v_fma_f32 alike), so host and device agree bit for bit. */
 static inline BG_HD float bg_scale2f(float x, int k)
 {
     uint32_t b;
@@ -349,15 +351,15 @@ static inline BG_HD float bg_expf(float x)
     float t = x * 1.4426950216293335f;
     int k = (int)(t + (t >= 0.0f ? 0.5f : -0.5f));
     float kf = (float)k;
-    float r = (x - kf * ln2_hi) - kf * ln2_lo;
+    float r = __builtin_fmaf(-kf, ln2_lo, x - kf * ln2_hi);
     float p = 0.00019841270113829523f;
-    p = p * r + 0.0013888889225199819f;
-    p = p * r + 0.008333333767950535f;
-    p = p * r + 0.0416666679084301f;
-    p = p * r + 0.1666666716337204f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
+    p = __builtin_fmaf(p, r, 0.0013888889225199819f);
+    p = __builtin_fmaf(p, r, 0.008333333767950535f);
+    p = __builtin_fmaf(p, r, 0.0416666679084301f);
+    p = __builtin_fmaf(p, r, 0.1666666716337204f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
     return bg_scale2f(p, k);
 }
 
