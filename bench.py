@@ -292,6 +292,8 @@ def main():
     launch_bytes = (DS_SYN_STEP_BYTES * S_rl + LIF_NEURON_STEP_BYTES * N) * (rl_steps / max(1, launches))
     achieved = launch_bytes / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
     traffic = load_traffic()
+    # the instantiation the full blocks run: exactly 9 steps for a 32-bit ring, else up to 16 / 32
+    kernel_name = "k_fused_block<LIF,DS,1,%d>" % (9 if K == 9 else (16 if K <= 16 else 32))
     out = {
         "metric": "synapse-updates/s at 10k-LIF growth",
         "value": value,
@@ -314,7 +316,7 @@ def main():
         "sim_seconds_per_hour": args.steps * 1e-4 / wall * 3600.0,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": "k_fused_block<LIF,DS,1,16>", "avg_launch_ms": avg_ms, "launches": launches,
+                     "kernel": kernel_name, "avg_launch_ms": avg_ms, "launches": launches,
                      "steps_per_launch": rl_steps / max(1, launches),
                      "algorithmic_bytes_per_launch": launch_bytes,
                      "traffic_GBps": (traffic["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9) if traffic and avg_ms > 0 else None,
