@@ -33,7 +33,13 @@ class Params(ctypes.Structure):
                                    "starter_Vreset", "excA", "excB", "excC", "excD", "inhA", "inhB", "inhC", "inhD")] + \
                [(k, ctypes.c_float) for k in ("epsilon", "beta", "rho", "target_rate", "min_radius", "start_radius")] + \
                [("block_steps", ctypes.c_int32), ("static_nconns", ctypes.c_int32), ("static_thresh", ctypes.c_float),
-                ("static_exc_w", _F2), ("static_inh_w", _F2)]
+                ("static_exc_w", _F2), ("static_inh_w", _F2),
+                ("input_kind", ctypes.c_int32), ("input_fr_mean", ctypes.c_float), ("input_weight", ctypes.c_float),
+                ("input_mask", ctypes.POINTER(ctypes.c_int32)), ("n_input_mask", ctypes.c_int32),
+                ("input_duration", ctypes.c_float), ("input_interval", ctypes.c_float), ("input_sync", ctypes.c_int32),
+                ("input_values", ctypes.c_float * 100)]
+
+INPUT_SYNC = {"yes": 0, "no": 1, "wave": 2}
 
 CONN_KINDS = {"ConnGrowth": 1, "ConnStatic": 2}
 
@@ -149,6 +155,36 @@ EXPORTED = ("bghost_model_create", "bghost_model_destroy", "bghost_model_net", "
 _ARRAY_DTYPES = {"neuron_type_map": np.int32, "starter_map": np.uint8}
 
 
+def read_stimulus(path: str, base_dir: str | None = None) -> dict:
+    """A stimulus input file (the -s option; Inputs/FSInput.cpp:40-97): SInputPoisson
+    (Inputs/SInputPoisson.cpp:24-108: IntParams fr_mean and weight, optional <Masks> list or mask
+    file) or SInputRegular (Inputs/SInputRegular.cpp:26-110: IntParams duration, interval, sync and
+    the 10x10 <Values><I> table). The result goes into a config as cfg["input"]."""
+    base_dir = base_dir or os.path.dirname(os.path.abspath(path))
+    root = ET.parse(path).getroot()
+    parms = root if root.tag == "InputParams" else root.find("InputParams")
+    name = parms.find("IMethod").get("name")
+    ip = parms.find("IntParams")
+    if name == "SInputPoisson":
+        inp = dict(kind="poisson", fr_mean=float(ip.get("fr_mean")), weight=float(ip.get("weight")), mask=None)
+        masks = parms.find("Masks")
+        if masks is not None:
+            for node in masks:
+                if node.tag == "M":
+                    inp["mask"] = [int(float(v)) for v in _t(node).split()]
+                elif node.tag == "LayoutFiles" and node.get("maskNListFileName"):
+                    f = os.path.join(base_dir, node.get("maskNListFileName"))
+                    inp["mask"] = [int(float(v)) for v in _t(ET.parse(f).getroot()).split()]
+        return inp
+    if name == "SInputRegular":
+        vals = [float(v) for v in _t(parms.find("Values/I")).split()]
+        if len(vals) != 100:
+            raise ValueError("SInputRegular needs a 10x10 <I> table")
+        return dict(kind="regular", duration=float(ip.get("duration")), interval=float(ip.get("interval")),
+                    sync=ip.get("sync"), values=vals)
+    raise ValueError(f"unknown input method {name}")
+
+
 def make_params(c: dict):
     p = Params()
     p.width, p.height = c["width"], c["height"]
@@ -173,7 +209,26 @@ def make_params(c: dict):
     for k in ("epsilon", "beta", "rho", "target_rate", "min_radius", "start_radius"):
         setattr(p, k, c[k])
     p.block_steps = c.get("block_steps", 0)
-    return p, (inh, act)
+    keep = [inh, act]
+    inp = c.get("input")
+    if inp and inp["kind"] == "poisson":
+        p.input_kind, p.input_fr_mean, p.input_weight = B.INPUT_POISSON, inp["fr_mean"], inp["weight"]
+        if inp.get("mask") is None:
+            p.n_input_mask = -1
+        else:
+            m = (ctypes.c_int32 * max(1, len(inp["mask"])))(*inp["mask"])
+            keep.append(m)
+            p.input_mask, p.n_input_mask = ctypes.cast(m, ctypes.POINTER(ctypes.c_int32)), len(inp["mask"])
+    elif inp and inp["kind"] == "regular":
+        p.input_kind, p.input_duration, p.input_interval = B.INPUT_REGULAR, inp["duration"], inp["interval"]
+        if inp["sync"] not in INPUT_SYNC:
+            raise ValueError(f"SInputRegular sync {inp['sync']!r} is not one of yes / no / wave")
+        p.input_sync = INPUT_SYNC[inp["sync"]]
+        for k, v in enumerate(inp["values"]):
+            p.input_values[k] = v
+    elif inp:
+        raise ValueError(f"unknown input kind {inp['kind']!r}")
+    return p, tuple(keep)
 
 
 class Model:

@@ -116,6 +116,8 @@ class HipSpikingModel {
     void setupLayout();
     void createAllNeurons();
     int setupStaticConnections();
+    int constructInput();
+    int initInput();
     bghip_synapse_types synapseTypes() const;
 
     bghip_ctx *ctx_;
@@ -130,6 +132,8 @@ class HipSpikingModel {
     std::vector<float> Vm_, Vthresh_, Vreset_, Vrest_, Vinit_, I0_, Inoise_, Iinject_, C1_, C2_, Trefract_, Tau_, Rm_;
     std::vector<float> A_, B_, C_, D_, u_, C3_;
     std::vector<float> radii_, rates_, radiiHistory_, ratesHistory_, burst_, spikesHist_;
+    std::vector<float> in_values_;     // regular input: per-neuron pulse value
+    std::vector<int32_t> in_shift_;    // regular input: per-neuron cycle shift
     std::vector<int32_t> spikeCount_, spikeOffset_;
     std::vector<uint64_t> hist_;
     bghip_growth growth_;
@@ -166,6 +170,64 @@ void HipSpikingModel::setupLayout()
             }
         }
     }
+}
+
+int HipSpikingModel::constructInput()
+{   // SInputRegular's constructor (Inputs/SInputRegular.cpp:26-110): the <Values> table tiled over
+    // the grid and, for sync "no", one shift per neuron drawn from the global rng; "wave" shifts by
+    // column. SInputPoisson draws nothing here.
+    if (p_.input_kind != BGHIP_INPUT_REGULAR) return BGHIP_OK;
+    const int W = p_.width, H = p_.height;
+    const float dt = p_.dt;
+    const int dur = (int)((double)(p_.input_duration / dt) + 0.5);
+    const int itv = (int)((double)(p_.input_interval / dt) + 0.5);
+    const int cycle = dur + itv;
+    if (cycle <= 0) return BGHIP_EINVAL;
+    in_values_.assign((size_t)N_, 0.0f);
+    in_shift_.assign((size_t)N_, 0);
+    for (int i = 0; i < H; i++)
+        for (int j = 0; j < W; j++) in_values_[(size_t)i * W + j] = p_.input_values[(i % 10) * 10 + j % 10];
+    if (p_.input_sync == BGHOST_SYNC_NO) {
+        for (int i = 0; i < H; i++)
+            for (int j = 0; j < W; j++) in_shift_[(size_t)i * W + j] = (int)rng_.inRange(0.0f, (float)(cycle - 1));
+    } else if (p_.input_sync == BGHOST_SYNC_WAVE) {
+        for (int i = 0; i < H; i++)
+            for (int j = 0; j < W; j++) in_shift_[(size_t)i * W + j] = (cycle / W) * j;
+    } else if (p_.input_sync != BGHOST_SYNC_YES) {
+        return BGHIP_EINVAL;
+    }
+    return BGHIP_OK;
+}
+
+int HipSpikingModel::initInput()
+{   // ISInput::init on the device (Inputs/SInputPoisson.cpp:124-146, Inputs/SInputRegular.cpp)
+    if (p_.input_kind == BGHIP_INPUT_NONE) return BGHIP_OK;
+    bghip_input in;
+    memset(&in, 0, sizeof in);
+    in.kind = p_.input_kind;
+    std::vector<uint8_t> mask;
+    if (p_.input_kind == BGHIP_INPUT_POISSON) {
+        in.fr_mean = p_.input_fr_mean;
+        in.weight = p_.input_weight;
+        in.neuron_type = ntype_.data();
+        if (p_.n_input_mask >= 0) {
+            mask.assign((size_t)N_, 0);
+            for (int k = 0; k < p_.n_input_mask; k++) {
+                const int i = p_.input_mask[k];
+                if (i < 0 || i >= N_) return BGHIP_EINVAL;
+                mask[i] = 1;
+            }
+            in.mask = mask.data();
+        }
+    } else if (p_.input_kind == BGHIP_INPUT_REGULAR) {
+        in.duration = p_.input_duration;
+        in.interval = p_.input_interval;
+        in.values = in_values_.data();
+        in.shift = in_shift_.data();
+    } else {
+        return BGHIP_EINVAL;
+    }
+    return bghip_upload_input(net_, &in, nullptr);
 }
 
 namespace {
@@ -311,6 +373,8 @@ int HipSpikingModel::setupSim()
     burst_.assign((size_t)(int)(p_.epoch_duration * p_.num_epochs), 0.0f);
     spikesHist_.assign((size_t)(int)(p_.epoch_duration * p_.num_epochs * 100), 0.0f);
     rng_.reseed(1u);
+    int rc0 = constructInput();   // the stimulus object exists before Simulator::setup
+    if (rc0) return rc0;
     setupLayout();
     createAllNeurons();
     for (size_t i = 0; i < N; i++) {   // ConnGrowth::setupConnections, recorder initDefaultValues
@@ -352,6 +416,7 @@ int HipSpikingModel::setupSim()
     if ((rc = bghip_upload_growth(net_, &growth_, radii_.data(), rates_.data()))) return rc;
     // m_conns->setupConnections (Core/Model.cpp:154): ConnGrowth starts without synapses
     if (p_.conn_kind == BGHOST_CONN_STATIC && (rc = setupStaticConnections())) return rc;
+    if ((rc = initInput())) return rc;   // pInput->init after the model's setup
     // the recorder's histories live on the device (XmlRecorder::initDefaultValues and
     // XmlGrowthRecorder::initDefaultValues give row 0 and zero bins)
     const bool growth = p_.conn_kind == BGHOST_CONN_GROWTH;
@@ -491,7 +556,7 @@ const void *HipSpikingModel::array(const char *name, int64_t *count)
 
 struct bghost_model {
     bg::HipSpikingModel *m;
-    std::vector<int32_t> inh, act;
+    std::vector<int32_t> inh, act, inmask;
 };
 
 extern "C" int bghost_model_create(bghip_ctx *ctx, const bghost_params *p, bghost_model **out)
@@ -502,8 +567,11 @@ extern "C" int bghost_model_create(bghip_ctx *ctx, const bghost_params *p, bghos
     bghost_params q = *p;
     w->inh.assign(p->inh_list, p->inh_list + (p->inh_list ? p->n_inh : 0));
     w->act.assign(p->active_list, p->active_list + (p->active_list ? p->n_active : 0));
+    w->inmask.assign(p->input_mask, p->input_mask + (p->input_mask && p->n_input_mask > 0 ? p->n_input_mask : 0));
     q.inh_list = w->inh.data();
     q.active_list = w->act.data();
+    q.input_mask = w->inmask.data();
+    if (!p->input_mask && q.n_input_mask > 0) q.n_input_mask = 0;
     const int N = p->width * p->height;
     for (int v : w->inh)
         if (v < 0 || v >= N) { delete w; return BGHIP_EINVAL; }

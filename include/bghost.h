@@ -42,11 +42,25 @@ typedef struct bghost_params {
      * weight ranges [minExc, maxExc], [minInh, maxInh] */
     int32_t static_nconns;
     float static_thresh, static_exc_w[2], static_inh_w[2];
+    /* stimulus input of the -s file (Inputs/FSInput.cpp:40-97): BGHIP_INPUT_NONE / _POISSON /
+     * _REGULAR. SInputPoisson: IntParams fr_mean (Hz) and weight (x 1e-8), <Masks> neuron list
+     * (n_input_mask < 0: every neuron). SInputRegular: duration and interval (s), sync
+     * BGHOST_SYNC_*, the 10x10 <Values><I> table (tiled over the grid). */
+    int32_t input_kind;
+    float input_fr_mean, input_weight;
+    const int32_t *input_mask; int32_t n_input_mask;
+    float input_duration, input_interval;
+    int32_t input_sync;
+    float input_values[100];
 } bghost_params;
+
+enum { BGHOST_SYNC_YES = 0, BGHOST_SYNC_NO = 1, BGHOST_SYNC_WAVE = 2 };
 
 typedef struct bghost_model bghost_model;
 
-/* Model::setupSim + HipSpikingModel device setup (allocate, upload). */
+/* Model::setupSim + HipSpikingModel device setup (allocate, upload), with the stimulus input
+ * constructed before the model's draws and initialised after them, as Core/BGDriver.cpp:76-87 and
+ * Core/Simulator.cpp:40-43 order them. */
 int bghost_model_create(bghip_ctx *ctx, const bghost_params *p, bghost_model **out);
 void bghost_model_destroy(bghost_model *m);
 bghip_net *bghost_model_net(bghost_model *m);

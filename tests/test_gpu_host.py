@@ -216,3 +216,32 @@ def test_host_static_connections_match_oracle(hip_ctx, name):
     finally:
         m.close()
         ora.close()
+
+
+@pytest.mark.parametrize("name", list(H.INPUT_STIM))
+def test_host_model_with_stimulus_input_matches_oracle(hip_ctx, tmp_path, name):
+    """The host model with a stimulus file (BrainGrid's -s): SInputRegular's asynchronous shifts
+    drawn before the model's draws, the wave shifts, and SInputPoisson's masked DS input synapses,
+    then every epoch's state, the input state and the XML output against the oracle."""
+    print(f"[host] input {name}", flush=True)
+    cfg = _host_cfg(name)
+    cfg["input"] = host.read_stimulus(os.path.join(H.CONFIG_DIR, H.INPUT_STIM[name]), H.CONFIG_DIR)
+    ocfg = H.load_config(name)
+    m = host.Model(hip_ctx, cfg)
+    ora = O.OracleNet(ocfg)
+    try:
+        names = SETUP_ARRAYS + (IZH_ARRAYS if "izh" in name else ())
+        bad = [k for k in names if not np.array_equal(m.array(k).view(np.uint8), ora.array(k).view(np.uint8))]
+        bad += H.compare_input(ora, m.net, ocfg, "setup")
+        assert not bad, bad
+        for ep in range(cfg["num_epochs"]):
+            m.run_epoch()
+            ora.run_epoch()
+            bad = H.compare_state(ora, m.net, ocfg, f"epoch{ep}") + H.compare_input(ora, m.net, ocfg, f"epoch{ep}")
+            assert not bad, bad
+        p = os.path.join(tmp_path, "out.xml")
+        m.save_xml(p)
+        assert open(p).read() == xmlout.oracle_growth_xml(ora, ocfg)
+    finally:
+        m.close()
+        ora.close()
