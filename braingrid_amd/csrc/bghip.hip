@@ -62,6 +62,9 @@ struct bghip_net {
     DevNet *d_self;        // device copy of dn (dn.self), refreshed when dn changes
     uint32_t *d_ev_off;    // per-step STDP: shard offsets of the event lists (dest, then syn)
     DevNet dn_shadow;      // dn as last copied to d_self
+    // per-step STDP: BG_GRAPH_STEPS steps captured as one graph, valid while dn equals g_dn
+    hipGraphExec_t g_exec = nullptr;
+    DevNet g_dn;
 };
 
 // Orders the network's own stream after outstanding batch work on it.
@@ -217,6 +220,7 @@ static void plan_net(bghip_net *n, Carve &c)
         c.add(&dn.ev.dest, N); c.add(&dn.ev.syn, S);
     }
     c.add(&n->d_self, 1);
+    c.add(&dn.tbase, 1);
     c.add(&n->d_out_map, S); c.add(&n->d_out_begin, N); c.add(&n->d_out_count, N); c.add(&n->d_cursor, N);
     c.add(&n->d_stats, 4); c.add(&n->d_total, 1);
 }
@@ -316,11 +320,18 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
     return BGHIP_OK;
 }
 
+static void drop_graph(bghip_net *n)
+{
+    if (n->g_exec) (void)hipGraphExecDestroy(n->g_exec);
+    n->g_exec = nullptr;
+}
+
 extern "C" void bghip_net_destroy(bghip_net *n)
 {
     if (!n) return;
     (void)hipSetDevice(n->ctx->device);
     (void)hipStreamSynchronize(n->stream);
+    drop_graph(n);
     (void)hipEventDestroy(n->ev0);
     (void)hipEventDestroy(n->ev1);
     (void)hipEventDestroy(n->ext_ev);
@@ -594,6 +605,29 @@ static void flush_pending(bghip_net *n)
     n->pending = false;
 }
 
+// Captures BG_GRAPH_STEPS per-step STDP steps (offsets 0..S-1 from *tbase) and the tbase bump as
+// one executable graph. Capture records only the kernels; work already queued on the stream stays
+// ordered before each replay.
+static int capture_stdp_graph(bghip_net *n)
+{
+    const DevNet &dn = n->dn;
+    const int nk = n->desc.neuron_model, sk = n->desc.synapse_model;
+    hipGraph_t g = nullptr;
+    HIP_TRY(n->ctx, hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t i = 0; i < BG_GRAPH_STEPS; i++) launch_stdp_step(dn, n->d_self, n->stream, nk, sk, i);
+    launch_step_bump(n->stream, dn.tbase, BG_GRAPH_STEPS);
+    const hipError_t e = hipStreamEndCapture(n->stream, &g);
+    if (e != hipSuccess) return set_err(n->ctx, BGHIP_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(&n->g_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+        n->g_exec = nullptr;
+        return set_err(n->ctx, BGHIP_EDEVICE, "graph instantiation failed: %s", hipGetErrorString(e2));
+    }
+    memcpy((void *)&n->g_dn, (const void *)&dn, sizeof(DevNet));
+    return BGHIP_OK;
+}
+
 extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
 {
     if (!n) return BGHIP_EINVAL;
@@ -623,12 +657,24 @@ extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
             HIP_TRY(n->ctx, hipMemcpy(n->d_self, &dn, sizeof(DevNet), hipMemcpyHostToDevice));
             memcpy((void *)&n->dn_shadow, (const void *)&dn, sizeof(DevNet));
         }
-        for (uint64_t i = 0; i < n_steps; i++) {
+        if (stdp) {
+            // steps run at *tbase + offset: whole runs of BG_GRAPH_STEPS replay one captured graph
+            // (its last node advances tbase), the rest is launched directly
+            launch_step_set(n->stream, dn.tbase, n->step);
+            uint64_t done = 0;
+            if (n_steps >= BG_GRAPH_STEPS) {
+                if (n->g_exec && memcmp(&dn, &n->g_dn, sizeof(DevNet)) != 0) drop_graph(n);
+                if (!n->g_exec) CHK(capture_stdp_graph(n));
+                for (; done + BG_GRAPH_STEPS <= n_steps; done += BG_GRAPH_STEPS)
+                    HIP_TRY(n->ctx, hipGraphLaunch(n->g_exec, n->stream));
+            }
+            for (uint64_t i = 0; done + i < n_steps; i++)
+                launch_stdp_step(dn, n->d_self, n->stream, nk, sk, (uint32_t)i);
+            n->timed_launches += 4 * n_steps;
+        }
+        for (uint64_t i = 0; i < n_steps && !stdp; i++) {
             uint64_t t = n->step + i;
-            if (stdp) {
-                launch_stdp_step(dn, n->d_self, n->stream, nk, sk, t);
-                n->timed_launches += 4;
-            } else {
+            {
                 launch_neuron_step(dn, n->stream, nk, t);
                 launch_synapse_step(dn, n->stream, sk, t);
                 n->timed_launches += 2;

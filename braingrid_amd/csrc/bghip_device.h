@@ -86,6 +86,7 @@ struct DevNet {
     EvLists ev;
     const uint32_t *out_begin, *out_count, *out_map;   // outgoing map (synapses of each source)
     const DevNet *self;          // this descriptor in device memory (for kernels that call out of line)
+    uint64_t *tbase;             // per-step STDP launches: step = *tbase + the launch's offset (graph replays)
 };
 
 enum { BG_INPUT_NONE = 0, BG_INPUT_POISSON = 1, BG_INPUT_REGULAR = 2 };

@@ -161,9 +161,21 @@ __device__ __forceinline__ void push_spike(const DevNet &n, int src, uint64_t t,
     }
 }
 
-template <int NK>
-__global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons(DevNet n, uint64_t t)
+// The single-network per-step kernels take the step as *n.tbase + off: a graph captured for S
+// steps (offsets 0..S-1, then k_step_bump) replays for every S steps of any advance call.
+static __global__ void k_step_set(uint64_t *tbase, uint64_t v)
 {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *tbase = v;
+}
+static __global__ void k_step_bump(uint64_t *tbase, uint32_t by)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *tbase += by;
+}
+
+template <int NK>
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons(DevNet n, uint32_t off)
+{
+    const uint64_t t = *n.tbase + off;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (BG_WAVE - 1);
     if (i < 2 * BG_EV_SHARDS) n.ev.count[i * BG_EV_PAD] = 0;   // launches have at least 256 threads
@@ -372,8 +384,9 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
 #define BG_STREAM_DESTS 4
 #endif
 template <int SK, int W32>
-__global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream(DevNet n, uint64_t t)
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream(DevNet n, uint32_t off)
 {
+    const uint64_t t = *n.tbase + off;
     const int lane = threadIdx.x & (BG_WAVE - 1);
     const int d0 = __builtin_amdgcn_readfirstlane(
         (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE) * BG_STREAM_DESTS);
@@ -408,9 +421,10 @@ __device__ __forceinline__ void stdp_event_syn_body(const DevNet &n, int d, uint
 // Grids of the event passes: BG_EV_SHARDS * G blocks; block b works on shard b mod BG_EV_SHARDS
 // together with the other G - 1 blocks of that shard.
 template <int SK>
-__global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn(const DevNet *np, uint64_t t)
+__global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn(const DevNet *np, uint32_t off)
 {
     const DevNet &n = *np;
+    const uint64_t t = *n.tbase + off;
     const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
     const uint32_t G = gridDim.x / BG_EV_SHARDS;
     const uint32_t c = n.ev.count[(BG_EV_SHARDS + s) * BG_EV_PAD];

@@ -13,7 +13,12 @@ void launch_neuron_step(const DevNet &dn, hipStream_t s, int nk, uint64_t t);
 #define BG_QUIET_STEPS 10000
 void launch_neuron_only(const DevNet &dn, hipStream_t s, int nk, uint64_t t0, int steps);
 void launch_synapse_step(const DevNet &dn, hipStream_t s, int sk, uint64_t t);
-void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int nk, int sk, uint64_t t);
+// one per-step STDP step at *dn.tbase + off (four launches)
+void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int nk, int sk, uint32_t off);
+void launch_step_set(hipStream_t s, uint64_t *tbase, uint64_t v);
+void launch_step_bump(hipStream_t s, uint64_t *tbase, uint32_t by);
+// steps per captured graph of the per-step STDP path
+#define BG_GRAPH_STEPS 256
 
 struct BatchNet;
 // Work items of a batch (destination waves or neuron threads) are numbered across the member

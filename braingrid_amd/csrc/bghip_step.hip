@@ -54,18 +54,18 @@ static unsigned event_grid(int N, unsigned gmax)
 }
 
 template <int NK, int SK, int W32>
-static void stdp_go(const DevNet &dn, const DevNet *d_self, hipStream_t s, uint64_t t)
+static void stdp_go(const DevNet &dn, const DevNet *d_self, hipStream_t s, uint32_t off)
 {
     const unsigned nb = (unsigned)((dn.N + BG_BLOCK - 1) / BG_BLOCK);
     const uint64_t waves = ((uint64_t)dn.N + BG_STREAM_DESTS - 1) / BG_STREAM_DESTS;
     const unsigned wb = (unsigned)((waves * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
-    hipLaunchKernelGGL(k_stdp_neurons<NK>, dim3(nb), dim3(BG_BLOCK), 0, s, dn, t);
-    hipLaunchKernelGGL((k_stdp_stream<SK, W32>), dim3(wb), dim3(BG_BLOCK), 0, s, dn, t);
-    hipLaunchKernelGGL(k_stdp_event_syn<SK>, dim3(event_grid(dn.N, 4)), dim3(BG_BLOCK), 0, s, d_self, t);
+    hipLaunchKernelGGL(k_stdp_neurons<NK>, dim3(nb), dim3(BG_BLOCK), 0, s, dn, off);
+    hipLaunchKernelGGL((k_stdp_stream<SK, W32>), dim3(wb), dim3(BG_BLOCK), 0, s, dn, off);
+    hipLaunchKernelGGL(k_stdp_event_syn<SK>, dim3(event_grid(dn.N, 4)), dim3(BG_BLOCK), 0, s, d_self, off);
     hipLaunchKernelGGL(k_stdp_event_sum, dim3(event_grid(dn.N, 8)), dim3(BG_BLOCK), 0, s, dn);
 }
 template <int NK, int SK>
-static void stdp_go_w(const DevNet &dn, const DevNet *d_self, hipStream_t s, uint64_t t)
+static void stdp_go_w(const DevNet &dn, const DevNet *d_self, hipStream_t s, uint32_t t)
 {
     switch (dn.W32) {
     case 1: stdp_go<NK, SK, 1>(dn, d_self, s, t); break;
@@ -75,7 +75,7 @@ static void stdp_go_w(const DevNet &dn, const DevNet *d_self, hipStream_t s, uin
     }
 }
 
-void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int nk, int sk, uint64_t t)
+void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int nk, int sk, uint32_t t)
 {
     if (nk == NK_LIF) {
         if (sk == SK_STDP) stdp_go_w<NK_LIF, SK_STDP>(dn, d_self, s, t);
@@ -84,4 +84,13 @@ void launch_stdp_step(const DevNet &dn, const DevNet *d_self, hipStream_t s, int
         if (sk == SK_STDP) stdp_go_w<NK_IZH, SK_STDP>(dn, d_self, s, t);
         else stdp_go_w<NK_IZH, SK_DYNSTDP>(dn, d_self, s, t);
     }
+}
+
+void launch_step_set(hipStream_t s, uint64_t *tbase, uint64_t v)
+{
+    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, s, tbase, v);
+}
+void launch_step_bump(hipStream_t s, uint64_t *tbase, uint32_t by)
+{
+    hipLaunchKernelGGL(k_step_bump, dim3(1), dim3(64), 0, s, tbase, by);
 }
