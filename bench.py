@@ -107,7 +107,8 @@ def batched_main(args):
     t_alone = t_alone_sample * len(specs) / len(sample)
     nets = SW.build(ctx, specs, log)
     S_total = sum(x.synapses for x in nets)
-    t_batched = SW.run_batched(ctx, nets, log)
+    shapes = []
+    t_batched = SW.run_batched(ctx, nets, log, tune_steps=args.tune_steps, shapes=shapes)
     fp_batched = {x.spec["id"]: x.fingerprint() for x in nets if x.spec["id"] in fp_alone}
     errs = sum(x.net.error_bits() != 0 for x in nets)
     for x in nets:
@@ -140,6 +141,8 @@ def batched_main(args):
         "speedup_vs_alone": t_alone / t_batched,
         "identical_results": f"{same}/{len(sample)} sampled networks bit-identical alone and batched",
         "networks_with_device_errors": int(errs),
+        "autotune_steps": args.tune_steps,
+        "launch_shapes": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in s.items()} for s in shapes],
     }
     ctx.close()
     print(json.dumps(out), flush=True)
@@ -154,7 +157,7 @@ def served_main(args):
     free, total = ctx.mem_info()
     budget = int(free * args.hbm_fraction)
     jobs = S.served_requests(args.requests, rate=args.rate, seed=args.seed)
-    eng = S.DeviceEngine(ctx)
+    eng = S.DeviceEngine(ctx, tune_steps=args.tune_steps)
     srv = S.Server(engine=eng, budget=budget, log=log)
     log(f"served: {len(jobs)} requests at {args.rate}/s, budget {budget / 2**30:.1f} GiB of {total / 2**30:.1f} GiB")
     srv.run(jobs, progress_every=20.0)
@@ -206,6 +209,8 @@ def main():
                     help="served form: share of the free HBM the resident networks may occupy")
     ap.add_argument("--alone-stride", type=int, default=4,
                     help="batched form: run every k-th network alone for the one-at-a-time comparison")
+    ap.add_argument("--tune-steps", type=int, default=8,
+                    help="batched/served forms: launch-shape autotuner, timed steps per candidate (0: off)")
     ap.add_argument("--steps", type=int, default=100000)
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--roofline-steps", type=int, default=20000)

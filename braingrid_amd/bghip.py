@@ -86,6 +86,15 @@ INPUT_STATE_FIELDS = {"nISIs": np.int32, "psr": np.float32, "r": np.float32, "u"
                       "lastSpike": np.uint64, "delayQueue": np.uint32, "delayIdx": np.int32}
 
 
+class BatchShape(ctypes.Structure):
+    """bghip_batch_shape: one batch group's kind and launch shape."""
+    _fields_ = [("mode", ctypes.c_int32), ("neuron_model", ctypes.c_int32), ("synapse_model", ctypes.c_int32),
+                ("ring_words", ctypes.c_int32), ("networks", ctypes.c_uint32), ("work_items", ctypes.c_uint32),
+                ("block", ctypes.c_int32), ("dests_per_wave", ctypes.c_int32), ("block_stream", ctypes.c_int32),
+                ("syn_blocks_per_shard", ctypes.c_int32), ("sum_blocks_per_shard", ctypes.c_int32),
+                ("tuned", ctypes.c_int32), ("step_us", ctypes.c_float)]
+
+
 class GrowthStats(ctypes.Structure):
     _fields_ = [("total_synapses", ctypes.c_int64), ("created", ctypes.c_int64), ("erased", ctypes.c_int64)]
 
@@ -159,6 +168,10 @@ def load():
         "bghip_batch_set": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.c_int]),
         "bghip_batch_advance": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "bghip_batch_sync": (ctypes.c_int, [vp]),
+        "bghip_batch_autotune": (ctypes.c_int, [vp, ctypes.c_uint32]),
+        "bghip_batch_group_count": (ctypes.c_int, [vp]),
+        "bghip_batch_get_shape": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(BatchShape)]),
+        "bghip_batch_set_shape": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(BatchShape)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -177,7 +190,8 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_batch_destroy", "bghip_batch_set", "bghip_batch_advance", "bghip_batch_sync",
             "bghip_upload_input", "bghip_download_input", "bghip_recorder_enable", "bghip_recorder_download",
             "bghip_net_save", "bghip_net_load", "bghip_net_get_desc", "bghip_recorder_info",
-            "bghip_add_synapses")
+            "bghip_add_synapses", "bghip_batch_autotune", "bghip_batch_group_count", "bghip_batch_get_shape",
+            "bghip_batch_set_shape")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -494,6 +508,30 @@ class Batch:
 
     def sync(self):
         self.ctx.check(load().bghip_batch_sync(self.h), "batch_sync")
+
+    def autotune(self, trial_steps: int):
+        """Turns the launch-shape autotuner on (trial_steps timed steps per candidate) or off (0)."""
+        self.ctx.check(load().bghip_batch_autotune(self.h, trial_steps), "batch_autotune")
+
+    def shapes(self) -> list[dict]:
+        """Launch shape of every group, as dicts of the bghip_batch_shape fields."""
+        n = load().bghip_batch_group_count(self.h)
+        if n < 0:
+            self.ctx.check(n, "batch_group_count")
+        out = []
+        for i in range(n):
+            s = BatchShape()
+            self.ctx.check(load().bghip_batch_get_shape(self.h, i, ctypes.byref(s)), "batch_get_shape")
+            out.append({f: getattr(s, f) for f, _ in BatchShape._fields_})
+        return out
+
+    def set_shape(self, group: int, **fields):
+        """Sets group `group`'s launch shape; fields not given keep their current values."""
+        s = BatchShape()
+        self.ctx.check(load().bghip_batch_get_shape(self.h, group, ctypes.byref(s)), "batch_get_shape")
+        for k, v in fields.items():
+            setattr(s, k, v)
+        self.ctx.check(load().bghip_batch_set_shape(self.h, group, ctypes.byref(s)), "batch_set_shape")
 
     def close(self):
         if self.h:

@@ -17,9 +17,18 @@
 #include "bghip_kernels_epoch.h"
 #include "bghip_launch.h"
 
+// A launch shape the batch autotuner picked, kept per device for groups of the same kind and
+// size class (work items within a factor of two), so re-formed groups do not repeat the trials.
+struct TuneEntry {
+    int mode, nk, sk, w32, size_class;
+    BatchShape shape;
+    float step_us;
+};
+
 struct bghip_ctx {
     int device;
     char err[512];
+    std::vector<TuneEntry> tuned;
 };
 
 struct bghip_net {
@@ -1352,8 +1361,13 @@ struct BatchGroup {
     BatchNet *d_nets = nullptr;    // device table (nn entries)
     uint32_t *d_base = nullptr;    // neuron prefix (nn + 1)
     uint16_t *d_nof = nullptr;     // member of each neuron / destination (total)
-    uint2 *d_chunks = nullptr;     // per-step STDP: runs of BG_STREAM_DESTS destinations
-    uint32_t nchunks = 0;
+    uint2 *d_chunks[3] = {};       // per-step STDP: runs of 1, 2 and 4 destinations per wavefront
+    uint32_t nchunks[3] = {};
+    BatchShape shape = {};         // launch shape in use
+    bool tuned = false;            // shape picked by the autotuner (or set by the caller)
+    float step_us = 0.0f;          // measured time per step of that shape
+    hipGraphExec_t g_exec = nullptr;   // per-step groups: BG_GRAPH_STEPS captured steps of this shape
+    BatchShape g_shape = {};
     uint64_t *d_sbase = nullptr;   // slot prefix (nn + 1)
     uint64_t *d_tprev = nullptr;   // flush parameters (nn)
     int *d_sel = nullptr;
@@ -1375,6 +1389,8 @@ struct bghip_batch {
     bghip_ctx *ctx;
     hipStream_t stream;
     hipEvent_t done;
+    hipEvent_t tev0, tev1;         // autotuner trial timing
+    uint32_t tune_steps = 0;       // autotuner: timed steps per trial (0: off)
     std::vector<bghip_net *> nets;
     std::vector<BatchGroup> groups;
     std::vector<int> modes;        // net_mode of each member when the groups were formed
@@ -1383,10 +1399,11 @@ struct bghip_batch {
 static void free_groups(bghip_batch *b)
 {
     for (auto &g : b->groups) {
+        if (g.g_exec) (void)hipGraphExecDestroy(g.g_exec);
         (void)hipFree(g.d_nets);
         (void)hipFree(g.d_base);
         (void)hipFree(g.d_nof);
-        (void)hipFree(g.d_chunks);
+        for (uint2 *c : g.d_chunks) (void)hipFree(c);
         (void)hipFree(g.d_sbase);
         (void)hipFree(g.d_tprev);
         (void)hipFree(g.d_sel);
@@ -1408,7 +1425,8 @@ extern "C" int bghip_batch_create(bghip_ctx *ctx, bghip_batch **out)
     if (!b) return BGHIP_ENOMEM;
     b->ctx = ctx;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&b->tev0) != hipSuccess || hipEventCreate(&b->tev1) != hipSuccess) {
         delete b;
         return set_err(ctx, BGHIP_EDEVICE, "batch stream/event creation failed");
     }
@@ -1422,9 +1440,41 @@ extern "C" void bghip_batch_destroy(bghip_batch *b)
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->stream);
     free_groups(b);
+    (void)hipEventDestroy(b->tev0);
+    (void)hipEventDestroy(b->tev1);
     (void)hipEventDestroy(b->done);
     (void)hipStreamDestroy(b->stream);
     delete b;
+}
+
+// The untuned launch shape: 256-thread blocks, four destinations per streaming wavefront, event
+// grids growing with the batch (up to 4 and 8 blocks per shard).
+static BatchShape default_shape(uint32_t total)
+{
+    const int g = (int)std::max<uint64_t>(1, ((uint64_t)total + 16 * BG_EV_SHARDS - 1) / (16 * BG_EV_SHARDS));
+    return BatchShape{BG_BLOCK, BG_STREAM_DESTS, BG_BLOCK, std::min(4, g), std::min(8, g)};
+}
+
+static int size_class(uint32_t total)
+{
+    int c = 0;
+    while (total > 1u) { total >>= 1; c++; }
+    return c;
+}
+
+static const TuneEntry *find_tuned(const bghip_ctx *ctx, const BatchGroup &g)
+{
+    for (const TuneEntry &e : ctx->tuned)
+        if (e.mode == g.mode && e.nk == g.nk && e.sk == g.sk && e.w32 == g.w32 && e.size_class == size_class(g.total))
+            return &e;
+    return nullptr;
+}
+
+static bool shape_valid(const BatchShape &s)
+{
+    const auto blk_ok = [](int v) { return v == 64 || v == 128 || v == 256; };
+    return blk_ok(s.blk) && blk_ok(s.blk_stream) && (s.dpw == 1 || s.dpw == 2 || s.dpw == 4) && s.g_syn >= 1 &&
+           s.g_syn <= 64 && s.g_sum >= 1 && s.g_sum <= 64;
 }
 
 // Groups the members by (neuron model, synapse model, ring words, mode); each group advances with
@@ -1503,16 +1553,156 @@ static int regroup(bghip_batch *b)
             g.ev.syn_off = g.d_ev + nc + BG_EV_SHARDS + 1;
             g.ev.dest = g.d_ev + nc + no;
             g.ev.syn = g.d_evs;
-            std::vector<uint2> chunks;
-            for (int k = 0; k < nn; k++)
-                for (uint32_t d = 0; d < base[k + 1] - base[k]; d += BG_STREAM_DESTS) chunks.push_back(make_uint2(k, d));
-            g.nchunks = (uint32_t)chunks.size();
-            HIP_TRY(b->ctx, hipMalloc((void **)&g.d_chunks, sizeof(uint2) * std::max<size_t>(chunks.size(), 1)));
-            HIP_TRY(b->ctx, hipMemcpy(g.d_chunks, chunks.data(), sizeof(uint2) * chunks.size(), hipMemcpyHostToDevice));
+            // runs of D = 1, 2, 4 destinations; a run's shard is that of its BG_STREAM_DESTS-run
+            // (parts[k].cbase + d / BG_STREAM_DESTS), the assignment the capacities were sized for
+            for (int v = 0; v < 3; v++) {
+                const uint32_t D = 1u << v;
+                std::vector<uint2> chunks;
+                for (int k = 0; k < nn; k++)
+                    for (uint32_t d = 0; d < base[k + 1] - base[k]; d += D) {
+                        const uint32_t shard = (uint32_t)((parts[k].cbase + d / BG_STREAM_DESTS) % BG_EV_SHARDS);
+                        chunks.push_back(make_uint2((uint32_t)k | (shard << 16), d));
+                    }
+                g.nchunks[v] = (uint32_t)chunks.size();
+                HIP_TRY(b->ctx, hipMalloc((void **)&g.d_chunks[v], sizeof(uint2) * std::max<size_t>(chunks.size(), 1)));
+                HIP_TRY(b->ctx, hipMemcpy(g.d_chunks[v], chunks.data(), sizeof(uint2) * chunks.size(),
+                                          hipMemcpyHostToDevice));
+            }
+        }
+        g.shape = default_shape(g.total);
+        if (const TuneEntry *te = find_tuned(b->ctx, g)) {
+            g.shape = te->shape;
+            g.step_us = te->step_us;
+            g.tuned = true;
         }
         HIP_TRY(b->ctx, hipMemcpy(g.d_base, base.data(), sizeof(uint32_t) * (nn + 1), hipMemcpyHostToDevice));
         HIP_TRY(b->ctx, hipMemcpy(g.d_sbase, sbase.data(), sizeof(uint64_t) * (nn + 1), hipMemcpyHostToDevice));
     }
+    return BGHIP_OK;
+}
+
+static BatchMap batch_map(const BatchGroup &g, const BatchShape &sh)
+{
+    const int v = sh.dpw == 1 ? 0 : (sh.dpw == 2 ? 1 : 2);
+    return BatchMap{g.d_nets, g.d_base, g.d_nof, g.total, (int)g.members.size(), g.d_chunks[v], g.nchunks[v]};
+}
+
+// ---- batch autotuner -----------------------------------------------------------------------
+// One axis of the search: a field of the shape and the values tried for it.
+struct ShapeAxis {
+    int BatchShape::*field;
+    std::vector<int> values;
+};
+
+static std::vector<ShapeAxis> quiet_space() { return {{&BatchShape::blk, {256, 128, 64}}}; }
+static std::vector<ShapeAxis> fused_space() { return {{&BatchShape::blk, {256, 128, 64}}}; }
+static std::vector<ShapeAxis> step_space(const BatchGroup &g)
+{
+    if (g.sk != BGHIP_SYN_STDP && g.sk != BGHIP_SYN_DYNSTDP) return {{&BatchShape::blk, {256, 128, 64}}};
+    return {{&BatchShape::dpw, {4, 2, 1}},
+            {&BatchShape::blk_stream, {256, 128, 64}},
+            {&BatchShape::g_syn, {1, 2, 4, 8, 16}},
+            {&BatchShape::g_sum, {1, 2, 4, 8, 16, 32}},
+            {&BatchShape::blk, {256, 128, 64}}};
+}
+static int space_size(const std::vector<ShapeAxis> &sp)
+{
+    int n = 0;
+    for (const ShapeAxis &a : sp) n += (int)a.values.size();
+    return n;
+}
+
+// Runs `work` (which enqueues `steps` steps) between two events on the batch stream and returns
+// the time per step in microseconds.
+template <class F>
+static int time_steps(bghip_batch *b, float &us, uint64_t steps, F &&work)
+{
+    HIP_TRY(b->ctx, hipEventRecord(b->tev0, b->stream));
+    work();
+    HIP_TRY(b->ctx, hipGetLastError());
+    HIP_TRY(b->ctx, hipEventRecord(b->tev1, b->stream));
+    HIP_TRY(b->ctx, hipEventSynchronize(b->tev1));
+    float ms = 0.0f;
+    HIP_TRY(b->ctx, hipEventElapsedTime(&ms, b->tev0, b->tev1));
+    us = ms * 1e3f / (float)std::max<uint64_t>(1, steps);
+    return BGHIP_OK;
+}
+
+// Coordinate descent over the axes, starting from the group's current shape: every candidate value
+// of an axis is tried with the other fields fixed at the best found so far, and kept if faster.
+// Trials run real steps of the member networks; every shape gives bit-identical results, so the
+// search changes only the time the steps take. The result is remembered per device for groups of
+// the same kind and size class.
+template <class Trial>
+static int tune_group(bghip_batch *b, BatchGroup &g, const std::vector<ShapeAxis> &space, Trial &&trial)
+{
+    BatchShape cur = g.shape;
+    float best = 0.0f;
+    CHK(trial(cur, best));
+    for (const ShapeAxis &a : space)
+        for (int v : a.values) {
+            if (cur.*(a.field) == v) continue;
+            BatchShape c = cur;
+            c.*(a.field) = v;
+            float us = 0.0f;
+            CHK(trial(c, us));
+            if (us < best) {
+                best = us;
+                cur = c;
+            }
+        }
+    g.shape = cur;
+    g.step_us = best;
+    g.tuned = true;
+    const TuneEntry e = {g.mode, g.nk, g.sk, g.w32, size_class(g.total), cur, best};
+    auto &tab = b->ctx->tuned;
+    auto it = std::find_if(tab.begin(), tab.end(), [&](const TuneEntry &x) {
+        return x.mode == e.mode && x.nk == e.nk && x.sk == e.sk && x.w32 == e.w32 && x.size_class == e.size_class;
+    });
+    if (it != tab.end()) *it = e;
+    else tab.push_back(e);
+    return BGHIP_OK;
+}
+
+// Captures BG_GRAPH_STEPS steps of a per-step group (offsets 0..S-1 from each member's table step)
+// and the table bump as one executable graph, replayed for every S steps of later calls.
+static int capture_batch_graph(bghip_batch *b, BatchGroup &g)
+{
+    const BatchMap bm = batch_map(g, g.shape);
+    hipGraph_t gr = nullptr;
+    HIP_TRY(b->ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t i = 0; i < BG_GRAPH_STEPS; i++) launch_step_batch(bm, b->stream, g.shape, g.nk, g.sk, g.w32, i, g.ev);
+    launch_batch_bump(b->stream, g.d_nets, (int)g.members.size(), BG_GRAPH_STEPS);
+    const hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    if (e != hipSuccess) return set_err(b->ctx, BGHIP_EDEVICE, "batch graph capture failed: %s", hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(&g.g_exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e2 != hipSuccess) {
+        g.g_exec = nullptr;
+        return set_err(b->ctx, BGHIP_EDEVICE, "batch graph instantiation failed: %s", hipGetErrorString(e2));
+    }
+    g.g_shape = g.shape;
+    return BGHIP_OK;
+}
+
+// Steps s .. n_steps-1 of this call for a per-step group: whole runs of BG_GRAPH_STEPS replay the
+// captured graph (after one bump that moves the table's steps past the s steps already run), the
+// rest is launched directly.
+static int run_step_group(bghip_batch *b, BatchGroup &g, uint64_t s, uint64_t n_steps)
+{
+    const BatchMap bm = batch_map(g, g.shape);
+    uint64_t off = s;   // offset of step s from the steps in the device table
+    if (n_steps - s >= BG_GRAPH_STEPS) {
+        if (g.g_exec && memcmp(&g.g_shape, &g.shape, sizeof(BatchShape)) != 0) {
+            (void)hipGraphExecDestroy(g.g_exec);
+            g.g_exec = nullptr;
+        }
+        if (!g.g_exec) CHK(capture_batch_graph(b, g));
+        if (off) launch_batch_bump(b->stream, g.d_nets, (int)g.members.size(), (uint32_t)off);
+        off = 0;
+        for (; s + BG_GRAPH_STEPS <= n_steps; s += BG_GRAPH_STEPS) HIP_TRY(b->ctx, hipGraphLaunch(g.g_exec, b->stream));
+    }
+    for (; s < n_steps; s++, off++) launch_step_batch(bm, b->stream, g.shape, g.nk, g.sk, g.w32, off, g.ev);
     return BGHIP_OK;
 }
 
@@ -1562,11 +1752,26 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
             tab[k].pull = x->pending ? 1 : 0;
         }
         HIP_TRY(b->ctx, hipMemcpyAsync(g.d_nets, tab, sizeof(BatchNet) * nn, hipMemcpyHostToDevice, b->stream));
-        const BatchMap bm = {g.d_nets, g.d_base, g.d_nof, g.total, nn, g.d_chunks, g.nchunks};
+        const bool tune = b->tune_steps > 0 && !g.tuned;
+        const uint64_t T = b->tune_steps;
         if (g.mode == BG_MODE_QUIET) {
-            for (uint64_t done = 0; done < n_steps;) {
+            uint64_t done = 0;
+            if (tune && n_steps >= 4 * 11 * T) {
+                // trial: a launch of T steps (first launch of the shape), then a timed one of 10 T
+                CHK(tune_group(b, g, quiet_space(), [&](const BatchShape &sh, float &us) {
+                    const BatchMap m = batch_map(g, sh);
+                    launch_neuron_only_batch(m, b->stream, sh, g.nk, done, (int)T);
+                    done += T;
+                    return time_steps(b, us, 10 * T, [&] {
+                        launch_neuron_only_batch(m, b->stream, sh, g.nk, done, (int)(10 * T));
+                        done += 10 * T;
+                    });
+                }));
+            }
+            const BatchMap bm = batch_map(g, g.shape);
+            while (done < n_steps) {
                 const int k = (int)std::min<uint64_t>(BG_QUIET_STEPS, n_steps - done);
-                launch_neuron_only_batch(bm, b->stream, g.nk, done, k);
+                launch_neuron_only_batch(bm, b->stream, g.shape, g.nk, done, k);
                 done += (uint64_t)k;
             }
             for (int k = 0; k < nn; k++) {
@@ -1575,8 +1780,19 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
                 x->pending = false;
             }
         } else if (g.mode == BG_MODE_STEP) {
-            for (uint64_t s = 0; s < n_steps; s++)
-                launch_step_batch(bm, b->stream, g.nk, g.sk, g.w32, s, g.ev);
+            uint64_t s = 0;
+            const std::vector<ShapeAxis> space = step_space(g);
+            if (tune && n_steps >= (uint64_t)(space_size(space) + 1) * (T + 1)) {
+                // trial: one step (first launch of the shape), then T timed steps
+                CHK(tune_group(b, g, space, [&](const BatchShape &sh, float &us) {
+                    const BatchMap m = batch_map(g, sh);
+                    launch_step_batch(m, b->stream, sh, g.nk, g.sk, g.w32, s++, g.ev);
+                    return time_steps(b, us, T, [&] {
+                        for (uint64_t i = 0; i < T; i++) launch_step_batch(m, b->stream, sh, g.nk, g.sk, g.w32, s++, g.ev);
+                    });
+                }));
+            }
+            CHK(run_step_group(b, g, s, n_steps));
             for (int k = 0; k < nn; k++) {
                 bghip_net *x = b->nets[g.members[k]];
                 x->step += n_steps;
@@ -1585,9 +1801,23 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
         } else {
             const int Kfull = std::max(1, g.kmax);
             const uint64_t L = (n_steps + (uint64_t)Kfull - 1) / (uint64_t)Kfull;
-            for (uint64_t l = 0; l < L; l++) {
+            uint64_t l = 0;
+            const uint64_t TL = std::max<uint64_t>(4, (T + (uint64_t)Kfull - 1) / (uint64_t)Kfull);
+            if (tune && L > 4 * (TL + 1)) {
+                // trial: one launch (first of the shape), then TL timed launches of Kfull steps
+                CHK(tune_group(b, g, fused_space(), [&](const BatchShape &sh, float &us) {
+                    const BatchMap m = batch_map(g, sh);
+                    launch_fused_batch(m, b->stream, sh, g.nk, g.sk, g.w32, g.kmax, Kfull, Kfull, (int)l++);
+                    return time_steps(b, us, TL * (uint64_t)Kfull, [&] {
+                        for (uint64_t i = 0; i < TL; i++)
+                            launch_fused_batch(m, b->stream, sh, g.nk, g.sk, g.w32, g.kmax, Kfull, Kfull, (int)l++);
+                    });
+                }));
+            }
+            const BatchMap bm = batch_map(g, g.shape);
+            for (; l < L; l++) {
                 const int K = (int)std::min<uint64_t>((uint64_t)Kfull, n_steps - l * (uint64_t)Kfull);
-                launch_fused_batch(bm, b->stream, g.nk, g.sk, g.w32, g.kmax, K, Kfull, (int)l);
+                launch_fused_batch(bm, b->stream, g.shape, g.nk, g.sk, g.w32, g.kmax, K, Kfull, (int)l);
             }
             uint64_t *tprev = g.h_tprev;
             int *sel = g.h_sel;
@@ -1626,6 +1856,58 @@ extern "C" int bghip_batch_sync(bghip_batch *b)
     if (!b) return BGHIP_EINVAL;
     HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
     HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_batch_autotune(bghip_batch *b, uint32_t trial_steps)
+{
+    if (!b || trial_steps > 1000) return BGHIP_EINVAL;
+    b->tune_steps = trial_steps;
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_batch_group_count(bghip_batch *b)
+{
+    if (!b) return BGHIP_EINVAL;
+    // a growth update or an upload since the last call may have moved a member to another mode
+    for (size_t i = 0; i < b->nets.size(); i++)
+        if (net_mode(b->nets[i]) != b->modes[i]) {
+            CHK(regroup(b));
+            break;
+        }
+    return (int)b->groups.size();
+}
+
+extern "C" int bghip_batch_get_shape(bghip_batch *b, int group, bghip_batch_shape *out)
+{
+    if (!b || !out || group < 0 || group >= (int)b->groups.size()) return BGHIP_EINVAL;
+    const BatchGroup &g = b->groups[group];
+    out->mode = g.mode;
+    out->neuron_model = g.nk;
+    out->synapse_model = g.sk;
+    out->ring_words = g.w32;
+    out->networks = (uint32_t)g.members.size();
+    out->work_items = g.total;
+    out->block = g.shape.blk;
+    out->dests_per_wave = g.shape.dpw;
+    out->block_stream = g.shape.blk_stream;
+    out->syn_blocks_per_shard = g.shape.g_syn;
+    out->sum_blocks_per_shard = g.shape.g_sum;
+    out->tuned = g.tuned ? 1 : 0;
+    out->step_us = g.step_us;
+    return BGHIP_OK;
+}
+
+extern "C" int bghip_batch_set_shape(bghip_batch *b, int group, const bghip_batch_shape *in)
+{
+    if (!b || !in || group < 0 || group >= (int)b->groups.size()) return BGHIP_EINVAL;
+    const BatchShape sh = {in->block, in->dests_per_wave, in->block_stream, in->syn_blocks_per_shard,
+                           in->sum_blocks_per_shard};
+    if (!shape_valid(sh)) return set_err(b->ctx, BGHIP_EINVAL, "invalid batch launch shape");
+    BatchGroup &g = b->groups[group];
+    g.shape = sh;
+    g.tuned = true;
+    g.step_us = 0.0f;
     return BGHIP_OK;
 }
 

@@ -878,7 +878,7 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons_batch(BatchMap bm, ui
 {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (BG_WAVE - 1);
-    if (w < 2 * BG_EV_SHARDS) ev.count[w * BG_EV_PAD] = 0;   // launches have at least 256 threads
+    if (w < 2 * BG_EV_SHARDS) ev.count[w * BG_EV_PAD] = 0;   // launches have at least 2 * BG_EV_SHARDS threads
     bool fired = false;
     int ni = 0, li = 0;
     if (w < bm.total) {
@@ -896,19 +896,21 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons_batch(BatchMap bm, ui
     }
 }
 
-// chunks[c] = {member, first destination} of the c-th run of BG_STREAM_DESTS destinations (runs
-// never cross networks)
-template <int SK, int W32>
+// chunks[c] = {member | shard << 16, first destination} of the c-th run of D destinations (runs
+// never cross networks). Shards are assigned per run of BG_STREAM_DESTS destinations whatever D is
+// (D divides BG_STREAM_DESTS), so the event-list capacities do not depend on D.
+template <int SK, int W32, int D>
 __global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream_batch(BatchMap bm, uint64_t step, EvLists ev)
 {
     const uint32_t c = __builtin_amdgcn_readfirstlane(
         (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
     if (c >= bm.nchunks) return;
     const uint2 e = bm.chunks[c];
-    const int i = __builtin_amdgcn_readfirstlane((int)e.x), d0 = __builtin_amdgcn_readfirstlane((int)e.y);
+    const int i = __builtin_amdgcn_readfirstlane((int)(e.x & 0xFFFFu)), d0 = __builtin_amdgcn_readfirstlane((int)e.y);
+    const uint32_t shard = __builtin_amdgcn_readfirstlane(e.x >> 16);
     const BatchNet &b = bm.nets[i];
-    stdp_stream_dests<SK, W32, BG_STREAM_DESTS>(b.n, d0, min(BG_STREAM_DESTS, b.n.N - d0), threadIdx.x & (BG_WAVE - 1),
-                                                b.t0 + step, bm.base[i] + (uint32_t)d0, c % BG_EV_SHARDS, ev);
+    stdp_stream_dests<SK, W32, D>(b.n, d0, min(D, b.n.N - d0), threadIdx.x & (BG_WAVE - 1), b.t0 + step,
+                                  bm.base[i] + (uint32_t)d0, shard, ev);
 }
 
 template <int SK>
@@ -918,7 +920,7 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn_batch(BatchMap bm, 
     const uint32_t G = gridDim.x / BG_EV_SHARDS;
     const uint32_t c = ev.count[(BG_EV_SHARDS + s) * BG_EV_PAD];
     const uint2 *list = ev.syn + ev.syn_off[s];
-    for (uint32_t k = g * BG_BLOCK + threadIdx.x; k < c; k += G * BG_BLOCK) {
+    for (uint32_t k = g * blockDim.x + threadIdx.x; k < c; k += G * blockDim.x) {
         const uint2 e = list[k];
         const int i = bm.nof[e.x];
         const BatchNet &b = bm.nets[i];
@@ -930,7 +932,7 @@ static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum_batch(BatchM
 {
     const int lane = threadIdx.x & (BG_WAVE - 1);
     const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
-    const uint32_t G = gridDim.x / BG_EV_SHARDS, wpb = BG_BLOCK / BG_WAVE;
+    const uint32_t G = gridDim.x / BG_EV_SHARDS, wpb = blockDim.x / BG_WAVE;
     const uint32_t c = ev.count[s * BG_EV_PAD];
     const uint32_t *list = ev.dest + ev.dest_off[s];
     for (uint32_t k = g * wpb + threadIdx.x / BG_WAVE; k < c; k += G * wpb) {
@@ -938,6 +940,14 @@ static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum_batch(BatchM
         const int i = __builtin_amdgcn_readfirstlane((int)bm.nof[w]);
         stdp_sum_body(bm.nets[i].n, (int)(w - bm.base[i]), lane);
     }
+}
+
+// Advances every member's step base by `by`: the last node of a captured run of batch steps, so
+// the next replay (or the launches after it) continue from there.
+static __global__ void k_batch_bump(BatchNet *nets, int nn, uint32_t by)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) nets[i].t0 += by;
 }
 
 template <int NK>

@@ -30,10 +30,25 @@ struct BatchMap {
     const uint16_t *nof;    // total entries
     uint32_t total;
     int nn;
-    const uint2 *chunks;    // per-step STDP streaming pass: {member, first destination} runs
+    const uint2 *chunks;    // per-step STDP streaming pass: {member | shard << 16, first destination} runs
     uint32_t nchunks;
 };
-void launch_fused_batch(const BatchMap &bm, hipStream_t s, int nk, int sk, int w32, int kmax, int K, int Kfull,
-                        int launch);
-void launch_neuron_only_batch(const BatchMap &bm, hipStream_t s, int nk, uint64_t step, int steps);
-void launch_step_batch(const BatchMap &bm, hipStream_t s, int nk, int sk, int w32, uint64_t step, const EvLists &ev);
+// Launch shape of a batch group's kernels. Every work item computes the same values whatever
+// block it lands in and however many destinations share its wavefront, so results do not depend
+// on the shape; only the time does.
+struct BatchShape {
+    int blk;          // threads per block: neuron, synapse, fused and neuron-only kernels (64/128/256)
+    int dpw;          // per-step STDP streaming pass: destinations per wavefront (1, 2 or 4)
+    int blk_stream;   // threads per block of the streaming pass (64/128/256)
+    int g_syn;        // blocks per event-list shard of the learning pass
+    int g_sum;        // blocks per event-list shard of the event summation pass
+};
+void launch_fused_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, int sk, int w32, int kmax,
+                        int K, int Kfull, int launch);
+void launch_neuron_only_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, uint64_t step,
+                              int steps);
+void launch_step_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, int sk, int w32,
+                       uint64_t step, const EvLists &ev);
+void launch_stdp_step_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, int sk, int w32,
+                            uint64_t step, const EvLists &ev);
+void launch_batch_bump(hipStream_t s, BatchNet *nets, int nn, uint32_t by);

@@ -81,10 +81,11 @@ class DeviceEngine:
     histograms, final radii); with fingerprint=True it also carries a digest of the whole final
     network state (a full download, for comparisons in tests)."""
 
-    def __init__(self, ctx, fingerprint: bool = False):
+    def __init__(self, ctx, fingerprint: bool = False, tune_steps: int = SW.TUNE_STEPS):
         from . import bghip as B
         self.ctx = ctx
         self.batch = B.Batch(ctx)
+        self.batch.autotune(tune_steps)
         self._members = None
         self.fingerprint = fingerprint
 

@@ -36,6 +36,8 @@ LIF_M = 200
 LIF_MAX_RATE = 200
 IZH_MAX_RATE = 400
 POISSON_HZ, POISSON_WEIGHT = 10.0, 25.0
+# batch launch-shape autotuner: timed steps per candidate shape (bghip_batch_autotune)
+TUNE_STEPS = 8
 
 
 def net_seed(global_seed: int, net_id: int) -> int:
@@ -216,11 +218,14 @@ def run_alone(nets, log=None) -> float:
     return time.perf_counter() - t0
 
 
-def run_batched(ctx: B.Context, nets, log=None) -> float:
-    """All networks advanced by shared launches, epoch by epoch. Returns seconds."""
+def run_batched(ctx: B.Context, nets, log=None, tune_steps: int = TUNE_STEPS, shapes: list | None = None) -> float:
+    """All networks advanced by shared launches, epoch by epoch, with the batch's launch-shape
+    autotuner at `tune_steps` timed steps per candidate (0: fixed default shapes). Returns seconds;
+    the groups' launch shapes at the end are appended to `shapes` when given."""
     for x in nets:
         x.net.sync()
     batch = B.Batch(ctx)
+    batch.autotune(tune_steps)
     t0 = time.perf_counter()
     live = [x for x in nets if x.epochs > 0]
     epoch = 0
@@ -241,5 +246,7 @@ def run_batched(ctx: B.Context, nets, log=None) -> float:
     for x in nets:
         x.net.sync()
     wall = time.perf_counter() - t0
+    if shapes is not None:
+        shapes.extend(batch.shapes())
     batch.close()
     return wall

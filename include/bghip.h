@@ -266,6 +266,36 @@ int bghip_batch_set(bghip_batch *b, bghip_net *const *nets, int n);
 int bghip_batch_advance(bghip_batch *b, uint64_t n_steps);
 int bghip_batch_sync(bghip_batch *b);
 
+/* Launch shapes. The members are advanced in groups (one per neuron model, synapse model, ring
+ * width and mode); each group's kernels run with a launch shape. Per-step groups replay a captured
+ * HIP graph of 256 steps for every whole 256 steps of an advance call. Results are bit-identical
+ * for every valid shape. The engine's launch-shape autotuner is also without a reference
+ * counterpart (the reference fixes its launch shapes at 256 threads per block, e.g.
+ * Neurons/AllLIFNeurons_d.cu:25, Synapses/AllSpikingSynapses_d.cu:337, Core/GPUSpikingModel.cu:235). */
+typedef struct bghip_batch_shape {
+    int32_t mode;                  /* out: 0 neurons only (no synapses), 1 fused blocks, 2 per step */
+    int32_t neuron_model, synapse_model, ring_words;   /* out: the group's kind */
+    uint32_t networks, work_items; /* out: members, and neurons (= destinations) over all members */
+    int32_t block;                 /* threads per block of the neuron, synapse and fused kernels: 64, 128, 256 */
+    int32_t dests_per_wave;        /* per-step STDP streaming pass: destinations per wavefront, 1, 2 or 4 */
+    int32_t block_stream;          /* threads per block of that pass: 64, 128, 256 */
+    int32_t syn_blocks_per_shard;  /* blocks per event-list shard of the learning pass, 1..64 */
+    int32_t sum_blocks_per_shard;  /* blocks per event-list shard of the event summation pass, 1..64 */
+    int32_t tuned;                 /* out: 1 if the shape was picked by the autotuner or set */
+    float step_us;                 /* out: the autotuner's measured time per step of the shape (0 if unmeasured) */
+} bghip_batch_shape;
+/* Autotuner: with trial_steps > 0 (at most 1000), the first advance call of a group that is long
+ * enough times its candidate shapes on the group's own next steps (trial_steps timed steps each,
+ * coordinate descent over the fields above) and keeps the fastest; the choice is remembered per
+ * device for groups of the same kind and size class (work items within a factor of two). Trials
+ * synchronise the host with the batch stream. 0 turns the tuner off (the default). */
+int bghip_batch_autotune(bghip_batch *b, uint32_t trial_steps);
+/* Number of groups (re-forms the groups first if a member changed mode). */
+int bghip_batch_group_count(bghip_batch *b);
+int bghip_batch_get_shape(bghip_batch *b, int group, bghip_batch_shape *out);
+/* Sets group `group`'s shape (input fields only) until the groups are re-formed; marks it tuned. */
+int bghip_batch_set_shape(bghip_batch *b, int group, const bghip_batch_shape *shape);
+
 /* ---- timing (measurement support) -------------------------------------------------------- */
 /* Average duration in milliseconds of the engine's fused step launches recorded with HIP events
  * on the network's stream since the last reset; count receives the number of launches timed. */

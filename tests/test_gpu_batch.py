@@ -46,6 +46,112 @@ def test_batch_results_equal_alone(hip_ctx):
     assert not bad, bad[:10]
 
 
+SHAPES = [
+    dict(block=64, dests_per_wave=1, block_stream=64, syn_blocks_per_shard=1, sum_blocks_per_shard=1),
+    dict(block=128, dests_per_wave=2, block_stream=128, syn_blocks_per_shard=16, sum_blocks_per_shard=32),
+    dict(block=256, dests_per_wave=4, block_stream=64, syn_blocks_per_shard=3, sum_blocks_per_shard=5),
+]
+
+
+def _grown_mix(hip_ctx):
+    """The mixed networks after one growth update (so the fused and per-step groups exist)."""
+    cfgs = [H.load_config(n) for n in MIX]
+    oras = [O.OracleNet(c) for c in cfgs]
+    nets = [H.gpu_net_from_oracle(hip_ctx, o, c) for o, c in zip(oras, cfgs)]
+    for o, n in zip(oras, nets):
+        o.advance(400)
+        n.advance(400)
+        o.update_connections()
+        n.update_connections()
+    return cfgs, oras, nets
+
+
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
+def test_batch_launch_shapes_give_identical_results(hip_ctx, shape):
+    """Every launch shape (block sizes, destinations per streaming wavefront, event-pass grids)
+    gives the oracle's results bit for bit, through direct launches and captured graph replays."""
+    print(f"[batch] launch shape {SHAPES[shape]}", flush=True)
+    cfgs, oras, nets = _grown_mix(hip_ctx)
+    batch = B.Batch(hip_ctx)
+    batch.set(nets)
+    groups = batch.shapes()
+    assert {g["mode"] for g in groups} >= {1, 2}, groups
+    for i in range(len(groups)):
+        batch.set_shape(i, **SHAPES[shape])
+    bad = []
+    for rnd, chunk in enumerate((300, 611)):   # 300: one graph replay + 44 direct steps
+        for o in oras:
+            o.advance(chunk)
+        batch.advance(chunk)
+        for k, (o, n, c) in enumerate(zip(oras, nets, cfgs)):
+            n.sync()
+            bad += H.compare_state(o, n, c, f"{MIX[k]}-shape{shape}-round{rnd}", check_maps=False)
+    for g in batch.shapes():
+        assert g["block"] == SHAPES[shape]["block"], g
+    batch.close()
+    for n in nets:
+        n.close()
+    assert not bad, bad[:10]
+
+
+def test_batch_shape_is_validated(hip_ctx):
+    cfgs, oras, nets = _grown_mix(hip_ctx)
+    batch = B.Batch(hip_ctx)
+    batch.set(nets[:2])
+    for bad in (dict(block=96), dict(dests_per_wave=3), dict(block_stream=512), dict(syn_blocks_per_shard=0),
+                dict(sum_blocks_per_shard=65)):
+        with pytest.raises(B.BgHipError):
+            batch.set_shape(0, **bad)
+    with pytest.raises(B.BgHipError):
+        batch.set_shape(len(batch.shapes()), block=64)
+    batch.close()
+    for n in nets:
+        n.close()
+
+
+def test_autotuner_tunes_every_group_and_keeps_results(hip_ctx):
+    """The autotuner's trials run on the members' own steps: afterwards every group has a measured
+    shape, the results are still the oracle's, and a new batch of the same kinds and sizes starts
+    from the remembered shapes."""
+    print("[batch] autotuner", flush=True)
+    cfgs = [H.load_config(n) for n in MIX]
+    oras = [O.OracleNet(c) for c in cfgs]
+    nets = [H.gpu_net_from_oracle(hip_ctx, o, c) for o, c in zip(oras, cfgs)]
+    batch = B.Batch(hip_ctx)
+    batch.autotune(4)
+    batch.set(nets)
+    bad = []
+    modes = set()
+    for rnd, chunk in enumerate((1000, 1500)):   # round 0: no synapses yet (neuron-only groups)
+        for o in oras:
+            o.advance(chunk)
+        batch.advance(chunk)
+        for k, (o, n, c) in enumerate(zip(oras, nets, cfgs)):
+            n.sync()
+            bad += H.compare_state(o, n, c, f"{MIX[k]}-tuned-round{rnd}", check_maps=False)
+        for g in batch.shapes():
+            modes.add(g["mode"])
+            assert g["tuned"] == 1 and g["step_us"] > 0, g
+        for o, n in zip(oras, nets):
+            o.update_connections()
+            n.update_connections()
+    assert modes == {0, 1, 2}, modes
+    remembered = {(g["mode"], g["neuron_model"], g["synapse_model"], g["ring_words"]): g for g in batch.shapes()}
+    batch.close()
+    again = B.Batch(hip_ctx)
+    again.set(nets)
+    for g in again.shapes():
+        key = (g["mode"], g["neuron_model"], g["synapse_model"], g["ring_words"])
+        if key in remembered and remembered[key]["work_items"] == g["work_items"]:
+            assert g["tuned"] == 1, g
+            for f in ("block", "dests_per_wave", "block_stream", "syn_blocks_per_shard", "sum_blocks_per_shard"):
+                assert g[f] == remembered[key][f], (f, g, remembered[key])
+    again.close()
+    for n in nets:
+        n.close()
+    assert not bad, bad[:10]
+
+
 def test_batch_membership_changes(hip_ctx):
     cfgs = [H.load_config(n) for n in ("conn-lif-ds", "conn-izh-stdp", "conn-lif-ds-dense")]
     oras = [O.OracleNet(c) for c in cfgs]
