@@ -15,6 +15,10 @@ wall time. `roofline` prices the dominant kernel (the fused block step kernel) w
 algorithmic bytes (24 B per DS synapse-step + 44 B per LIF neuron-step) over its average launch
 duration, measured with HIP events on the network's stream. `cpu_baseline` times the CPU oracle
 (the reference's single-threaded algorithm restated in C) on a bounded sample of the same network.
+`batched` summarises the batched form (the 256-network sweep with shared launches against a
+sample of its networks run alone: simulated seconds per wall hour both ways) and `served` the
+served form (the first 200 requests of the open-loop job stream: jobs/s, p50/p99 completion
+latency); `--form batched` / `--form served` run either form alone at full size.
 """
 from __future__ import annotations
 
@@ -85,6 +89,14 @@ def log(msg):
 
 
 def batched_main(args):
+    print(json.dumps(batched_result(args)), flush=True)
+
+
+def served_main(args):
+    print(json.dumps(served_result(args)), flush=True)
+
+
+def batched_result(args):
     """The batched form: a sweep of independent networks run one at a time with their own
     launches, then all together with shared launches; same initial states, same results."""
     from braingrid_amd import bghip as B
@@ -145,10 +157,10 @@ def batched_main(args):
         "launch_shapes": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in s.items()} for s in shapes],
     }
     ctx.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def served_main(args):
+def served_result(args):
     """The served form: an open-loop Poisson stream of simulation jobs, continuously batched."""
     from braingrid_amd import bghip as B
     from braingrid_amd import serve as S
@@ -191,7 +203,39 @@ def served_main(args):
         "counts": st["counts"],
         "phase_s": st["host_phase_s"],
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def forms_summary(args):
+    """The batched and served forms at reduced sample sizes, summarised for the headline line: the
+    full 256-network sweep batched against every `forms_alone_stride`-th network alone, and the
+    first `forms_requests` requests of the served stream."""
+    out = {}
+    a = argparse.Namespace(**vars(args))
+    a.alone_stride = args.forms_alone_stride
+    try:
+        r = batched_result(a)
+        out["batched"] = {k: r[k] for k in ("value", "unit", "alone_value", "speedup_vs_alone", "batched_wall_s",
+                                            "alone_wall_s", "alone_sample", "identical_results",
+                                            "networks_with_device_errors")}
+        out["batched"]["workload"] = r["config"]["workload"]
+        out["batched"]["launch_shapes"] = [{k: s[k] for k in ("mode", "networks", "block", "dests_per_wave",
+                                                              "block_stream", "syn_blocks_per_shard",
+                                                              "sum_blocks_per_shard", "tuned", "step_us")}
+                                           for s in r["launch_shapes"]]
+    except Exception as e:  # the headline line is still printed
+        out["batched"] = {"error": repr(e)}
+    a = argparse.Namespace(**vars(args))
+    a.requests = args.forms_requests
+    try:
+        r = served_result(a)
+        out["served"] = {k: r[k] for k in ("value", "unit", "latency_p50_s", "latency_p99_s", "latency_mean_s",
+                                           "sim_seconds_per_hour", "hbm_occupancy_mean", "hbm_occupancy_max",
+                                           "wall_s", "counts")}
+        out["served"]["workload"] = r["config"]["workload"]
+    except Exception as e:
+        out["served"] = {"error": repr(e)}
+    return out
 
 
 def main():
@@ -216,6 +260,12 @@ def main():
     ap.add_argument("--roofline-steps", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="single form: skip the batched and served summaries added to the headline line")
+    ap.add_argument("--forms-alone-stride", type=int, default=16,
+                    help="single form's batched summary: run every k-th sweep network alone")
+    ap.add_argument("--forms-requests", type=int, default=200,
+                    help="single form's served summary: requests of the stream")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device-to-device and exit (reference for HBM traffic counters)")
     args = ap.parse_args()
@@ -300,7 +350,7 @@ def main():
     # the instantiation the full blocks run: exactly 9 steps for a 32-bit ring, else up to 16 / 32
     kernel_name = "k_fused_block<LIF,DS,1,%d>" % (9 if K == 9 else (16 if K <= 16 else 32))
     out = {
-        "metric": "synapse-updates/s at 10k-LIF growth",
+        "metric": "synapse-updates/s at 10k-LIF growth; batched vs per-net sims/hr; p50/p99",
         "value": value,
         "unit": "synapse-updates/s",
         "n_gpus": 1,
@@ -333,6 +383,8 @@ def main():
     ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    if not args.no_forms:
+        out.update(forms_summary(args))
     print(json.dumps(out), flush=True)
 
 

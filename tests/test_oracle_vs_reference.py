@@ -193,3 +193,21 @@ def test_static_fixtures_pin_the_tie_order(name, k, thresh):
         reached = sorted(d[j] for (a, j) in ref if a == s)
         assert reached == sorted(d[cand].tolist())
     assert len(ref) == len(stable) and len(ref ^ stable) > 50
+
+
+def test_rate_fixture_is_the_oracles_reference_mode():
+    """tests/golden/ref_rates_10k.npz (the compiled reference's per-neuron spike counts on the
+    100 x 100 growth network, eight noise seeds) equals the oracle in reference mode, spike count
+    for spike count, on the first seed's first epoch."""
+    fx = np.load(os.path.join(GOLDEN, "ref_rates_10k.npz"))
+    counts, seeds = fx["counts"], fx["seeds"]
+    assert counts.shape == (8, 2, 10000)
+    c = O.default_config()
+    c.update(width=100, height=100, max_synapses_per_neuron=200, max_firing_rate=200, num_epochs=2,
+             epoch_duration=10.0, seed=int(seeds[0]), layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1)
+    ora = O.OracleNet(c)
+    ora.run_epoch()
+    got = np.rint(ora.array("ratesHistory")[10000:20000].astype(np.float64) * 10.0)
+    assert ora.total_synapses == 0
+    ora.close()
+    assert np.array_equal(got, counts[0, 0])
