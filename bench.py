@@ -175,6 +175,7 @@ def served_result(args):
     srv.run(jobs, progress_every=20.0)
     st = srv.stats(total_bytes=total)
     eng.close()
+    alone = served_alone(ctx, jobs, S, max(1, args.served_alone_stride))
     ctx.close()
     out = {
         "metric": "completed simulation jobs/s of an open-loop Poisson job stream (continuous batching)",
@@ -203,7 +204,42 @@ def served_result(args):
         "counts": st["counts"],
         "phase_s": st["host_phase_s"],
     }
+    out.update(alone)
+    if alone.get("alone_sim_seconds_per_hour"):
+        out["speedup_vs_alone"] = st["sim_seconds_per_hour"] / alone["alone_sim_seconds_per_hour"]
     return out
+
+
+def served_alone(ctx, jobs, S, stride: int) -> dict:
+    """Every `stride`-th completed job of the stream run again on its own (plain per-network
+    launches, epoch work after every epoch, one job after another), timed, and its output compared
+    with the one it got in the served mix."""
+    from braingrid_amd import sweep as SW
+    import numpy as np
+
+    done = [j for j in sorted(jobs, key=lambda j: j.id) if j.state == S.DONE]
+    sample = done[::stride]
+    if not sample:
+        return {}
+    same = 0
+    wall = 0.0
+    for j in sample:
+        x = SW.SweepNet(ctx, j.spec)
+        x.net.sync()
+        t0 = time.perf_counter()
+        for _ in range(x.epochs):
+            x.net.advance(x.steps_per_epoch)
+            x.end_epoch()
+        x.net.sync()
+        wall += time.perf_counter() - t0
+        r = x.summary()
+        same += all(np.array_equal(np.asarray(r[k]), np.asarray(j.result[k])) for k in r if k in j.result)
+        x.close()
+    sim = sum(j.spec["sim_s"] for j in sample)
+    return {"alone_sim_seconds_per_hour": sim / wall * 3600.0,
+            "alone_sample": f"every {stride}th completed job ({len(sample)} of {len(done)}, {sim:.0f} simulated s) "
+                            f"run alone in {wall:.1f} s",
+            "identical_results": f"{same}/{len(sample)} sampled jobs with the same output alone and served"}
 
 
 def forms_summary(args):
@@ -231,7 +267,8 @@ def forms_summary(args):
         r = served_result(a)
         out["served"] = {k: r[k] for k in ("value", "unit", "latency_p50_s", "latency_p99_s", "latency_mean_s",
                                            "sim_seconds_per_hour", "hbm_occupancy_mean", "hbm_occupancy_max",
-                                           "wall_s", "counts")}
+                                           "wall_s", "counts", "alone_sim_seconds_per_hour", "alone_sample",
+                                           "speedup_vs_alone", "identical_results") if k in r}
         out["served"]["workload"] = r["config"]["workload"]
     except Exception as e:
         out["served"] = {"error": repr(e)}
@@ -253,6 +290,8 @@ def main():
                     help="served form: share of the free HBM the resident networks may occupy")
     ap.add_argument("--alone-stride", type=int, default=4,
                     help="batched form: run every k-th network alone for the one-at-a-time comparison")
+    ap.add_argument("--served-alone-stride", type=int, default=10,
+                    help="served form: run every k-th completed job alone for the one-at-a-time comparison")
     ap.add_argument("--tune-steps", type=int, default=8,
                     help="batched/served forms: launch-shape autotuner, timed steps per candidate (0: off)")
     ap.add_argument("--steps", type=int, default=100000)

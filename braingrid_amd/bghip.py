@@ -139,6 +139,7 @@ def load():
         "bghip_upload_growth": (ctypes.c_int, [vp, ctypes.POINTER(Growth), _fp, _fp]),
         "bghip_advance": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "bghip_update_connections": (ctypes.c_int, [vp, ctypes.POINTER(GrowthStats)]),
+        "bghip_update_weights": (ctypes.c_int, [vp, _fp, ctypes.POINTER(GrowthStats)]),
         "bghip_update_history": (ctypes.c_int, [vp]),
         "bghip_sync": (ctypes.c_int, [vp]),
         "bghip_net_error_bits": (ctypes.c_uint32, [vp]),
@@ -191,7 +192,7 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_upload_input", "bghip_download_input", "bghip_recorder_enable", "bghip_recorder_download",
             "bghip_net_save", "bghip_net_load", "bghip_net_get_desc", "bghip_recorder_info",
             "bghip_add_synapses", "bghip_batch_autotune", "bghip_batch_group_count", "bghip_batch_get_shape",
-            "bghip_batch_set_shape")
+            "bghip_batch_set_shape", "bghip_update_weights")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -348,6 +349,13 @@ class Net:
     def update_connections(self) -> GrowthStats:
         st = GrowthStats()
         self._c(load().bghip_update_connections(self.h, ctypes.byref(st)), "update_connections")
+        return st
+
+    def update_weights(self, W: np.ndarray) -> GrowthStats:
+        """ConnGrowth::updateSynapsesWeights with W[src, dest] in place of the overlap areas."""
+        W = np.ascontiguousarray(W, dtype=np.float32)
+        st = GrowthStats()
+        self._c(load().bghip_update_weights(self.h, _ptr(W, _fp), ctypes.byref(st)), "update_weights")
         return st
 
     def update_history(self):

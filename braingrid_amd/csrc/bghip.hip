@@ -29,6 +29,9 @@ struct bghip_ctx {
     int device;
     char err[512];
     std::vector<TuneEntry> tuned;
+    // per-step runs replay captured graphs; BGHIP_GRAPHS=0 in the environment launches the same
+    // kernels directly instead (for tools that do not follow graph launches)
+    bool graphs;
 };
 
 struct bghip_net {
@@ -114,6 +117,8 @@ extern "C" int bghip_ctx_create(int device, bghip_ctx **out)
     if (!c) return BGHIP_ENOMEM;
     c->device = device;
     c->err[0] = 0;
+    const char *g = getenv("BGHIP_GRAPHS");
+    c->graphs = !(g && strcmp(g, "0") == 0);
     *out = c;
     return BGHIP_OK;
 }
@@ -671,7 +676,7 @@ extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
             // (its last node advances tbase), the rest is launched directly
             launch_step_set(n->stream, dn.tbase, n->step);
             uint64_t done = 0;
-            if (n_steps >= BG_GRAPH_STEPS) {
+            if (n_steps >= BG_GRAPH_STEPS && n->ctx->graphs) {
                 if (n->g_exec && memcmp(&dn, &n->g_dn, sizeof(DevNet)) != 0) drop_graph(n);
                 if (!n->g_exec) CHK(capture_stdp_graph(n));
                 for (; done + BG_GRAPH_STEPS <= n_steps; done += BG_GRAPH_STEPS)
@@ -816,6 +821,8 @@ extern "C" int bghip_add_synapses(bghip_net *n, int64_t count, const int32_t *sr
     return BGHIP_OK;
 }
 
+static int apply_weights(bghip_net *n, const float *Wd, bghip_growth_stats *stats);
+
 extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
 {
     if (!n) return BGHIP_EINVAL;
@@ -829,11 +836,19 @@ extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
     float c = g.epochDuration * g.rho;
     hipLaunchKernelGGL(k_growth_radii, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, g.epochDuration, g.epsilon,
                        g.maxRate, g.beta, c);
+    return apply_weights(n, nullptr, stats);
+}
+
+// ConnGrowth::updateSynapsesWeights (Connections/ConnGrowth.cpp:388-463) with W(s, d) = the overlap
+// areas (Wd null) or a device copy of the caller's matrix, then the incoming map.
+static int apply_weights(bghip_net *n, const float *Wd, bghip_growth_stats *stats)
+{
+    DevNet &dn = n->dn;
     const CreateCtx cc = create_ctx(n);
     HIP_TRY(n->ctx, hipMemsetAsync(n->d_stats, 0, 16, n->stream));
     if (dn.M > 0) {
         unsigned grid = (unsigned)std::min(dn.N, 4096);
-        hipLaunchKernelGGL(k_growth_dest, dim3(grid), dim3(BG_BLOCK), 0, n->stream, dn, cc, n->d_stats);
+        hipLaunchKernelGGL(k_growth_dest, dim3(grid), dim3(BG_BLOCK), 0, n->stream, dn, cc, n->d_stats, Wd);
     }
     HIP_TRY(n->ctx, hipGetLastError());
     CHK(rebuild_incoming(n));
@@ -846,6 +861,27 @@ extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
         stats->erased = st[1];
     }
     return BGHIP_OK;
+}
+
+extern "C" int bghip_update_weights(bghip_net *n, const float *W, bghip_growth_stats *stats)
+{
+    if (!n || !W) return BGHIP_EINVAL;
+    net_begin(n);
+    if (!n->has_layout) return set_err(n->ctx, BGHIP_EINVAL, "layout not uploaded");
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    flush_pending(n);
+    const size_t bytes = sizeof(float) * (size_t)n->dn.N * (size_t)n->dn.N;
+    float *Wd = nullptr;
+    HIP_TRY(n->ctx, hipMalloc((void **)&Wd, std::max<size_t>(bytes, 4)));
+    int rc = BGHIP_OK;
+    if (hipMemcpyAsync(Wd, W, bytes, hipMemcpyHostToDevice, n->stream) != hipSuccess)
+        rc = set_err(n->ctx, BGHIP_EDEVICE, "weight matrix upload failed");
+    if (rc == BGHIP_OK) rc = apply_weights(n, Wd, stats);
+    // the matrix is freed once the kernels reading it are done
+    const hipError_t e = hipStreamSynchronize(n->stream);
+    (void)hipFree(Wd);
+    if (rc == BGHIP_OK && e != hipSuccess) rc = set_err(n->ctx, BGHIP_EDEVICE, "%s", hipGetErrorString(e));
+    return rc;
 }
 
 extern "C" int bghip_update_history(bghip_net *n)
@@ -1692,7 +1728,7 @@ static int run_step_group(bghip_batch *b, BatchGroup &g, uint64_t s, uint64_t n_
 {
     const BatchMap bm = batch_map(g, g.shape);
     uint64_t off = s;   // offset of step s from the steps in the device table
-    if (n_steps - s >= BG_GRAPH_STEPS) {
+    if (n_steps - s >= BG_GRAPH_STEPS && b->ctx->graphs) {
         if (g.g_exec && memcmp(&g.g_shape, &g.shape, sizeof(BatchShape)) != 0) {
             (void)hipGraphExecDestroy(g.g_exec);
             g.g_exec = nullptr;

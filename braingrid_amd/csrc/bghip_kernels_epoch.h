@@ -94,17 +94,26 @@ __device__ void create_synapse(const DevNet &n, const CreateCtx &c, uint64_t slo
 // in ascending source order. Source coordinates and radii stream through LDS in tiles; the
 // connected-source set is an LDS bitmap; create decisions are compacted to slots with a
 // wavefront-ballot + workgroup prefix scan.
-__device__ void growth_serial(const DevNet &n, const CreateCtx &c, int d, const uint32_t *conn, uint32_t *stats)
+// W(s, d) of the update: the circles' overlap area (the growth rule's W = area,
+// Connections/ConnGrowth.cpp:395), or entry [s * N + d] of a caller's dense matrix when Wd is set.
+__device__ __forceinline__ float pair_weight(const DevNet &n, const float *Wd, int s, int d, float xs, float ys,
+                                             float rs, float xd, float yd, float rd)
+{
+    return Wd ? Wd[(uint64_t)s * (uint64_t)n.N + (uint64_t)d] : overlap_area(xs, ys, rs, xd, yd, rd, s == d);
+}
+
+__device__ void growth_serial(const DevNet &n, const CreateCtx &c, int d, const uint32_t *conn, uint32_t *stats,
+                              const float *Wd)
 {
     // Replays the reference's per-source order one source at a time. Used for rows that contain
-    // holes or need erasures (the reference's growth rule itself never produces area < 0).
+    // holes or need erasures (the growth rule itself never produces area < 0; a caller's matrix can).
     const int N = n.N, M = n.M;
     const uint64_t base = (uint64_t)d * M;
     const float xd = n.xloc[d], yd = n.yloc[d], rd = n.radii[d];
     const int td = n.ntype[d];
     uint32_t count = n.syn_count[d], hi = n.slot_hi[d], created = 0, erased = 0;
     for (int s = 0; s < N; s++) {
-        float area = overlap_area(n.xloc[s], n.yloc[s], n.radii[s], xd, yd, rd, s == d);
+        float area = pair_weight(n, Wd, s, d, n.xloc[s], n.yloc[s], n.radii[s], xd, yd, rd);
         int type = syn_type_of(n.ntype[s], td);
         bool connected = (conn[s >> 5] >> (s & 31)) & 1u;
         if (connected) {
@@ -136,7 +145,7 @@ __device__ void growth_serial(const DevNet &n, const CreateCtx &c, int d, const 
     atomicAdd(&stats[1], erased);
 }
 
-__global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c, uint32_t *stats)
+__global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c, uint32_t *stats, const float *Wd)
 {
     __shared__ float sx[BG_GROWTH_TILE], sy[BG_GROWTH_TILE], sr[BG_GROWTH_TILE];
     __shared__ uint32_t conn[BG_CONN_WORDS];
@@ -160,14 +169,14 @@ __global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c,
             if (!(meta & BG_META_IN_USE)) continue;
             int s = (int)(meta & BG_META_SRC_MASK);
             atomicOr(&conn[s >> 5], 1u << (s & 31));
-            float area = overlap_area(n.xloc[s], n.yloc[s], n.radii[s], xd, yd, rd, s == d);
+            float area = pair_weight(n, Wd, s, d, n.xloc[s], n.yloc[s], n.radii[s], xd, yd, rd);
             int type = syn_type_of(n.ntype[s], td);
             if (area < 0) atomicAdd(&s_erase, 1u);
             else n.W[base + k] = area * (float)syn_sign_of(type) * c.W_adj;
         }
         __syncthreads();
         if (s_erase || cnt0 != hi0) {
-            if (tid == 0) growth_serial(n, c, d, conn, stats);
+            if (tid == 0) growth_serial(n, c, d, conn, stats, Wd);
             continue;
         }
         // rows without holes: the j-th creation (ascending source) takes slot hi0 + j
@@ -186,7 +195,7 @@ __global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c,
                 bool create = false;
                 float area = 0.0f;
                 if (q < len && !((conn[s >> 5] >> (s & 31)) & 1u)) {
-                    area = overlap_area(sx[q], sy[q], sr[q], xd, yd, rd, s == d);
+                    area = pair_weight(n, Wd, s, d, sx[q], sy[q], sr[q], xd, yd, rd);
                     create = area > 0;
                 }
                 const uint64_t ball = __ballot(create);

@@ -158,7 +158,10 @@ typedef struct bghip_input_state {
 } bghip_input_state;
 
 /* ---- context ---------------------------------------------------------------------------- */
-/* Replaces GPUSpikingModel::setupSim's cudaSetDevice (Core/GPUSpikingModel.cu:109). */
+/* Replaces GPUSpikingModel::setupSim's cudaSetDevice (Core/GPUSpikingModel.cu:109). Per-step runs
+ * of the context's networks and batches replay captured HIP graphs of 256 steps; with BGHIP_GRAPHS=0
+ * in the environment at this call the same kernels are launched directly (for tools that do not
+ * follow graph launches). */
 int bghip_ctx_create(int device, bghip_ctx **out);
 void bghip_ctx_destroy(bghip_ctx *ctx);
 const char *bghip_last_error(const bghip_ctx *ctx);
@@ -215,6 +218,15 @@ int bghip_advance(bghip_net *net, uint64_t n_steps);
  * overlap, weight update, synapse create/erase, index-map rebuild, all on the device.
  * stats may be NULL; when given the call synchronises. */
 int bghip_update_connections(bghip_net *net, bghip_growth_stats *stats);
+/* The weight-update half of it with a caller's matrix instead of the overlap areas: the device
+ * side of ConnGrowth::updateSynapsesWeights (Connections/ConnGrowth_d.cu:18-61, which uploads
+ * W_h[src * N + dest] = W(src, dest) and runs updateSynapsesWeightsDevice,
+ * Synapses/AllSynapsesDeviceFuncs_d.cu:858-916). W is host memory, N x N floats, W[src * N + dest]:
+ * an existing synapse with W < 0 is erased, otherwise its weight becomes W * sign * 1e-8; a missing
+ * one with W > 0 is created in the destination's first free slot (slots freed by this call's erasures
+ * included), sources in ascending order; then the incoming map is rebuilt. Radii and rates are not
+ * touched. Needs the layout (neuron types); synchronises. */
+int bghip_update_weights(bghip_net *net, const float *W, bghip_growth_stats *stats);
 /* The model part of GPUSpikingModel::updateHistory (Core/GPUSpikingModel.cu:266-272): applies the
  * recorder's minRadius clamp (Recorders/XmlGrowthRecorder.cpp:89-90) and clears spike counts
  * (Neurons/AllSpikingNeurons.cpp:79-87). Download spikes and rates/radii before calling it. */

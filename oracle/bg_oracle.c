@@ -928,12 +928,14 @@ static float overlap_xy(float xi, float yi, float r1, float xj, float yj, float 
     return (float)(0.5 * (double)(r22 * (angCBD - f_sinf(angCBD)) + r12 * (angCAD - f_sinf(angCAD))));
 }
 
+static int apply_weights(bgo_net *n, const float *Wm);
+
 int bgo_update_connections(bgo_net *n)
 {
     if (n->has_error) return -1;
     if (n->p.conn_kind != BGO_CONN_GROWTH) return 0;
     const bgo_params *p = &n->p;
-    int N = n->N, M = n->M;
+    int N = n->N;
     /* ConnGrowth::updateConns (Connections/ConnGrowth.cpp:295-311), VectorMatrix operators in float */
     float maxRate = p->target_rate / p->epsilon;   /* ConnGrowth.cpp:201-203 */
     float c = p->epoch_duration * p->rho;
@@ -948,9 +950,18 @@ int bgo_update_connections(bgo_net *n)
         float outgrowth = 1.0f - t;
         n->radii[i] += outgrowth * c;
     }
-    /* ConnGrowth::updateSynapsesWeights (Connections/ConnGrowth.cpp:388-463). The reference visits
-     * (src, dest) with src outer; every operation touches only dest's slots, so visiting dest outer
-     * and src inner performs the same per-destination sequence. */
+    return apply_weights(n, NULL);
+}
+
+/* ConnGrowth::updateSynapsesWeights (Connections/ConnGrowth.cpp:388-463) with W(src, dest) = Wm[src * N
+ * + dest], or the circles' overlap area when Wm is NULL (the growth rule's W = area, :395). The
+ * reference visits (src, dest) with src outer; every operation touches only dest's slots, so
+ * visiting dest outer and src inner performs the same per-destination sequence. An existing
+ * synapse with W < 0 is erased (eraseSynapse, Synapses/AllSynapses.cpp:339-344) and its slot is free
+ * for a later source's new synapse, as addSynapse takes the first free slot (:357-377). */
+static int apply_weights(bgo_net *n, const float *Wm)
+{
+    int N = n->N, M = n->M;
     const float adj = (float)1.0e-8;   /* AllSynapses::SYNAPSE_STRENGTH_ADJUSTMENT */
     for (int d = 0; d < N; d++) {
         size_t base = (size_t)d * M;
@@ -962,7 +973,7 @@ int bgo_update_connections(bgo_net *n)
         }
         int first_free = 0;
         for (int s = 0; s < N; s++) {
-            float area = overlap_area(n, s, d);
+            float area = Wm ? Wm[(size_t)s * N + d] : overlap_area(n, s, d);
             int t = syn_type(n, s, d);
             int k = n->scratch_slot[s];
             if (k >= 0) {
@@ -987,6 +998,13 @@ int bgo_update_connections(bgo_net *n)
         for (int s = 0; s < N; s++) n->scratch_slot[s] = -1;
     }
     return bgo_rebuild_index_map(n);
+}
+
+int bgo_update_weights(bgo_net *n, const float *W)
+{
+    if (n->has_error) return -1;
+    if (!W) return fail(n, "no weight matrix");
+    return apply_weights(n, W);
 }
 
 /* XmlRecorder::compileHistories (Recorders/XmlRecorder.cpp:71-104) +

@@ -86,6 +86,10 @@ void bgo_destroy(bgo_net *n);
 int bgo_advance(bgo_net *n, int64_t n_steps);
 /* IModel::updateConnections (growth + index map). */
 int bgo_update_connections(bgo_net *n);
+/* ConnGrowth::updateSynapsesWeights with a caller's dense weight matrix W[src * N + dest] (instead
+ * of the overlap areas), then the index map: W < 0 erases an existing synapse, W >= 0 sets its
+ * weight, W > 0 creates a missing one. */
+int bgo_update_weights(bgo_net *n, const float *W);
 /* IModel::updateHistory (recorder compileHistories incl. minRadius clamp, clear spike counts). */
 int bgo_update_history(bgo_net *n);
 /* One full epoch: advance(epoch steps) + update_connections + update_history. */

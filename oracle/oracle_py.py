@@ -86,6 +86,8 @@ def lib():
             getattr(L, f).restype = ctypes.c_int
         L.bgo_advance.argtypes = [ctypes.c_void_p, ctypes.c_int64]
         L.bgo_advance.restype = ctypes.c_int
+        L.bgo_update_weights.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.bgo_update_weights.restype = ctypes.c_int
         L.bgo_add_synapse.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float]
         L.bgo_add_synapse.restype = ctypes.c_int64
         L.bgo_set_radii.argtypes = [ctypes.c_void_p, ctypes.c_float]
@@ -314,6 +316,14 @@ class OracleNet:
 
     def update_connections(self):
         self._check(lib().bgo_update_connections(self.h), "update_connections")
+
+    def update_weights(self, W: np.ndarray):
+        """ConnGrowth::updateSynapsesWeights with W[src, dest] in place of the overlap areas."""
+        W = np.ascontiguousarray(W, dtype=np.float32)
+        n = self.cfg["width"] * self.cfg["height"]
+        assert W.shape == (n, n)
+        self._check(lib().bgo_update_weights(self.h, W.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "update_weights")
 
     def update_history(self):
         self._check(lib().bgo_update_history(self.h), "update_history")

@@ -67,3 +67,43 @@ def test_block_size_does_not_change_results(hip_ctx, block_steps):
     print(f"[parity] conn-lif-ds-dense block_steps={block_steps}", flush=True)
     bad, spikes, total = _run(hip_ctx, "conn-lif-ds-dense", block_steps=block_steps)
     assert not bad, bad[:10]
+
+
+def test_dense_weight_update_with_erasures_matches_oracle(hip_ctx):
+    """bghip_update_weights (ConnGrowth::updateSynapsesWeights with a caller's W matrix): existing
+    synapses erased (their slots reused by later sources' new synapses), reweighted or kept at zero
+    weight, new ones created; slot state and index maps equal the oracle's, and so does the network
+    over the steps that follow and a second update."""
+    cfg = O.default_config()
+    cfg.update(width=6, height=6, max_synapses_per_neuron=30, layout="DynamicLayout", frac_exc=0.8,
+               frac_starter=0.1, noise_mode=1, math_mode=1, sum_mode=1, epoch_duration=0.05, num_epochs=2)
+    ora = O.OracleNet(cfg)
+    ora.set_radii(0.75)
+    net = H.gpu_net_from_oracle(hip_ctx, ora, cfg)
+    ora.update_connections()
+    net.update_connections()
+    N, M = 36, 30
+    rng = np.random.default_rng(11)
+    bad = []
+    for rnd in range(2):
+        src = ora.array("sourceNeuronIndex")
+        existing = np.zeros((N, N), bool)
+        for k in np.flatnonzero(ora.array("in_use")):
+            existing[src[k], k // M] = True
+        Wm = rng.uniform(-1.0, 1.0, (N, N)).astype(np.float32)
+        Wm[rng.random((N, N)) < 0.75] = 0.0
+        Wm[existing] = rng.choice(np.array([-0.5, 0.0, 0.25, 3.0], np.float32), existing.sum())
+        ora.update_weights(Wm)
+        st = net.update_weights(Wm)
+        print(f"[weights] round {rnd}: {st.created} created, {st.erased} erased, {st.total_synapses} synapses",
+              flush=True)
+        assert st.erased == int(((Wm < 0) & existing).sum()) > 0
+        assert st.created == int(((Wm > 0) & ~existing).sum()) > 0
+        assert st.total_synapses == ora.total_synapses
+        bad += H.compare_state(ora, net, cfg, f"weights-{rnd}")
+        ora.advance(300)
+        net.advance(300)
+        net.sync()
+        bad += H.compare_state(ora, net, cfg, f"weights-{rnd}-advanced")
+    net.close()
+    assert not bad, bad[:10]

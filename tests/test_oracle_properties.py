@@ -222,3 +222,71 @@ def test_poisson_intervals_follow_the_rejection_density():
     b = isi[977:]
     assert abs(np.corrcoef(a, b)[0, 1]) < 0.03
     assert abs(np.corrcoef(isi[:-1], isi[1:])[0, 1]) < 0.03
+
+
+def _ref_update_weights(in_use, src, W, counts, Wm, ntype, M):
+    """Connections/ConnGrowth.cpp:405-461 as the reference runs it: source outer, destination inner;
+    an existing synapse (s, d) is erased when W(s, d) < 0 (Synapses/AllSynapses.cpp:339-344) or gets
+    W(s, d) * sign * 1e-8; a missing one with W(s, d) > 0 goes into d's first free slot
+    (Synapses/AllSynapses.cpp:357-377). Sign: -1 for an inhibitory source (Layouts/Layout.cpp:129-141)."""
+    N = len(counts)
+    adj = F32(1.0e-8)
+    for s in range(N):
+        for d in range(N):
+            row = range(d * M, (d + 1) * M)
+            sign = F32(-1.0) if ntype[s] == 1 else F32(1.0)
+            w = F32(Wm[s, d])
+            connected = False
+            for k in row:
+                if in_use[k] and src[k] == s:
+                    connected = True
+                    if w < 0:
+                        in_use[k] = 0
+                        counts[d] -= 1
+                    else:
+                        W[k] = w * sign * adj
+            if not connected and w > 0:
+                k = next(k for k in row if not in_use[k])
+                in_use[k], src[k], W[k] = 1, s, w * sign * adj
+                counts[d] += 1
+
+
+def test_update_weights_matches_reference_loop_order_with_erasures():
+    """The oracle's dense-W weight update (destination outer) equals the reference's source-outer
+    loop on a network with existing synapses: some erased, their slots reused by later sources,
+    some reweighted (zero weights kept), new ones created; NaN entries neither erase nor create."""
+    cfg = O.default_config()
+    cfg.update(width=6, height=6, max_synapses_per_neuron=30, layout="DynamicLayout", frac_exc=0.8,
+               frac_starter=0.1, noise_mode=1, math_mode=1, sum_mode=1)
+    net = O.OracleNet(cfg)
+    net.set_radii(0.75)
+    net.update_connections()
+    N, M = 36, 30
+    in_use = net.array("in_use").astype(np.int64)
+    src = net.array("sourceNeuronIndex").astype(np.int64)
+    W = net.array("W").copy()
+    counts = net.array("synapse_counts").astype(np.int64)
+    ntype = net.array("neuron_type_map")
+    assert in_use.sum() > 100
+    rng = np.random.default_rng(5)
+    Wm = rng.uniform(-1.0, 1.0, (N, N)).astype(np.float32)
+    Wm[rng.random((N, N)) < 0.7] = 0.0          # keep rows under M
+    existing = np.zeros((N, N), bool)
+    for k in np.flatnonzero(in_use):
+        existing[src[k], k // M] = True
+    Wm[existing] = rng.choice(np.array([-0.5, 0.0, 0.25, np.nan], np.float32), existing.sum())
+    _ref_update_weights(in_use, src, W, counts, Wm, ntype, M)
+    net.update_weights(Wm)
+    got_use = net.array("in_use").astype(np.int64)
+    assert np.array_equal(got_use, in_use)
+    live = got_use == 1
+    assert np.array_equal(net.array("sourceNeuronIndex")[live], src[live])
+    # NaN weights stay NaN (their sign bit depends on how the compiler multiplies by -1)
+    got_w = net.array("W")[live]
+    nan = np.isnan(W[live])
+    assert nan.sum() > 5 and np.array_equal(np.isnan(got_w), nan)
+    assert np.array_equal(got_w[~nan].view(np.uint32), W[live][~nan].view(np.uint32))
+    assert np.array_equal(net.array("synapse_counts").astype(np.int64), counts)
+    erased = int(((Wm < 0) & existing).sum())
+    assert erased > 10 and int(((Wm > 0) & ~existing).sum()) > 10
+    net.close()
