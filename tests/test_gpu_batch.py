@@ -176,3 +176,49 @@ def test_batch_membership_changes(hip_ctx):
     for n in nets:
         n.close()
     assert not bad, bad[:10]
+
+
+def test_stdp_rows_growing_after_the_group_formed(hip_ctx):
+    """Weight updates that add sources to existing destinations (bghip_update_weights) between batch
+    advances lengthen rows without changing any member's mode: the per-step STDP group, its tables
+    and its captured graph stay, and every member still equals the oracle."""
+    names = ["conn-lif-stdp", "conn-izh-stdp", "conn-lif-dynstdp"]
+    rng = np.random.default_rng(3)
+    cfgs = [H.load_config(n) for n in names]
+    oras = [O.OracleNet(c) for c in cfgs]
+    nets = [H.gpu_net_from_oracle(hip_ctx, o, c) for o, c in zip(oras, cfgs)]
+    for o, n in zip(oras, nets):
+        o.advance(400)
+        n.advance(400)
+        o.update_connections()
+        n.update_connections()
+    batch = B.Batch(hip_ctx)
+    batch.set(nets)
+    bad = []
+    grew = 0
+    for rnd in range(3):
+        for o in oras:
+            o.advance(700)
+        batch.advance(700)
+        assert all(g["mode"] == 2 for g in batch.shapes()), batch.shapes()
+        for k, (o, n, c) in enumerate(zip(oras, nets, cfgs)):
+            n.sync()
+            bad += H.compare_state(o, n, c, f"{names[k]}-grow-round{rnd}", check_maps=False)
+        before = [o.total_synapses for o in oras]
+        for o, n, c in zip(oras, nets, cfgs):
+            N = c["width"] * c["height"]
+            Wm = np.where(rng.random((N, N)) < 0.25, 0.5, 0.0).astype(np.float32)
+            np.fill_diagonal(Wm, 0.0)
+            M = c["max_synapses_per_neuron"]
+            src = o.array("sourceNeuronIndex")
+            for k in np.flatnonzero(o.array("in_use")):
+                Wm[src[k], k // M] = 0.75    # existing synapses stay (reweighted)
+            o.update_weights(Wm)
+            n.update_weights(Wm)
+        grew += sum(o.total_synapses > b for o, b in zip(oras, before))
+    print(f"[batch] {grew} growth updates lengthened rows inside one group", flush=True)
+    assert grew >= 3
+    batch.close()
+    for n in nets:
+        n.close()
+    assert not bad, bad[:10]
