@@ -140,6 +140,8 @@ def load():
         "bghip_advance": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "bghip_update_connections": (ctypes.c_int, [vp, ctypes.POINTER(GrowthStats)]),
         "bghip_update_weights": (ctypes.c_int, [vp, _fp, ctypes.POINTER(GrowthStats)]),
+        "bghip_download_spikes": (ctypes.c_int, [vp, _ip, _ip, _qp]),
+        "bghip_clear_spike_counts": (ctypes.c_int, [vp]),
         "bghip_update_history": (ctypes.c_int, [vp]),
         "bghip_sync": (ctypes.c_int, [vp]),
         "bghip_net_error_bits": (ctypes.c_uint32, [vp]),
@@ -192,7 +194,7 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_upload_input", "bghip_download_input", "bghip_recorder_enable", "bghip_recorder_download",
             "bghip_net_save", "bghip_net_load", "bghip_net_get_desc", "bghip_recorder_info",
             "bghip_add_synapses", "bghip_batch_autotune", "bghip_batch_group_count", "bghip_batch_get_shape",
-            "bghip_batch_set_shape", "bghip_update_weights")
+            "bghip_batch_set_shape", "bghip_update_weights", "bghip_download_spikes", "bghip_clear_spike_counts")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -390,6 +392,18 @@ class Net:
             setattr(nb, f, _ptr(a, _CT[a.dtype.type]))
         self._c(load().bghip_download_neurons(self.h, ctypes.byref(nb)), "download_neurons")
         return out
+
+    def download_spikes(self) -> dict:
+        """spikeCount, spikeCountOffset and spike_history (bghip_download_spikes)."""
+        N, ms = self.N, self.desc.max_spikes
+        out = {"spikeCount": np.zeros(N, np.int32), "spikeCountOffset": np.zeros(N, np.int32),
+               "spike_history": np.zeros(N * ms, np.uint64)}
+        self._c(load().bghip_download_spikes(self.h, _ptr(out["spikeCount"], _ip), _ptr(out["spikeCountOffset"], _ip),
+                                             _ptr(out["spike_history"], _qp)), "download_spikes")
+        return out
+
+    def clear_spike_counts(self):
+        self._c(load().bghip_clear_spike_counts(self.h), "clear_spike_counts")
 
     def download_synapses(self) -> dict:
         S = self.N * self.M

@@ -1001,6 +1001,33 @@ extern "C" int bghip_download_neurons(bghip_net *n, bghip_neurons *h)
     return bghip_sync(n);
 }
 
+extern "C" int bghip_download_spikes(bghip_net *n, int32_t *counts, int32_t *offsets, uint64_t *history)
+{
+    if (!n) return BGHIP_EINVAL;
+    net_begin(n);
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    const DevNet &dn = n->dn;
+    const size_t N = dn.N;
+    if (counts) CHK(d2h(n, counts, dn.cnt, N));
+    if (offsets) CHK(d2h(n, offsets, dn.off, N));
+    if (history) CHK(d2h(n, history, dn.hist, N * (size_t)dn.max_spikes));
+    return bghip_sync(n);
+}
+
+extern "C" int bghip_clear_spike_counts(bghip_net *n)
+{
+    if (!n) return BGHIP_EINVAL;
+    net_begin(n);
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    // k_update_history without the radius clamp and history rows is exactly clearSpikeCounts
+    const unsigned nb = (unsigned)((n->dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    hipLaunchKernelGGL(k_update_history, dim3(nb), dim3(BG_BLOCK), 0, n->stream, n->dn, 0.0f, 0, nullptr, nullptr);
+    HIP_TRY(n->ctx, hipGetLastError());
+    n->hist_step0 = n->step;   // the recorder's next compile starts after the cleared spikes
+    n->hist_range_known = true;
+    return BGHIP_OK;
+}
+
 extern "C" int64_t bghip_total_synapses(bghip_net *n) { return n ? (int64_t)net_total(n) : -1; }
 
 extern "C" int bghip_download_synapses(bghip_net *n, bghip_synapses *h)

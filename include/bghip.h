@@ -257,6 +257,14 @@ int bghip_recorder_info(const bghip_net *net, int64_t *n_burst, int64_t *n_spike
 
 /* ---- downloads (blocking) --------------------------------------------------------------- */
 int bghip_download_neurons(bghip_net *net, bghip_neurons *h);
+/* The spike bookkeeping only: AllSpikingNeurons::copyDeviceSpikeCountsToHost and
+ * copyDeviceSpikeHistoryToHost (Neurons/AllSpikingNeurons_d.cu:17-44). counts and offsets hold N
+ * ints, history N * max_spikes step numbers (row i = neuron i's ring); NULL pointers are skipped. */
+int bghip_download_spikes(bghip_net *net, int32_t *counts, int32_t *offsets, uint64_t *history);
+/* AllSpikingNeurons::clearSpikeCounts (Neurons/AllSpikingNeurons.cpp:79-87; the device copy
+ * Neurons/AllSpikingNeurons_d.cu:53-59): offset = (count + offset) mod max_spikes, count = 0, for a
+ * caller with recorders of its own (bghip_update_history does this itself). Stream-ordered. */
+int bghip_clear_spike_counts(bghip_net *net);
 int bghip_download_synapses(bghip_net *net, bghip_synapses *h);
 int64_t bghip_total_synapses(bghip_net *net);
 int bghip_download_index_map(bghip_net *net, bghip_index_map *h);

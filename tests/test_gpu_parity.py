@@ -107,3 +107,30 @@ def test_dense_weight_update_with_erasures_matches_oracle(hip_ctx):
         bad += H.compare_state(ora, net, cfg, f"weights-{rnd}-advanced")
     net.close()
     assert not bad, bad[:10]
+
+
+def test_spike_download_and_clear_match_oracle(hip_ctx):
+    """bghip_download_spikes returns the oracle's spike counts, offsets and history rings, and
+    bghip_clear_spike_counts rotates the offsets and zeroes the counts as clearSpikeCounts does
+    (the oracle clears inside its history update); the network then runs on bit for bit."""
+    cfg = H.load_config("static-lif-ds")
+    ora = O.OracleNet(cfg)
+    net = H.gpu_net_from_oracle(hip_ctx, ora, cfg)
+    bad = []
+    for rnd in range(3):
+        ora.advance(1700)
+        net.advance(1700)
+        sp = net.download_spikes()
+        for k in ("spikeCount", "spikeCountOffset", "spike_history"):
+            if not np.array_equal(sp[k], ora.array(k).reshape(-1).astype(sp[k].dtype)):
+                bad.append(f"round {rnd}: {k}")
+        assert int(sp["spikeCount"].sum()) > 0
+        ora.update_history()
+        net.clear_spike_counts()
+        sp = net.download_spikes()
+        assert int(sp["spikeCount"].sum()) == 0
+        if not np.array_equal(sp["spikeCountOffset"], ora.array("spikeCountOffset")):
+            bad.append(f"round {rnd}: offsets after clear")
+        bad += H.compare_state(ora, net, cfg, f"cleared-{rnd}")
+    net.close()
+    assert not bad, bad[:10]
