@@ -1426,6 +1426,8 @@ struct BatchGroup {
     uint16_t *d_nof = nullptr;     // member of each neuron / destination (total)
     uint2 *d_chunks[3] = {};       // per-step STDP: runs of 1, 2 and 4 destinations per wavefront
     uint32_t nchunks[3] = {};
+    uint2 *d_nwaves = nullptr;     // thread-per-neuron kernels: {member, first neuron} per wavefront
+    uint32_t nnwaves = 0;
     BatchShape shape = {};         // launch shape in use
     bool tuned = false;            // shape picked by the autotuner (or set by the caller)
     float step_us = 0.0f;          // measured time per step of that shape
@@ -1467,6 +1469,7 @@ static void free_groups(bghip_batch *b)
         (void)hipFree(g.d_base);
         (void)hipFree(g.d_nof);
         for (uint2 *c : g.d_chunks) (void)hipFree(c);
+        (void)hipFree(g.d_nwaves);
         (void)hipFree(g.d_sbase);
         (void)hipFree(g.d_tprev);
         (void)hipFree(g.d_sel);
@@ -1585,6 +1588,12 @@ static int regroup(bghip_batch *b)
         for (int k = 0; k < nn; k++) std::fill(nof.begin() + base[k], nof.begin() + base[k + 1], (uint16_t)k);
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_nof, sizeof(uint16_t) * nof.size()));
         HIP_TRY(b->ctx, hipMemcpy(g.d_nof, nof.data(), sizeof(uint16_t) * nof.size(), hipMemcpyHostToDevice));
+        std::vector<uint2> nwaves;
+        for (int k = 0; k < nn; k++)
+            for (uint32_t first = 0; first < base[k + 1] - base[k]; first += BG_WAVE) nwaves.push_back(make_uint2(k, first));
+        g.nnwaves = (uint32_t)nwaves.size();
+        HIP_TRY(b->ctx, hipMalloc((void **)&g.d_nwaves, sizeof(uint2) * std::max<size_t>(nwaves.size(), 1)));
+        HIP_TRY(b->ctx, hipMemcpy(g.d_nwaves, nwaves.data(), sizeof(uint2) * nwaves.size(), hipMemcpyHostToDevice));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_nets, sizeof(BatchNet) * nn));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_base, sizeof(uint32_t) * (nn + 1)));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sbase, sizeof(uint64_t) * (nn + 1)));
@@ -1647,7 +1656,8 @@ static int regroup(bghip_batch *b)
 static BatchMap batch_map(const BatchGroup &g, const BatchShape &sh)
 {
     const int v = sh.dpw == 1 ? 0 : (sh.dpw == 2 ? 1 : 2);
-    return BatchMap{g.d_nets, g.d_base, g.d_nof, g.total, (int)g.members.size(), g.d_chunks[v], g.nchunks[v]};
+    return BatchMap{g.d_nets, g.d_base,      g.d_nof,    g.total, (int)g.members.size(), g.d_chunks[v],
+                    g.nchunks[v], g.d_nwaves, g.nnwaves};
 }
 
 // ---- batch autotuner -----------------------------------------------------------------------

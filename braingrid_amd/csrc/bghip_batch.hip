@@ -9,10 +9,6 @@ static unsigned wave_blocks(uint32_t waves, int blk)
 {
     return (unsigned)(((uint64_t)waves * BG_WAVE + (uint64_t)blk - 1) / (uint64_t)blk);
 }
-static unsigned thread_blocks(uint32_t threads, int blk)
-{
-    return (unsigned)(((uint64_t)threads + (uint64_t)blk - 1) / (uint64_t)blk);
-}
 
 template <int NK, int SK, int W32>
 static void fused(const BatchMap &bm, hipStream_t s, int blk, int kmax, int K, int Kfull, int launch)
@@ -68,7 +64,7 @@ static void syn_w(const BatchMap &bm, hipStream_t s, int blk, int w32, uint64_t 
 void launch_neuron_only_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, uint64_t step,
                               int steps)
 {
-    const dim3 grid(thread_blocks(bm.total, sh.blk)), block(sh.blk);
+    const dim3 grid(wave_blocks(bm.nnwaves, sh.blk)), block(sh.blk);
     if (nk == NK_LIF)
         hipLaunchKernelGGL(k_neuron_only_batch<NK_LIF>, grid, block, 0, s, bm, step, steps);
     else
@@ -82,7 +78,7 @@ void launch_step_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, 
         launch_stdp_step_batch(bm, s, sh, nk, sk, w32, step, ev);
         return;
     }
-    const dim3 grid(thread_blocks(bm.total, sh.blk)), block(sh.blk);
+    const dim3 grid(wave_blocks(bm.nnwaves, sh.blk)), block(sh.blk);
     if (nk == NK_LIF)
         hipLaunchKernelGGL(k_neuron_batch<NK_LIF>, grid, block, 0, s, bm, step);
     else

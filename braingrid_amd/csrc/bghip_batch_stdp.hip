@@ -44,8 +44,7 @@ void launch_stdp_step_batch(const BatchMap &bm, hipStream_t s, const BatchShape 
                             uint64_t step, const EvLists &ev)
 {
     // the neuron pass clears the 2 * BG_EV_SHARDS event counters, one per thread
-    const unsigned nb = std::max((unsigned)((bm.total + (uint32_t)sh.blk - 1) / (uint32_t)sh.blk),
-                                 (unsigned)((2 * BG_EV_SHARDS + sh.blk - 1) / sh.blk));
+    const unsigned nb = std::max(wave_blocks(bm.nnwaves, sh.blk), (unsigned)((2 * BG_EV_SHARDS + sh.blk - 1) / sh.blk));
     if (nk == NK_LIF)
         hipLaunchKernelGGL(k_stdp_neurons_batch<NK_LIF>, dim3(nb), dim3(sh.blk), 0, s, bm, step, ev);
     else

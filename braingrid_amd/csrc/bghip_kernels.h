@@ -851,14 +851,34 @@ __global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_FUS
     fused_block_body<NK, SK, W32, KMAX>(b.n, (int)(w - bm.base[i]), t0, K, pull, tprev0, sel);
 }
 
+// Thread-per-neuron kernels take their neurons from the per-wave table nwaves[v] = {member, first
+// neuron}: every wavefront holds neurons of one network (its last wavefront may be partial), so the
+// member's descriptor is wave-uniform and read through scalar loads instead of per-lane gathers.
+struct NeuronLane {
+    int member;     // wave-uniform
+    int neuron;     // this lane's neuron in the member network
+    bool valid;     // the lane has a neuron
+};
+__device__ __forceinline__ NeuronLane neuron_lane(const BatchMap &bm)
+{
+    NeuronLane r = {0, 0, false};
+    const uint32_t v = __builtin_amdgcn_readfirstlane(
+        (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
+    if (v >= bm.nnwaves) return r;
+    const uint2 e = bm.nwaves[v];
+    r.member = __builtin_amdgcn_readfirstlane((int)e.x);
+    r.neuron = (int)e.y + (int)(threadIdx.x & (BG_WAVE - 1));
+    r.valid = r.neuron < bm.nets[r.member].n.N;
+    return r;
+}
+
 template <int NK>
 __global__ __launch_bounds__(BG_BLOCK) void k_neuron_batch(BatchMap bm, uint64_t step)
 {
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= bm.total) return;
-    const int i = bm.nof[w];
-    const BatchNet &b = bm.nets[i];
-    neuron_step_body<NK>(b.n, (int)(w - bm.base[i]), b.t0 + step);
+    const NeuronLane q = neuron_lane(bm);
+    if (!q.valid) return;
+    const BatchNet &b = bm.nets[q.member];
+    neuron_step_body<NK>(b.n, q.neuron, b.t0 + step);
 }
 
 template <int SK, int W32>
@@ -879,20 +899,15 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons_batch(BatchMap bm, ui
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (BG_WAVE - 1);
     if (w < 2 * BG_EV_SHARDS) ev.count[w * BG_EV_PAD] = 0;   // launches have at least 2 * BG_EV_SHARDS threads
+    const NeuronLane q = neuron_lane(bm);
+    const BatchNet &b = bm.nets[q.member];
     bool fired = false;
-    int ni = 0, li = 0;
-    if (w < bm.total) {
-        ni = bm.nof[w];
-        li = (int)(w - bm.base[ni]);
-        fired = neuron_step_body<NK>(bm.nets[ni].n, li, bm.nets[ni].t0 + step);
-    }
+    if (q.valid) fired = neuron_step_body<NK>(b.n, q.neuron, b.t0 + step);
     uint64_t fm = __ballot(fired);
     while (fm) {
         const int l = __ffsll((unsigned long long)fm) - 1;
         fm &= fm - 1;
-        const int bi = __builtin_amdgcn_readfirstlane(__shfl(ni, l));
-        const BatchNet &b = bm.nets[bi];
-        push_spike(b.n, __shfl(li, l), b.t0 + step, lane);
+        push_spike(b.n, __shfl(q.neuron, l), b.t0 + step, lane);
     }
 }
 
@@ -953,9 +968,8 @@ static __global__ void k_batch_bump(BatchNet *nets, int nn, uint32_t by)
 template <int NK>
 __global__ __launch_bounds__(BG_BLOCK) void k_neuron_only_batch(BatchMap bm, uint64_t step, int steps)
 {
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= bm.total) return;
-    const int i = bm.nof[w];
-    const BatchNet &b = bm.nets[i];
-    neuron_only_body<NK>(b.n, (int)(w - bm.base[i]), b.t0 + step, steps);
+    const NeuronLane q = neuron_lane(bm);
+    if (!q.valid) return;
+    const BatchNet &b = bm.nets[q.member];
+    neuron_only_body<NK>(b.n, q.neuron, b.t0 + step, steps);
 }

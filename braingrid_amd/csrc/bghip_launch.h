@@ -32,6 +32,8 @@ struct BatchMap {
     int nn;
     const uint2 *chunks;    // per-step STDP streaming pass: {member | shard << 16, first destination} runs
     uint32_t nchunks;
+    const uint2 *nwaves;    // thread-per-neuron kernels: {member, first neuron} of each wavefront
+    uint32_t nnwaves;
 };
 // Launch shape of a batch group's kernels. Every work item computes the same values whatever
 // block it lands in and however many destinations share its wavefront, so results do not depend

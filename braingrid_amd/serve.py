@@ -340,19 +340,36 @@ class Server:
         advanced = list(self.active)
         if advanced:
             # the advance call returns only when the device has taken most of the epoch's launches,
-            # so the host-side planning of queued jobs runs beside it on a worker thread (the C
-            # calls and numpy release the interpreter lock)
-            worker = threading.Thread(target=self.prepare)
+            # so the host-side work of queued jobs -- planning, then admitting those that fit and
+            # creating their networks on their own streams -- runs beside it on a worker thread
+            # (the C calls and numpy release the interpreter lock); admitted jobs join at the next
+            # tick, and the networks advanced now are the ones listed above
+            failure = []
+
+            def host_work():
+                try:
+                    self.prepare()
+                    t = self.clock()
+                    self.admit()
+                    self.phase_s["admit"] += self.clock() - t
+                except BaseException as e:   # re-raised on the scheduling thread
+                    failure.append(e)
+
+            worker = threading.Thread(target=host_work)
             worker.start()
             try:
                 self.engine.advance([j.handle for j in advanced], self.epoch_steps)
             finally:
                 worker.join()
-        c1 = self.clock()
-        self.admit()
-        c2 = self.clock()
+            if failure:
+                raise failure[0]
+            c1 = c2 = self.clock()
+        else:
+            c1 = self.clock()
+            self.admit()
+            c2 = self.clock()
+            self.phase_s["admit"] += c2 - c1
         self.phase_s["advance"] += c1 - c0
-        self.phase_s["admit"] += c2 - c1
         self.occupancy.append((c2, self.used))
         if not advanced:
             return
