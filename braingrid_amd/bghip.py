@@ -160,6 +160,7 @@ def load():
         "bghip_net_save": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "bghip_add_synapses": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
                                               ctypes.POINTER(ctypes.c_int32), _fp]),
+        "bghip_add_synapses_ex": (ctypes.c_int, [vp, ctypes.c_int64, _ip, _ip, _fp, _ip, _fp]),
         "bghip_net_get_desc": (ctypes.c_int, [vp, ctypes.POINTER(NetDesc)]),
         "bghip_recorder_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
@@ -193,7 +194,7 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_batch_destroy", "bghip_batch_set", "bghip_batch_advance", "bghip_batch_sync",
             "bghip_upload_input", "bghip_download_input", "bghip_recorder_enable", "bghip_recorder_download",
             "bghip_net_save", "bghip_net_load", "bghip_net_get_desc", "bghip_recorder_info",
-            "bghip_add_synapses", "bghip_batch_autotune", "bghip_batch_group_count", "bghip_batch_get_shape",
+            "bghip_add_synapses", "bghip_add_synapses_ex", "bghip_batch_autotune", "bghip_batch_group_count", "bghip_batch_get_shape",
             "bghip_batch_set_shape", "bghip_update_weights", "bghip_download_spikes", "bghip_clear_spike_counts")
 
 
@@ -259,15 +260,24 @@ class Net:
     def _c(self, rc, what):
         self.ctx.check(rc, what)
 
-    def add_synapses(self, src, dest, W):
-        """bghip_add_synapses: addSynapse for each src[i] -> dest[i] in order, then W[iSyn] = W[i]."""
+    def add_synapses(self, src, dest, W, total_delay=None, Wex=None):
+        """bghip_add_synapses(_ex): addSynapse for each src[i] -> dest[i] in order, then W[iSyn] = W[i]
+        and, when given, the synapse's total delay (steps) and STDP Wex."""
         s = np.ascontiguousarray(src, np.int32)
         d = np.ascontiguousarray(dest, np.int32)
         w = np.ascontiguousarray(W, np.float32)
-        if not (s.shape == d.shape == w.shape and s.ndim == 1):
-            raise ValueError("src, dest and W must be 1-D arrays of one length")
+        td = None if total_delay is None else np.ascontiguousarray(total_delay, np.int32)
+        wx = None if Wex is None else np.ascontiguousarray(Wex, np.float32)
+        if not (s.shape == d.shape == w.shape and s.ndim == 1) or any(a is not None and a.shape != s.shape
+                                                                      for a in (td, wx)):
+            raise ValueError("src, dest, W (and total_delay, Wex) must be 1-D arrays of one length")
         _i32 = ctypes.POINTER(ctypes.c_int32)
-        self._c(load().bghip_add_synapses(self.h, s.size, _ptr(s, _i32), _ptr(d, _i32), _ptr(w, _fp)), "add_synapses")
+        if td is None and wx is None:
+            self._c(load().bghip_add_synapses(self.h, s.size, _ptr(s, _i32), _ptr(d, _i32), _ptr(w, _fp)),
+                    "add_synapses")
+        else:
+            self._c(load().bghip_add_synapses_ex(self.h, s.size, _ptr(s, _i32), _ptr(d, _i32), _ptr(w, _fp),
+                                                 _ptr(td, _i32), _ptr(wx, _fp)), "add_synapses")
 
     # ---- checkpoints
     def save(self, path: str):

@@ -765,6 +765,12 @@ static CreateCtx create_ctx(const bghip_net *n)
 extern "C" int bghip_add_synapses(bghip_net *n, int64_t count, const int32_t *src, const int32_t *dest,
                                   const float *W)
 {
+    return bghip_add_synapses_ex(n, count, src, dest, W, nullptr, nullptr);
+}
+
+extern "C" int bghip_add_synapses_ex(bghip_net *n, int64_t count, const int32_t *src, const int32_t *dest,
+                                     const float *W, const int32_t *total_delay, const float *Wex)
+{
     if (!n || count < 0 || (count > 0 && (!src || !dest || !W))) return BGHIP_EINVAL;
     net_begin(n);
     if (!n->has_layout) return set_err(n->ctx, BGHIP_EINVAL, "add_synapses: layout not uploaded");
@@ -772,10 +778,18 @@ extern "C" int bghip_add_synapses(bghip_net *n, int64_t count, const int32_t *sr
     if (count == 0) return BGHIP_OK;
     // group by destination, keeping call order within each destination (counting sort)
     std::vector<int64_t> per(N + 1, 0);
+    const int dmax = std::min(255, n->dn.R - 1);
+    int dmin_new = 0;
     for (int64_t i = 0; i < count; i++) {
         if (src[i] < 0 || src[i] >= N || dest[i] < 0 || dest[i] >= N)
             return set_err(n->ctx, BGHIP_EINVAL, "add_synapses: neuron index out of range at entry %lld",
                            (long long)i);
+        if (total_delay) {
+            if (total_delay[i] < 1 || total_delay[i] > dmax)
+                return set_err(n->ctx, BGHIP_EINVAL, "add_synapses: total delay %d at entry %lld outside 1..%d",
+                               total_delay[i], (long long)i, dmax);
+            dmin_new = dmin_new ? std::min(dmin_new, (int)total_delay[i]) : (int)total_delay[i];
+        }
         per[dest[i] + 1]++;
     }
     std::vector<int32_t> dests;
@@ -785,13 +799,15 @@ extern "C" int bghip_add_synapses(bghip_net *n, int64_t count, const int32_t *sr
         per[d + 1] += per[d];
     }
     begin.push_back(count);
-    std::vector<int32_t> s2(count);
-    std::vector<float> w2(count);
+    std::vector<int32_t> s2(count), d2(total_delay ? count : 0);
+    std::vector<float> w2(count), x2(Wex ? count : 0);
     std::vector<int64_t> fill(per.begin(), per.end() - 1);
     for (int64_t i = 0; i < count; i++) {
         const int64_t k = fill[dest[i]]++;
         s2[k] = src[i];
         w2[k] = W[i];
+        if (total_delay) d2[k] = total_delay[i];
+        if (Wex) x2[k] = Wex[i];
     }
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     flush_pending(n);
@@ -799,24 +815,34 @@ extern "C" int bghip_add_synapses(bghip_net *n, int64_t count, const int32_t *sr
     char *buf = nullptr;
     const size_t o_dest = 0, o_begin = align256(sizeof(int32_t) * nd),
                  o_src = o_begin + align256(sizeof(int64_t) * (nd + 1)), o_w = o_src + align256(sizeof(int32_t) * count),
-                 bytes = o_w + sizeof(float) * count;
+                 o_d = o_w + align256(sizeof(float) * count), o_x = o_d + align256(sizeof(int32_t) * d2.size()),
+                 bytes = o_x + sizeof(float) * x2.size() + 4;
     if (hipMalloc((void **)&buf, bytes) != hipSuccess) return set_err(n->ctx, BGHIP_ENOMEM, "add_synapses: staging");
     int rc = BGHIP_OK;
     if (hipMemcpyAsync(buf + o_dest, dests.data(), sizeof(int32_t) * nd, hipMemcpyHostToDevice, n->stream) ||
         hipMemcpyAsync(buf + o_begin, begin.data(), sizeof(int64_t) * (nd + 1), hipMemcpyHostToDevice, n->stream) ||
         hipMemcpyAsync(buf + o_src, s2.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, n->stream) ||
-        hipMemcpyAsync(buf + o_w, w2.data(), sizeof(float) * count, hipMemcpyHostToDevice, n->stream)) {
+        hipMemcpyAsync(buf + o_w, w2.data(), sizeof(float) * count, hipMemcpyHostToDevice, n->stream) ||
+        (total_delay &&
+         hipMemcpyAsync(buf + o_d, d2.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, n->stream)) ||
+        (Wex && hipMemcpyAsync(buf + o_x, x2.data(), sizeof(float) * count, hipMemcpyHostToDevice, n->stream))) {
         rc = set_err(n->ctx, BGHIP_EDEVICE, "add_synapses: upload");
     } else {
         const unsigned grid = (unsigned)(((int64_t)nd * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
         hipLaunchKernelGGL(k_add_synapses, dim3(grid), dim3(BG_BLOCK), 0, n->stream, n->dn, create_ctx(n),
                            (const int32_t *)(buf + o_dest), (const int64_t *)(buf + o_begin),
-                           (const int32_t *)(buf + o_src), (const float *)(buf + o_w), nd);
+                           (const int32_t *)(buf + o_src), (const float *)(buf + o_w),
+                           total_delay ? (const int32_t *)(buf + o_d) : nullptr,
+                           Wex ? (const float *)(buf + o_x) : nullptr, nd);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(n->stream) != hipSuccess)
             rc = set_err(n->ctx, BGHIP_EDEVICE, "add_synapses: kernel");
     }
     (void)hipFree(buf);
     if (rc) return rc;
+    if (dmin_new) {   // per-synapse delays bound the fused block length like uploaded ones
+        n->min_delay_uploaded = n->min_delay_uploaded ? std::min(n->min_delay_uploaded, dmin_new) : dmin_new;
+        n->kmax = block_kmax(n);
+    }
     CHK(rebuild_incoming(n));
     return BGHIP_OK;
 }

@@ -240,9 +240,12 @@ __global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c,
 // slots 64 at a time, ballots the free ones, and lane L of a chunk takes entry
 // (entries placed so far) + (free slots below L). Entries beyond the free slots are dropped with
 // the slot-overflow error bit.
+// Optional per-entry overrides of createSynapse's per-type values: tdel (total delay in steps) and
+// wex (the STDP weight bound Wex); null keeps the per-type value.
 __global__ __launch_bounds__(BG_BLOCK) void k_add_synapses(DevNet n, CreateCtx c, const int32_t *dests,
                                                           const int64_t *begin, const int32_t *src,
-                                                          const float *w, int nd)
+                                                          const float *w, const int32_t *tdel, const float *wex,
+                                                          int nd)
 {
     const int lane = threadIdx.x & 63;
     const int q = (int)((blockIdx.x * (uint64_t)BG_BLOCK + threadIdx.x) >> 6);
@@ -261,6 +264,10 @@ __global__ __launch_bounds__(BG_BLOCK) void k_add_synapses(DevNet n, CreateCtx c
         if (free_slot && j < cnt) {
             const int s = src[b + j];
             create_synapse(n, c, base + k, s, syn_type_of(n.ntype[s], td), w[b + j]);
+            if (tdel)
+                n.meta[base + k] = (n.meta[base + k] & ~(0xFFu << BG_META_DELAY_SHIFT)) |
+                                   ((uint32_t)tdel[b + j] << BG_META_DELAY_SHIFT);
+            if (wex && (c.syn_kind == SK_STDP || c.syn_kind == SK_DYNSTDP)) n.Wex[base + k] = wex[b + j];
             top = (uint32_t)k + 1u;
         }
         placed += (int64_t)__popcll(ball);

@@ -65,9 +65,11 @@ def _random_targets(rng, N: int, k: int) -> np.ndarray:
     return t + (t >= np.arange(N)[:, None])
 
 
-def izh_stdp_arrays(N: int, seed: int, k: int = 100, dt: float = 1e-4, ring_bits: int = 256, max_rate: int = 200,
-                    epoch_s: float = 1.0):
-    """Host arrays of the static Izhikevich + STDP network (also used by tests to load the oracle)."""
+def izh_stdp_draws(N: int, seed: int, k: int = 100, dt: float = 1e-4, ring_bits: int = 256, max_rate: int = 200,
+                   epoch_s: float = 1.0):
+    """The random draws of the static Izhikevich + STDP network: the neuron arrays, and in meta the
+    synapse list (src, dst, W, total_delay, Wex) ordered per destination by ascending source, the
+    order in which addSynapse fills the destination's slots."""
     rng = np.random.default_rng(seed)
     n_exc = int(round(0.8 * N))
     ntype = np.full(N, 2, np.int32)
@@ -113,6 +115,20 @@ def izh_stdp_arrays(N: int, seed: int, k: int = 100, dt: float = 1e-4, ring_bits
     types = MD.synapse_types(dt, STDP_CFG)
     styp = np.where(ntype[src] == 1, np.where(ntype[dst] == 1, 0, 1), np.where(ntype[dst] == 1, 2, 3))
     tdel = np.array([MD.total_delay(v * 1e-3, dt) for v in range(21)])[delay_ms].astype(np.int32)
+    wex = np.where(ntype[src] == 2, F32(WEX_EXC), F32(WEX_INH)).astype(F32)
+    meta = dict(kind="izh_stdp", N=N, M=M, S=S, max_spikes=max_spikes, ring_bits=ring_bits, ntype=ntype, src=src,
+                dst=dst, slot=slot, W=w, total_delay=tdel, Wex=wex, styp=styp, indeg=indeg, types=types,
+                S_active=len(src))
+    return neurons, meta
+
+
+def izh_stdp_arrays(N: int, seed: int, k: int = 100, dt: float = 1e-4, ring_bits: int = 256, max_rate: int = 200,
+                    epoch_s: float = 1.0):
+    """Host arrays of the static Izhikevich + STDP network, every slot field as the reference's
+    addSynapse/createSynapse leave it (used by tests to load the oracle)."""
+    neurons, meta = izh_stdp_draws(N, seed, k, dt, ring_bits, max_rate, epoch_s)
+    S, slot, src, dst, styp, types = meta["S"], meta["slot"], meta["src"], meta["dst"], meta["styp"], meta["types"]
+    w, tdel, indeg = meta["W"], meta["total_delay"], meta["indeg"]
     syn = {f: np.zeros(S * (ring_bits // 32) if f == "delayQueue" else S, dt_) for f, dt_ in B.SYN_DTYPES.items()}
     syn["in_use"][slot] = 1
     syn["sourceNeuronIndex"][slot] = src
@@ -126,7 +142,7 @@ def izh_stdp_arrays(N: int, seed: int, k: int = 100, dt: float = 1e-4, ring_bits
                  ("tauneg", types.tauneg), ("Wex", types.Wex), ("mupos", types.mupos), ("muneg", types.muneg)):
         syn[f][slot] = F32(v)
     syn["useFroemkeDanSTDP"][slot] = types.useFroemkeDanSTDP
-    syn["Wex"][slot] = np.where(ntype[src] == 2, F32(WEX_EXC), F32(WEX_INH))
+    syn["Wex"][slot] = meta["Wex"]
     syn["r"][slot] = F32(types.r0)
     syn["u"][slot] = F32(types.u0)
     syn["U"][slot] = np.array(types.U, F32)[styp]
@@ -134,8 +150,6 @@ def izh_stdp_arrays(N: int, seed: int, k: int = 100, dt: float = 1e-4, ring_bits
     syn["F"][slot] = np.array(types.F, F32)[styp]
     syn["lastSpike"][:] = np.iinfo(np.uint64).max
     syn["synapse_counts"] = indeg.astype(np.uint32)
-    meta = dict(kind="izh_stdp", N=N, M=M, max_spikes=max_spikes, ring_bits=ring_bits, ntype=ntype, src=src,
-                dst=dst, slot=slot, W=w, total_delay=tdel, types=types, S_active=len(src))
     return neurons, syn, meta
 
 
@@ -146,10 +160,16 @@ def izh_stdp_desc(meta: dict, seed: int) -> B.NetDesc:
 
 def izh_stdp(ctx: B.Context, N: int, seed: int, k: int = 100, ring_bits: int = 256, max_rate: int = 200,
              epoch_s: float = 1.0, arrays=None):
-    """Create the network on the device; `arrays` may hold izh_stdp_arrays' result computed earlier."""
-    neurons, syn, meta = arrays or izh_stdp_arrays(N, seed, k, ring_bits=ring_bits, max_rate=max_rate,
-                                                   epoch_s=epoch_s)
+    """Create the network on the device; `arrays` may hold izh_stdp_draws' result computed earlier.
+    The synapses are created on the device from their list (bghip_add_synapses_ex: addSynapse in
+    list order, createSynapse's per-type values, then the synapse's W, total delay and Wex), so the
+    slots equal izh_stdp_arrays' without building them on the host. The neuron types reach the
+    device as a layout (synapse types follow from them; the coordinates are not used: there is no
+    growth)."""
+    neurons, meta = arrays or izh_stdp_draws(N, seed, k, ring_bits=ring_bits, max_rate=max_rate, epoch_s=epoch_s)
     net = B.Net(ctx, izh_stdp_desc(meta, seed))
     net.upload_neurons(neurons)
-    net.upload_synapses(syn, 0)
+    n = meta["N"]
+    net.upload_layout(np.arange(n, dtype=F32), np.zeros(n, F32), meta["ntype"].astype(np.int32))
+    net.add_synapses(meta["src"], meta["dst"], meta["W"], total_delay=meta["total_delay"], Wex=meta["Wex"])
     return net, meta

@@ -90,8 +90,8 @@ def plan(spec: dict) -> dict:
         max_spikes = int(np.float32(EPOCH_S) * np.float32(LIF_MAX_RATE))
         desc = MD.net_desc(N, LIF_M, B.NEURON_LIF, B.SYN_DS, max_spikes, DT, spec["seed"])
         return dict(kind=kind, desc=desc, arrays=None)
-    arrays = NW.izh_stdp_arrays(N, spec["seed"], k=min(100, N - 1), max_rate=IZH_MAX_RATE, epoch_s=EPOCH_S)
-    return dict(kind=kind, desc=NW.izh_stdp_desc(arrays[2], spec["seed"]), arrays=arrays)
+    arrays = NW.izh_stdp_draws(N, spec["seed"], k=min(100, N - 1), max_rate=IZH_MAX_RATE, epoch_s=EPOCH_S)
+    return dict(kind=kind, desc=NW.izh_stdp_desc(arrays[1], spec["seed"]), arrays=arrays)
 
 
 def footprint(p: dict) -> int:

@@ -201,6 +201,12 @@ int bghip_download_input(bghip_net *net, bghip_input_state *state);
  * destination are dropped and set BGHIP_ERR_SLOT_OVERFLOW. Needs bghip_upload_layout first;
  * blocks. */
 int bghip_add_synapses(bghip_net *net, int64_t count, const int32_t *src, const int32_t *dest, const float *W);
+/* The same with per-synapse values a connection rule sets after addSynapse, as the reference's
+ * per-synapse arrays allow (total_delay, Synapses/AllSpikingSynapses.h:360; Wex,
+ * Synapses/AllSTDPSynapses.h:380): total_delay[i] in steps (1 .. min(255, ring_bits - 1); it also bounds the fused block
+ * length) and, for STDP synapses, Wex[i]; either may be NULL for the per-type value. */
+int bghip_add_synapses_ex(bghip_net *net, int64_t count, const int32_t *src, const int32_t *dest, const float *W,
+                          const int32_t *total_delay, const float *Wex);
 
 /* Checkpoint of the whole network state, device and host side, as a versioned binary file
  * (Model::serialize / deserialize, Core/Model.cpp:62-98, whose round trip the reference breaks:

@@ -58,3 +58,39 @@ def test_izh_dynstdp_multi_epoch_full_rings(hip_ctx):
     finally:
         net.close()
         ora.close()
+
+
+def test_device_built_network_equals_uploaded_slots(hip_ctx):
+    """The product builder creates the synapses on the device from their list
+    (bghip_add_synapses_ex); the same network uploaded slot by slot (bghip_upload_synapses of
+    izh_stdp_arrays) has identical slot arrays, and out-of-range per-synapse delays are refused."""
+    import numpy as np
+    from braingrid_amd import bghip as B
+    N, k, seed = 400, 40, 13
+    neu, syn, meta = NW.izh_stdp_arrays(N, seed, k=k)
+    a, _ = NW.izh_stdp(hip_ctx, N, seed, k=k)
+    b = B.Net(hip_ctx, NW.izh_stdp_desc(meta, seed))
+    try:
+        b.upload_neurons(neu)
+        b.upload_synapses(syn, 0)
+        sa, sb = a.download_synapses(), b.download_synapses()
+        # slot fields are defined for synapses in use (the reference's per-slot arrays hold
+        # uninitialised memory until createSynapse/resetSynapse writes a slot)
+        use = np.asarray(sb["in_use"]).astype(bool)
+        assert np.array_equal(np.asarray(sa["in_use"]), np.asarray(sb["in_use"])) and use.sum() == meta["S_active"]
+        bad = []
+        for f in sb:
+            x, y = np.asarray(sa[f]), np.asarray(sb[f])
+            if x.shape == use.shape:
+                x, y = x[use], y[use]
+            if not np.array_equal(x.view(np.uint8), y.view(np.uint8)):
+                bad.append(f)
+        assert not bad, bad
+        assert a.total_synapses == b.total_synapses == meta["S_active"]
+        one = np.zeros(1, np.int32)
+        for d in (0, 256):
+            with pytest.raises(B.BgHipError):
+                a.add_synapses(one, one + 1, np.ones(1, np.float32), total_delay=np.full(1, d, np.int32))
+    finally:
+        a.close()
+        b.close()
