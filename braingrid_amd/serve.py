@@ -28,6 +28,7 @@ import shutil
 import tempfile
 import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -148,6 +149,8 @@ class Server:
     spill_dir: str | None = None         # where checkpoints of evicted jobs go (a temporary folder)
     clock: object = time.perf_counter
     log: object = None
+    plan_threads: int = 8                # queued jobs planned in parallel (numpy releases the lock)
+    plan_ahead: int = 128                # queued jobs planned per tick
     jobs: dict = field(default_factory=dict)
     queue: list = field(default_factory=list)      # heap of (arrival, id)
     active: list = field(default_factory=list)     # admitted jobs in admission order
@@ -204,22 +207,32 @@ class Server:
         if self.log:
             self.log(f"job {job.id} {state}: {err}")
 
-    def prepare(self, limit: int = 8):
-        """Plans (host-side setup of) up to `limit` queued jobs in arrival order."""
-        done = 0
+    def prepare(self, limit: int | None = None):
+        """Plans (host-side setup of) up to `limit` (default plan_ahead) queued jobs in arrival
+        order, on up to plan_threads threads."""
+        limit = self.plan_ahead if limit is None else limit
+        todo = []
         for arrival, jid in sorted(self.queue)[: 4 * limit]:
             job = self.jobs[jid]
-            if job.state != QUEUED or job.plan is not None:
-                continue
+            if job.state == QUEUED and job.plan is None and job.error is None:
+                todo.append(job)
+                if len(todo) >= limit:
+                    break
+
+        def plan_one(job):
             try:
-                job.plan = self.engine.plan(job.spec)
-                job.footprint = int(self.engine.footprint(job.plan))
+                plan = self.engine.plan(job.spec)
+                return plan, int(self.engine.footprint(plan)), None
             except Exception as e:
-                job.plan = None
-                job.error = f"{type(e).__name__}: {e}"
-            done += 1
-            if done >= limit:
-                break
+                return None, 0, f"{type(e).__name__}: {e}"
+
+        if len(todo) > 1 and self.plan_threads > 1:
+            with ThreadPoolExecutor(max_workers=min(self.plan_threads, len(todo))) as ex:
+                results = list(ex.map(plan_one, todo))
+        else:
+            results = [plan_one(j) for j in todo]
+        for job, (plan, fp, err) in zip(todo, results):
+            job.plan, job.footprint, job.error = plan, fp, err
 
     def admit(self):
         """Admits queued jobs in arrival order while they fit; up to `lookahead` jobs that do not
