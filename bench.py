@@ -76,6 +76,71 @@ def cpu_baseline(budget_s: float):
                       f"{dt:.1f} s"}
 
 
+def oracle_sweep_net(spec: dict):
+    """The oracle in the reference's mode (Norm noise, C library math, sequential summation) holding
+    the network of one sweep or served job specification."""
+    from oracle import oracle_py as O
+    from braingrid_amd import networks as NW
+    from braingrid_amd import sweep as SW
+    if spec["kind"] == "lif_growth":
+        c = O.default_config()
+        c.update(width=spec["side"], height=spec["side"], layout="DynamicLayout", frac_exc=spec["fE"],
+                 frac_starter=0.1, target_rate=spec["tR"], seed=spec["seed"], epoch_duration=SW.EPOCH_S,
+                 num_epochs=int(round(spec["sim_s"] / SW.EPOCH_S)), max_synapses_per_neuron=SW.LIF_M,
+                 max_firing_rate=SW.LIF_MAX_RATE)
+        return O.OracleNet(c)
+    from tests import helpers as H
+    N = spec["side"] ** 2
+    neu, syn, meta = NW.izh_stdp_arrays(N, spec["seed"], k=min(100, N - 1), max_rate=SW.IZH_MAX_RATE,
+                                        epoch_s=SW.EPOCH_S)
+    ora, _ = H.izh_static_oracle(neu, syn, meta, spec["seed"], epoch_s=SW.EPOCH_S, max_rate=SW.IZH_MAX_RATE,
+                                 noise_mode=0, math_mode=0, sum_mode=0,
+                                 input=dict(kind="poisson", fr_mean=SW.POISSON_HZ, weight=SW.POISSON_WEIGHT))
+    return ora
+
+
+def forms_cpu_baseline(specs: list, sample: list, steps: int, threads: int) -> dict:
+    """CPU baseline of a many-network workload: every sampled network in the oracle (reference
+    mode, one thread each, `threads` at a time) runs `steps` steps and one epoch-end update; its
+    CPU time per simulated second is steps scaled to a 1 s epoch plus that update. The workload's
+    CPU time is the sample's scaled to all networks per family by neuron-seconds (neurons x
+    simulated seconds; the Izhikevich networks have ~100 synapses per neuron), and its rate on
+    `threads` cores assumes one network per core with perfect packing."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def cost(spec):
+        ora = oracle_sweep_net(spec)
+        t0 = time.thread_time()
+        ora.advance(steps)
+        t1 = time.thread_time()
+        ora.update_connections()
+        ora.update_history()
+        t2 = time.thread_time()
+        ora.close()
+        return (t1 - t0) * (1.0 / (steps * 1e-4)) + (t2 - t1)   # core-seconds per simulated second
+
+    t_start = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        costs = list(ex.map(cost, sample))
+    wall = time.perf_counter() - t_start
+    def nsec(s):
+        return s["side"] ** 2 * s["sim_s"]
+
+    core_s_all = 0.0
+    for kind in {s["kind"] for s in specs}:
+        pop = sum(nsec(s) for s in specs if s["kind"] == kind)
+        got = [(c, s) for c, s in zip(costs, sample) if s["kind"] == kind]
+        if not got:
+            raise RuntimeError(f"CPU baseline sample has no {kind} network")
+        core_s_all += sum(c * s["sim_s"] for c, s in got) * pop / sum(nsec(s) for _, s in got)
+    sim_all = sum(s["sim_s"] for s in specs)
+    return {"value": sim_all / (core_s_all / threads) * 3600.0, "unit": "sim-s/h", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (reference algorithm: Norm noise, sequential summation, Poisson input), "
+                      f"{len(sample)} of {len(specs)} networks, {steps} steps and one epoch-end update each, one "
+                      f"network per thread on {threads} threads ({wall:.1f} s); the workload's core-seconds scaled "
+                      f"from the sample per family by neuron-seconds"}
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "fused_block_10k_traffic.json")
     if not os.path.exists(p):
@@ -157,6 +222,10 @@ def batched_result(args):
         "launch_shapes": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in s.items()} for s in shapes],
     }
     ctx.close()
+    if not args.no_cpu:
+        cs = specs[:: max(1, len(specs) // max(1, args.cpu_networks))]
+        out["cpu_baseline"] = forms_cpu_baseline(specs, cs, args.cpu_steps, args.cpu_threads)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     return out
 
 
@@ -207,6 +276,12 @@ def served_result(args):
     out.update(alone)
     if alone.get("alone_sim_seconds_per_hour"):
         out["speedup_vs_alone"] = st["sim_seconds_per_hour"] / alone["alone_sim_seconds_per_hour"]
+    if not args.no_cpu:
+        specs = [j.spec for j in sorted(jobs, key=lambda j: j.id) if j.state == S.DONE]
+        if specs:
+            cs = specs[:: max(1, len(specs) // max(1, args.cpu_networks))]
+            out["cpu_baseline"] = forms_cpu_baseline(specs, cs, args.cpu_steps, args.cpu_threads)
+            out["speedup_vs_cpu"] = st["sim_seconds_per_hour"] / out["cpu_baseline"]["value"]
     return out
 
 
@@ -255,6 +330,9 @@ def forms_summary(args):
                                             "alone_wall_s", "alone_sample", "identical_results",
                                             "networks_with_device_errors")}
         out["batched"]["workload"] = r["config"]["workload"]
+        for k in ("cpu_baseline", "speedup_vs_cpu"):
+            if k in r:
+                out["batched"][k] = r[k]
         out["batched"]["launch_shapes"] = [{k: s[k] for k in ("mode", "networks", "block", "dests_per_wave",
                                                               "block_stream", "syn_blocks_per_shard",
                                                               "sum_blocks_per_shard", "tuned", "step_us")}
@@ -268,7 +346,8 @@ def forms_summary(args):
         out["served"] = {k: r[k] for k in ("value", "unit", "latency_p50_s", "latency_p99_s", "latency_mean_s",
                                            "sim_seconds_per_hour", "hbm_occupancy_mean", "hbm_occupancy_max",
                                            "wall_s", "counts", "alone_sim_seconds_per_hour", "alone_sample",
-                                           "speedup_vs_alone", "identical_results") if k in r}
+                                           "speedup_vs_alone", "identical_results", "cpu_baseline",
+                                           "speedup_vs_cpu") if k in r}
         out["served"]["workload"] = r["config"]["workload"]
     except Exception as e:
         out["served"] = {"error": repr(e)}
@@ -299,6 +378,12 @@ def main():
     ap.add_argument("--roofline-steps", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-networks", type=int, default=12,
+                    help="batched/served forms: networks of the workload timed in the CPU oracle")
+    ap.add_argument("--cpu-steps", type=int, default=200,
+                    help="batched/served forms: oracle steps per sampled network (plus one epoch-end update)")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="batched/served forms: host threads of the CPU baseline (one network each)")
     ap.add_argument("--no-forms", action="store_true",
                     help="single form: skip the batched and served summaries added to the headline line")
     ap.add_argument("--forms-alone-stride", type=int, default=16,

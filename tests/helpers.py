@@ -164,7 +164,7 @@ def compare_state(ora: O.OracleNet, net: B.Net, cfg: dict, where: str, check_map
 
 
 def izh_static_oracle(neurons: dict, syn: dict, meta: dict, seed: int, epoch_s: float = 1.0,
-                      max_rate: int = 200) -> tuple[O.OracleNet, dict]:
+                      max_rate: int = 200, **overrides) -> tuple[O.OracleNet, dict]:
     """An oracle network holding the static Izhikevich + DynamicSTDP network of
     braingrid_amd.networks.izh_stdp_arrays, built through the oracle's own addSynapse/createSynapse.
     Neuron constants drawn by the oracle's setup are replaced with the builder's; W, Wex and the
@@ -179,6 +179,7 @@ def izh_static_oracle(neurons: dict, syn: dict, meta: dict, seed: int, epoch_s: 
                seed=seed, ring_bits=meta["ring_bits"], stdp=NW.STDP_CFG, noise_mode=1, math_mode=1, sum_mode=1,
                Iinject=NW.IZH_NEURON["Iinject"], Inoise=NW.IZH_NEURON["Inoise"], Vthresh=NW.IZH_NEURON["Vthresh"],
                Vresting=NW.IZH_NEURON["Vresting"], Vreset=NW.IZH_NEURON["Vreset"], Vinit=NW.IZH_NEURON["Vinit"])
+    cfg.update(overrides)   # e.g. the reference's noise/math/summation modes, stimulus input
     ora = O.OracleNet(cfg)
     ora.setup_consts = {f: ora.array(f).copy() for f in ("C1", "C2", "C3", "I0")}
     for f in ("Vm", "Vthresh", "Vreset", "I0", "Inoise", "C1", "C2", "Trefract", "Aconst", "Bconst", "Cconst",
