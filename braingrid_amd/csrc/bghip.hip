@@ -1485,6 +1485,7 @@ struct bghip_batch {
     std::vector<bghip_net *> nets;
     std::vector<BatchGroup> groups;
     std::vector<int> modes;        // net_mode of each member when the groups were formed
+    std::vector<int> kmaxs;        // and its block length (a group's is the minimum of its members')
 };
 
 static void free_groups(bghip_batch *b)
@@ -1577,10 +1578,12 @@ static int regroup(bghip_batch *b)
     free_groups(b);
     const int n = (int)b->nets.size();
     b->modes.assign(n, 0);
+    b->kmaxs.assign(n, 0);
     for (int i = 0; i < n; i++) {
         bghip_net *x = b->nets[i];
         const int mode = net_mode(x);
         b->modes[i] = mode;
+        b->kmaxs[i] = x->kmax;
         const int sk = mode == BG_MODE_QUIET ? -1 : x->desc.synapse_model;
         const int w32 = mode == BG_MODE_QUIET ? 0 : x->dn.W32;
         BatchGroup *grp = nullptr;
@@ -1677,6 +1680,14 @@ static int regroup(bghip_batch *b)
         HIP_TRY(b->ctx, hipMemcpy(g.d_sbase, sbase.data(), sizeof(uint64_t) * (nn + 1), hipMemcpyHostToDevice));
     }
     return BGHIP_OK;
+}
+
+// True if a member changed mode or block length since the groups were formed.
+static bool members_changed(const bghip_batch *b)
+{
+    for (size_t i = 0; i < b->nets.size(); i++)
+        if (net_mode(b->nets[i]) != b->modes[i] || b->nets[i]->kmax != b->kmaxs[i]) return true;
+    return false;
 }
 
 static BatchMap batch_map(const BatchGroup &g, const BatchShape &sh)
@@ -1829,12 +1840,9 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
         HIP_TRY(b->ctx, hipEventRecord(x->ext_ev, x->stream));
         HIP_TRY(b->ctx, hipStreamWaitEvent(b->stream, x->ext_ev, 0));
     }
-    // a growth update or an upload may have moved a member to another mode
-    for (size_t i = 0; i < b->nets.size(); i++)
-        if (net_mode(b->nets[i]) != b->modes[i]) {
-            CHK(regroup(b));
-            break;
-        }
+    // a growth update or an upload may have moved a member to another mode, and new synapses with
+    // shorter delays may have shortened its blocks
+    if (members_changed(b)) CHK(regroup(b));
     for (auto &g : b->groups) {
         const int nn = (int)g.members.size();
         if (g.staged_pending) {   // the previous call's copies out of the staging buffers
@@ -1969,11 +1977,7 @@ extern "C" int bghip_batch_group_count(bghip_batch *b)
 {
     if (!b) return BGHIP_EINVAL;
     // a growth update or an upload since the last call may have moved a member to another mode
-    for (size_t i = 0; i < b->nets.size(); i++)
-        if (net_mode(b->nets[i]) != b->modes[i]) {
-            CHK(regroup(b));
-            break;
-        }
+    if (members_changed(b)) CHK(regroup(b));
     return (int)b->groups.size();
 }
 

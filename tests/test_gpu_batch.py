@@ -222,3 +222,54 @@ def test_stdp_rows_growing_after_the_group_formed(hip_ctx):
     for n in nets:
         n.close()
     assert not bad, bad[:10]
+
+
+def test_batch_follows_a_member_whose_blocks_shorten(hip_ctx):
+    """A fused (Spiking/DS) member gains a synapse with a 3-step delay between batch advances
+    (bghip_add_synapses_ex); its block length drops from 9 to 3 steps and the batch re-forms its
+    groups, so no spike is delivered inside the block it was fired in: the member equals the
+    oracle, which gets the same synapse through addSynapse and the same total delay."""
+    cfg = H.load_config("conn-izh-ds")   # active: every neuron fires several times in 0.5 s
+    ora = O.OracleNet(cfg)
+    net = H.gpu_net_from_oracle(hip_ctx, ora, cfg)
+    other_cfg = H.load_config("conn-lif-ds")
+    other_ora = O.OracleNet(other_cfg)
+    other = H.gpu_net_from_oracle(hip_ctx, other_ora, other_cfg)
+    batch = B.Batch(hip_ctx)
+    try:
+        for o, n in ((ora, net), (other_ora, other)):
+            o.advance(400)
+            n.advance(400)
+            o.update_connections()
+            n.update_connections()
+        batch.set([net, other])
+        batch.advance(500)
+        for o in (ora, other_ora):
+            o.advance(500)
+        assert any(g["mode"] == 1 for g in batch.shapes())
+        M = cfg["max_synapses_per_neuron"]
+        src = ora.array("sourceNeuronIndex")
+        have = {int(src[k]) for k in range(M) if ora.array("in_use")[k]}   # sources of neuron 0
+        fired = ora.array("spikeCount")
+        s = max((i for i in range(1, cfg["width"] * cfg["height"]) if i not in have), key=lambda i: fired[i])
+        assert fired[s] > 0
+        w = np.float32(2.0e-7)
+        net.add_synapses(np.array([s]), np.array([0]), np.array([w]), total_delay=np.array([3]))
+        slot = ora.add_synapse(s, 0, float(w))
+        ora.view("total_delay")[slot] = 3
+        ora.rebuild_index_map()
+        bad = []
+        for rnd, chunk in enumerate((600, 1300)):
+            for o in (ora, other_ora):
+                o.advance(chunk)
+            batch.advance(chunk)
+            net.sync()
+            other.sync()
+            bad += H.compare_state(ora, net, cfg, f"short-delay-{rnd}", check_maps=False)
+            bad += H.compare_state(other_ora, other, other_cfg, f"other-{rnd}", check_maps=False)
+        assert ora.array("spikeCount").sum() > 0
+        assert not bad, bad[:10]
+    finally:
+        batch.close()
+        net.close()
+        other.close()
