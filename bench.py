@@ -11,7 +11,7 @@ pre-grown: every radius is set to 8.9 and one growth update runs on the GPU, whi
 boundary is crossed.
 
 Output: one JSON line. `value` = synapse-updates/s = sum over timed steps of active synapses /
-wall time. `roofline` prices the dominant kernel (the fused block step kernel) with the survey's
+wall time. `roofline` prices the dominant kernel (the fused block step kernel) with the per-unit
 algorithmic bytes (24 B per DS synapse-step + 44 B per LIF neuron-step) over its average launch
 duration, measured with HIP events on the network's stream. `cpu_baseline` times the CPU oracle
 (the reference's single-threaded algorithm restated in C) on a bounded sample of the same network.
@@ -403,8 +403,9 @@ def main():
         print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "copies": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs one network per GPU on a one-GPU box; "
-                                   "--gpus above 1 is not supported (see DESIGN.md for how it would shard)"}))
+        print(json.dumps({"error": "this bench measures one GPU: --gpus above 1 is not supported (the batched "
+                                   "and served forms would shard whole networks across GPUs; one network would "
+                                   "split by destination neuron and exchange fire masks once per block)"}))
         sys.exit(2)
     if args.form == "batched":
         batched_main(args)

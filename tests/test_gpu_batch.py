@@ -122,7 +122,7 @@ def test_autotuner_tunes_every_group_and_keeps_results(hip_ctx):
     batch.set(nets)
     bad = []
     modes = set()
-    for rnd, chunk in enumerate((1000, 1500)):   # round 0: no synapses yet (neuron-only groups)
+    for rnd, chunk in enumerate((1000, 1500)):   # first chunk: no synapses yet (neuron-only groups)
         for o in oras:
             o.advance(chunk)
         batch.advance(chunk)
