@@ -39,3 +39,17 @@ def test_static_connection_parameters():
     assert c["conn"] == "ConnStatic" and p.conn_kind == host.CONN_KINDS["ConnStatic"]
     assert p.static_nconns == 14 and abs(p.static_thresh - 3.0) < 1e-7
     assert tuple(p.static_exc_w) == pytest.approx((0.0, 2e-7)) and tuple(p.static_inh_w) == pytest.approx((-1e-7, 0.0))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and a JSON message, before
+    touching any device."""
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
