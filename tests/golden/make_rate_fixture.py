@@ -11,11 +11,15 @@ realisations of one network.
 
 Stored: `counts` (uint16, seeds x epochs x neurons), every neuron's spike count per epoch, taken
 from the ConnGrowth rates after each growth update (rate = count / epoch duration,
-Connections/ConnGrowth.cpp:295-310), and `seeds`.
+Connections/ConnGrowth.cpp:295-310); `burst` (uint32, seeds x 1 s bins), the recorder's
+burstinessHist (Recorders/XmlRecorder.cpp:71-104), and `radii` (float32, seeds x neurons), the
+final row of radiiHistory (Recorders/XmlGrowthRecorder.cpp:72-97), both from the recorder output
+the harness writes; and `seeds`.
 
     python tests/golden/make_rate_fixture.py
 """
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -56,7 +60,14 @@ def config_xml(seed: int) -> str:
     return x
 
 
-def run_seed(seed: int) -> np.ndarray:
+def xml_matrix(text: str, name: str) -> np.ndarray:
+    """The values of <Matrix name="name" ...> in the recorder's XML output."""
+    m = re.search(r'<Matrix name="%s"[^>]*rows="(\d+)" columns="(\d+)"[^>]*>(.*?)</Matrix>' % name, text, re.S)
+    rows, cols = int(m.group(1)), int(m.group(2))
+    return np.array(m.group(3).split(), np.float64).reshape(rows, cols)
+
+
+def run_seed(seed: int):
     exe = os.path.join(O.REF_DIR, "bgref_dump")
     with tempfile.TemporaryDirectory() as td:
         cfg = os.path.join(td, "rates.xml")
@@ -66,20 +77,29 @@ def run_seed(seed: int) -> np.ndarray:
         subprocess.run([exe, cfg, dump, "--dynamic-layout", "0.9", "0.1", "--no-history"], cwd=td, check=True,
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
         recs = O.read_dump(dump)
+        with open(dump + ".xml") as f:
+            xml = f.read()
     rows = [d["rates"] for tag, ep, d in recs if tag == "grown"]
     assert len(rows) == EPOCHS
     counts = np.rint(np.array(rows, np.float64) * EPOCH_S)
     assert counts.max() < 65536
-    return counts.astype(np.uint16)
+    burst = xml_matrix(xml, "burstinessHist")[0]
+    radii = xml_matrix(xml, "radiiHistory")[-1]
+    assert burst.shape == (int(EPOCHS * EPOCH_S),) and radii.shape == (SIDE * SIDE,)
+    return counts.astype(np.uint16), np.rint(burst).astype(np.uint32), radii.astype(np.float32)
 
 
 def main():
     if not os.path.exists(os.path.join(O.REF_DIR, "bgref_dump")):
         sys.exit("oracle/_ref/bgref_dump missing: make -f oracle/Makefile.ref")
     with ThreadPoolExecutor(len(SEEDS)) as ex:
-        counts = np.array(list(ex.map(run_seed, SEEDS)))
-    np.savez_compressed(OUT, counts=counts, seeds=np.array(SEEDS, np.int32))
-    print(OUT, counts.shape, "spikes per seed", counts.sum(axis=(1, 2)), os.path.getsize(OUT), "bytes")
+        res = list(ex.map(run_seed, SEEDS))
+    counts = np.array([r[0] for r in res])
+    burst = np.array([r[1] for r in res])
+    radii = np.array([r[2] for r in res])
+    np.savez_compressed(OUT, counts=counts, burst=burst, radii=radii, seeds=np.array(SEEDS, np.int32))
+    print(OUT, counts.shape, "spikes per seed", counts.sum(axis=(1, 2)), "burst totals", burst.sum(axis=1),
+          os.path.getsize(OUT), "bytes")
 
 
 if __name__ == "__main__":

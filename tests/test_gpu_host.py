@@ -247,6 +247,26 @@ def test_host_model_with_stimulus_input_matches_oracle(hip_ctx, tmp_path, name):
         ora.close()
 
 
+def _burst_and_radii_agree(g_burst, r_burst, g_radii, r_radii, g_counts):
+    """The recorder's burstinessHist (spikes per 1 s bin) and final radii of the eight seeds, against
+    the compiled reference's. Tolerances: the burst totals equal the spike counts on each side;
+    summed over the seeds, the whole histogram within 1% and every 1 s bin within 4% (a bin's
+    eight-run sum varies by about 0.8% between independent noise streams, so 4% is a bound a
+    mis-binned or missing second breaks and the noise does not); the mean final radius within 1e-5
+    relative (the reference's seed-to-seed spread is about 1.3e-6)."""
+    assert np.array_equal(g_burst.sum(axis=1), g_counts.sum(axis=(1, 2)))
+    gb, rb = g_burst.sum(axis=0), r_burst.sum(axis=0)
+    per_bin = (gb - rb) / rb
+    print(f"[rates] burstinessHist: total {(gb.sum() - rb.sum()) / rb.sum() * 100:.3f}%, per 1 s bin "
+          f"{np.round(per_bin * 100, 2).tolist()}%", flush=True)
+    assert abs(gb.sum() - rb.sum()) / rb.sum() < 0.01
+    assert np.all(np.abs(per_bin) < 0.04)
+    rel = (g_radii.mean() - r_radii.mean()) / r_radii.mean()
+    print(f"[rates] mean final radius {g_radii.mean():.7f} vs reference {r_radii.mean():.7f} ({rel:.2e} relative)",
+          flush=True)
+    assert abs(rel) < 1e-5
+
+
 def test_firing_rates_within_one_percent_of_compiled_reference(hip_ctx):
     """Firing rates against the compiled reference's CPU path with its own noise stream. Workload:
     the first two 10 s epochs of the 10,000-neuron growth network (100 x 100, test-small parameter
@@ -264,6 +284,8 @@ def test_firing_rates_within_one_percent_of_compiled_reference(hip_ctx):
     ref, seeds = fx["counts"].astype(np.float64), [int(s) for s in fx["seeds"]]
     S, E, N = ref.shape
     gpu = np.zeros_like(ref)
+    g_burst = np.zeros(fx["burst"].shape)
+    g_radii = np.zeros(fx["radii"].shape)
     for k, s in enumerate(seeds):
         print(f"[rates] seed {s}", flush=True)
         c = host.base_config()
@@ -273,7 +295,10 @@ def test_firing_rates_within_one_percent_of_compiled_reference(hip_ctx):
         for _ in range(E):
             m.run_epoch()
         gpu[k] = np.rint(m.array("ratesHistory")[N:(E + 1) * N].astype(np.float64).reshape(E, N) * 10.0)
+        g_burst[k] = m.array("burstinessHist")
+        g_radii[k] = m.array("radiiHistory")[E * N:(E + 1) * N]
         m.close()
+    _burst_and_radii_agree(g_burst, fx["burst"].astype(np.float64), g_radii, fx["radii"].astype(np.float64), gpu)
     g_tot, r_tot = gpu.sum(axis=(1, 2)), ref.sum(axis=(1, 2))
     rel = (g_tot.mean() - r_tot.mean()) / r_tot.mean()
     spread = np.std(r_tot / r_tot.mean(), ddof=1)

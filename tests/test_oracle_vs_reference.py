@@ -208,6 +208,10 @@ def test_rate_fixture_is_the_oracles_reference_mode():
     ora = O.OracleNet(c)
     ora.run_epoch()
     got = np.rint(ora.array("ratesHistory")[10000:20000].astype(np.float64) * 10.0)
+    # 1 s bins 0..8 are complete after the first epoch (bin 9 also receives step 100,000's spikes:
+    # 100000 * dt, dt = float 1e-4 widened to double, is just below 10)
+    burst = ora.array("burstinessHist")[:9].astype(np.int64)
     assert ora.total_synapses == 0
     ora.close()
     assert np.array_equal(got, counts[0, 0])
+    assert np.array_equal(burst, fx["burst"][0, :9])
