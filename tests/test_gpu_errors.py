@@ -1,0 +1,88 @@
+"""Capacity limits: where the reference asserts (a full spike history,
+Neurons/AllSpikingNeurons.cpp:112; a full row, Synapses/AllSynapses.cpp:231) or returns silently and
+then writes W through an uninitialised index (Synapses/AllSynapses.cpp:359-362,
+Connections/ConnGrowth.cpp:451-453), the device sets an error bit, keeps every write inside its
+arrays, and bghip_sync reports the bit. The oracle stops with an error at the same point. Both
+conditions are forced from the host through the network's parameters; every kernel ends normally."""
+import numpy as np
+import pytest
+
+from braingrid_amd import bghip as B
+from oracle import oracle_py as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def test_spike_history_overflow_is_reported_at_the_oracles_step(hip_ctx):
+    """max_spikes = 3 on the active Izhikevich network: the device reports error bit 2 on exactly the
+    step whose spike makes some neuron's count reach max_spikes, and every state array before that
+    step equals the oracle's."""
+    cfg = H.load_config("conn-izh-ds")
+    cfg["max_firing_rate"] = round(3.0 / cfg["epoch_duration"])
+    assert H.max_spikes(cfg) == 3
+    ora = O.OracleNet(cfg)
+    net = H.gpu_net_from_oracle(hip_ctx, ora, cfg)
+    probe = O.OracleNet(cfg)
+    k = 0
+    try:
+        while k < H.steps_per_epoch(cfg):
+            k += 1
+            probe.advance(1)
+    except RuntimeError as e:
+        assert "spike history overflow" in str(e)
+    else:
+        pytest.fail("no neuron reached max_spikes within one epoch")
+    probe.close()
+    print(f"[errors] oracle stops at step {k}", flush=True)
+    try:
+        ora.advance(k - 1)
+        net.advance(k - 1)
+        net.sync()
+        assert net.error_bits() == 0
+        bad = H.compare_state(ora, net, cfg, "before-overflow", check_maps=False)
+        assert not bad, bad[:10]
+        net.advance(1)
+        with pytest.raises(B.BgHipError, match="device error bits 0x2"):
+            net.sync()
+        assert net.error_bits() == 2
+    finally:
+        net.close()
+        ora.close()
+
+
+def test_full_row_is_reported(hip_ctx):
+    """Rows of M = 8: a first call fills destination 0 to M - 3 and gives destination 1 three
+    synapses, in first-free-slot order exactly as the oracle's addSynapse places them; a second call
+    gives destination 0 five more, of which three fit, and the device reports error bit 4 where the
+    oracle refuses the (M + 1)-th. The bit stays set: later calls on the network report it too."""
+    cfg = H.load_config("conn-lif-ds")
+    cfg["max_synapses_per_neuron"] = 8
+    N, M = cfg["width"] * cfg["height"], cfg["max_synapses_per_neuron"]
+    assert N > M + 3
+    ora = O.OracleNet(cfg)
+    net = H.gpu_net_from_oracle(hip_ctx, ora, cfg)
+    try:
+        src = np.r_[np.arange(1, M - 2), [5, 9, 13]].astype(np.int32)
+        dst = np.r_[np.zeros(M - 3), np.ones(3)].astype(np.int32)
+        w = np.linspace(1e-7, 2e-7, src.size, dtype=np.float32)
+        net.add_synapses(src, dst, w)
+        for s, d, x in zip(src, dst, w):
+            ora.add_synapse(int(s), int(d), float(x))
+        net.sync()
+        bad = H.compare_state(ora, net, cfg, "rows-filled", check_maps=False)
+        assert not bad, bad[:10]
+        more = np.arange(M - 2, M + 3, dtype=np.int32)
+        net.add_synapses(more, np.zeros(5, np.int32), np.full(5, 3e-7, np.float32))
+        for s in more[:3]:
+            ora.add_synapse(int(s), 0, 3e-7)
+        with pytest.raises(RuntimeError, match="no free synapse slot"):
+            ora.add_synapse(int(more[3]), 0, 3e-7)
+        with pytest.raises(B.BgHipError, match="device error bits 0x4"):
+            net.sync()
+        assert net.error_bits() == 4
+        with pytest.raises(B.BgHipError, match="device error bits 0x4"):
+            net.download_synapses()
+    finally:
+        net.close()
+        ora.close()
