@@ -136,31 +136,41 @@ def test_spike_download_and_clear_match_oracle(hip_ctx):
     assert not bad, bad[:10]
 
 
-def test_benchmark_network_bit_exact_at_full_size(hip_ctx):
-    """The measured workload at its full size: 100 x 100 LIF + DS growth network (test-small
-    distributions, DynamicLayout 0.9 / 0.1, M = 1000, max_firing_rate 1000), every radius set to 8.9
-    and one growth update on each side (8,440,432 synapses created on the device and in the oracle),
-    membranes started within 10% of the reset-to-threshold range below threshold (seeded) so that
-    nearly every neuron fires, 270 steps (thirty 9-step fused launches, about 80,000 spikes each
-    delivered to about 844 synapses), then a second growth update over the existing synapses. Every
-    state array equals the oracle's after each stage."""
+FULL_SIZE = {
+    # side, M, radius, seed, synapses after the first growth update, minimum spikes in 270 steps
+    "10k-benchmark": (100, 1000, 8.9, 1, 8440432, 50000),
+    "40k-largest-served": (200, 100, 2.6, 4, 3441176, 200000),
+}
+
+
+@pytest.mark.parametrize("case", list(FULL_SIZE))
+def test_benchmark_network_bit_exact_at_full_size(hip_ctx, case):
+    """Full sizes: the measured workload (100 x 100 LIF + DS growth network, test-small
+    distributions, DynamicLayout 0.9 / 0.1, M = 1000, max_firing_rate 1000, every radius set to 8.9:
+    8,440,432 synapses) and the largest network of the served workload (200 x 200, 40,000 neurons,
+    M = 100, radius 2.6: 3,441,176 synapses). One growth update on each side creates the synapses on
+    the device and in the oracle; membranes start within 10% of the reset-to-threshold range below
+    threshold (seeded) so that nearly every neuron fires; 270 steps (thirty 9-step fused launches,
+    80,000 / 255,000 spikes); then a second growth update over the existing synapses. Every state
+    array equals the oracle's after each stage."""
+    side, M, radius, seed, want, min_spikes = FULL_SIZE[case]
     cfg = O.default_config()
-    cfg.update(width=100, height=100, max_synapses_per_neuron=1000, max_firing_rate=1000, num_epochs=1,
-               epoch_duration=10.0, seed=1, layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1,
+    cfg.update(width=side, height=side, max_synapses_per_neuron=M, max_firing_rate=1000, num_epochs=1,
+               epoch_duration=10.0, seed=seed, layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1,
                noise_mode=1, math_mode=1, sum_mode=1)
     ora = O.OracleNet(cfg)
-    ora.set_radii(8.9)
+    ora.set_radii(radius)
     vm, vth, vr = ora.view("Vm"), ora.array("Vthresh"), ora.array("Vreset")
     u = np.random.default_rng(5).random(vm.size).astype(np.float32)
     vm[:] = (vth - u * np.float32(0.1) * (vth - vr)).astype(np.float32)
     net = H.gpu_net_from_oracle(hip_ctx, ora, cfg)
     try:
-        print("[parity] 10k network: growth to 8.44M synapses", flush=True)
+        print(f"[parity] {case}: growth update", flush=True)
         ora.update_connections()
         st = net.update_connections()
-        assert st.total_synapses == ora.total_synapses == 8440432
+        assert st.total_synapses == ora.total_synapses == want
         bad = H.compare_state(ora, net, cfg, "grown")
-        print("[parity] 10k network: 270 steps", flush=True)
+        print(f"[parity] {case}: 270 steps", flush=True)
         ora.advance(270)
         net.advance(270)
         net.sync()
@@ -169,8 +179,8 @@ def test_benchmark_network_bit_exact_at_full_size(hip_ctx):
         ora.update_connections()
         net.update_connections()
         bad += H.compare_state(ora, net, cfg, "grown-again")
-        print(f"[parity] 10k network: {spikes} spikes, {len(bad)} differences", flush=True)
-        assert spikes > 50000
+        print(f"[parity] {case}: {spikes} spikes, {len(bad)} differences", flush=True)
+        assert spikes > min_spikes
         assert not bad, bad[:10]
     finally:
         net.close()
