@@ -10,7 +10,8 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N,k,seed,chunks", [(200, 20, 7, (250, 750, 1000, 1000)), (1000, 100, 5, (600, 600))])
+@pytest.mark.parametrize("N,k,seed,chunks", [(200, 20, 7, (250, 750, 1000, 1000)), (1000, 100, 5, (600, 600)),
+                                           (14400, 100, 9, (600, 600))])   # the largest sweep network: 1.44M synapses
 def test_izh_dynstdp_static_parity(hip_ctx, N, k, seed, chunks):
     print(f"\n[izh_dynstdp_static N={N} k={k} seed={seed}]", flush=True)
     neu, syn, meta = NW.izh_stdp_arrays(N, seed, k=k)
