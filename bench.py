@@ -11,10 +11,16 @@ pre-grown: every radius is set to 8.9 and one growth update runs on the GPU, whi
 boundary is crossed.
 
 Output: one JSON line. `value` = synapse-updates/s = sum over timed steps of active synapses /
-wall time. `roofline` prices the dominant kernel (the fused block step kernel) with the per-unit
-algorithmic bytes (24 B per DS synapse-step + 44 B per LIF neuron-step) over its average launch
-duration, measured with HIP events on the network's stream. `cpu_baseline` times the CPU oracle
-(the reference's single-threaded algorithm restated in C) on a bounded sample of the same network.
+wall time. `roofline` prices the dominant kernel (the fused block step kernel, K = 9 steps per
+launch) with the bytes a temporally blocked kernel must move per launch: each synapse's meta,
+delay-ring word and psr read once and ring/psr written once per block (20 B), the DS state of
+every synapse that receives a spike in the block (W, r, u, lastSpike read, r, u, lastSpike
+written: 36 B; the count of such synapses is measured from the spike counts and out-degrees over
+the timed launches) and the neuron state once per block (44 B); over the average launch duration
+measured with HIP events on the network's stream. `per_step` does the same for the per-step
+kernels that replace the reference's advanceNeurons / advanceSynapses one for one (block_steps =
+1: 24 B per DS synapse-step, 44 B per LIF neuron-step). `cpu_baseline` times the CPU oracle (the
+reference's single-threaded algorithm restated in C) on a bounded sample of the same network.
 `batched` summarises the batched form (the 256-network sweep with shared launches against a
 sample of its networks run alone: simulated seconds per wall hour both ways) and `served` the
 served form (the first 200 requests of the open-loop job stream: jobs/s, p50/p99 completion
@@ -32,8 +38,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-DS_SYN_STEP_BYTES = 24
-LIF_NEURON_STEP_BYTES = 44
+DS_SYN_STEP_BYTES = 24            # per-step kernel: psr r/w 8, decay 4, ring r/w 8, source index 4
+LIF_NEURON_STEP_BYTES = 44        # Vm, Vthresh, sp, I0, Inoise, C1, C2, refr read; Vm, sp, refr written
+DS_SYN_BLOCK_BYTES = 20           # blocked kernel, per synapse and block: meta 4, ring r/w 8, psr r/w 8
+DS_DELIVERY_BYTES = 36            # per delivered spike: W, r, u (12) + lastSpike (8) read; r, u, lastSpike written
 GRID = 100
 R0 = 8.9
 M = 1000
@@ -141,12 +149,41 @@ def forms_cpu_baseline(specs: list, sample: list, steps: int, threads: int) -> d
                       f"from the sample per family by neuron-seconds"}
 
 
+KERNEL_SOURCES = ("bghip_kernels.h", "bghip_device.h", "bghip_launch.h", "bghip_fused_lif.hip")
+
+
+def kernel_source_sha() -> str:
+    """Digest of the sources the headline kernel is compiled from (and the detmath header)."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "braingrid_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    with open(os.path.join(ROOT, "include", "bg_detmath.h"), "rb") as fh:
+        h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def load_traffic():
+    """The counter-measured HBM bytes per launch of the headline kernel, if the profile was taken
+    on the kernel sources of this tree (otherwise None: a stale figure is not reported)."""
     p = os.path.join(ROOT, "profiles", "fused_block_10k_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f)
+        t = json.load(f)
+    if t.get("kernel_source_sha") != kernel_source_sha():
+        return None
+    return t
+
+
+def deliveries_between(net, counts0, outdeg):
+    """Synapses that received a spike between two spike-count snapshots of the same epoch: every
+    spike of a source reaches each of its outgoing synapses once (a source's refractory period is
+    longer than a block, so no synapse receives two spikes in one block)."""
+    import numpy as np
+    c1 = net.download_spikes()["spikeCount"].astype(np.int64)
+    return int(((c1 - counts0) * outdeg).sum())
 
 
 def log(msg):
@@ -317,6 +354,35 @@ def served_alone(ctx, jobs, S, stride: int) -> dict:
             "identical_results": f"{same}/{len(sample)} sampled jobs with the same output alone and served"}
 
 
+def per_step_roofline(ctx, steps: int) -> dict:
+    """The same pre-grown network advanced with the per-step kernels (block_steps = 1: one neuron
+    launch and one synapse launch per step, the reference's advanceNeurons / advanceSynapses
+    shape), timed with HIP events over `steps` steps; priced at the per-step bytes."""
+    from braingrid_amd import host
+    cfg = workload_cfg()
+    cfg["block_steps"] = 1
+    m = host.Model(ctx, cfg)
+    m.set_radii(R0)
+    m.update_connections()
+    m.advance(200)
+    m.net.sync()
+    S = m.net.total_synapses
+    N = GRID * GRID
+    m.net.timing(True)
+    m.advance(steps)
+    avg_ms, launches = m.net.timing(False)
+    m.close()
+    step_ms = avg_ms * launches / max(1, steps)
+    step_bytes = DS_SYN_STEP_BYTES * S + LIF_NEURON_STEP_BYTES * N
+    achieved = step_bytes / (step_ms * 1e-3) if step_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "kernels": "k_neuron_step<LIF> + k_synapse_step<DS,1> per step",
+            "ms_per_step": step_ms, "steps": steps, "synapses": int(S),
+            "synapse_updates_per_s": S / (step_ms * 1e-3) if step_ms > 0 else 0.0,
+            "byte_model": f"per step: {DS_SYN_STEP_BYTES} B x synapses + {LIF_NEURON_STEP_BYTES} B x neurons",
+            "algorithmic_bytes_per_step": step_bytes}
+
+
 def forms_summary(args):
     """The batched and served forms at reduced sample sizes, summarised for the headline line: the
     full 256-network sweep batched against every `forms_alone_stride`-th network alone, and the
@@ -384,6 +450,10 @@ def main():
                     help="batched/served forms: oracle steps per sampled network (plus one epoch-end update)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="batched/served forms: host threads of the CPU baseline (one network each)")
+    ap.add_argument("--no-per-step", action="store_true",
+                    help="single form: skip the per-step kernels' roofline on the same network")
+    ap.add_argument("--per-step-steps", type=int, default=2000,
+                    help="single form: steps of the per-step kernels' roofline sub-phase")
     ap.add_argument("--no-forms", action="store_true",
                     help="single form: skip the batched and served summaries added to the headline line")
     ap.add_argument("--forms-alone-stride", type=int, default=16,
@@ -434,11 +504,16 @@ def main():
     model.net.sync()
     done = args.warmup
 
-    # roofline sub-phase: back-to-back fused launches only, timed with HIP events on the net stream
+    # roofline sub-phase: back-to-back fused launches only, timed with HIP events on the net stream;
+    # the spike counts before and after give the synapses that received a spike in these blocks
     rl_steps = min(args.roofline_steps, steps_per_epoch - done % steps_per_epoch - 1)
+    import numpy as np
+    outdeg = model.net.download_index_map()["outgoingSynapseCount"].astype(np.int64)
+    counts0 = model.net.download_spikes()["spikeCount"].astype(np.int64)
     model.net.timing(True)
     model.advance(rl_steps)
     avg_ms, launches = model.net.timing(False)
+    delivered = deliveries_between(model.net, counts0, outdeg)
     done += rl_steps
     S_rl = model.net.total_synapses
     K = max(1, round(rl_steps / max(1, launches)))
@@ -469,7 +544,9 @@ def main():
     rate_hz = float(neu["spikeCount"].mean()) / (in_epoch * 1e-4)
 
     value = synapse_updates / wall
-    launch_bytes = (DS_SYN_STEP_BYTES * S_rl + LIF_NEURON_STEP_BYTES * N) * (rl_steps / max(1, launches))
+    per_launch_delivered = delivered / max(1, launches)
+    launch_bytes = (DS_SYN_BLOCK_BYTES * S_rl + DS_DELIVERY_BYTES * per_launch_delivered
+                    + LIF_NEURON_STEP_BYTES * N)
     achieved = launch_bytes / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
     traffic = load_traffic()
     # the instantiation the full blocks run: exactly 9 steps for a 32-bit ring, else up to 16 / 32
@@ -491,20 +568,35 @@ def main():
                                "epoch 1e5 steps", "neurons": N, "synapses": int(S0),
                    "max_synapses_per_neuron": M, "max_firing_rate": MAX_RATE, "block_steps": K,
                    "mean_rate_hz_last_epoch": round(rate_hz, 2), "growth_updates_timed": growth_updates,
-                   "pregrow_s": round(t_pregrow, 4)},
+                   "pregrow_s": round(t_pregrow, 4),
+                   "deviations": "maxFiringRate 1000 (the pre-grown network fires at ~300 Hz); per-type "
+                                 "delays of the reference (EE 1.5 ms = 16 steps, others 0.8 ms = 9 steps); "
+                                 "INH Trefract 2 ms as in the reference"},
         "neuron_steps_per_s": N * args.steps / wall,
         "sim_seconds_per_hour": args.steps * 1e-4 / wall * 3600.0,
+        "timed_region": f"{args.steps} steps in whole calls of the host model, growth update and history "
+                        "compilation at each epoch boundary crossed, bracketed by device synchronisation",
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic.get("bytes_per_launch") if traffic else None,
                      "kernel": kernel_name, "avg_launch_ms": avg_ms, "launches": launches,
                      "steps_per_launch": rl_steps / max(1, launches),
+                     "byte_model": f"per launch: {DS_SYN_BLOCK_BYTES} B x synapses (meta, ring r/w, psr r/w once "
+                                   f"per block) + {DS_DELIVERY_BYTES} B x synapses receiving a spike in the block "
+                                   f"(W, r, u, lastSpike read; r, u, lastSpike written) + "
+                                   f"{LIF_NEURON_STEP_BYTES} B x neurons",
+                     "synapses": int(S_rl), "delivered_per_launch": per_launch_delivered,
+                     "delivered_fraction": per_launch_delivered / max(1, S_rl),
                      "algorithmic_bytes_per_launch": launch_bytes,
                      "traffic_GBps": (traffic["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9) if traffic and avg_ms > 0 else None,
-                     "traffic_source": "profiles/fused_block_10k_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
-                                       "copy-calibrated)" if traffic else None},
+                     "traffic_frac": (traffic["bytes_per_launch"] / (avg_ms * 1e-3) / HBM_PEAK) if traffic and avg_ms > 0 else None,
+                     "traffic_source": ("profiles/fused_block_10k_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                        "copy-calibrated, same kernel sources)") if traffic else
+                                       "no counter profile of the current kernel sources"},
         "device_error_bits": int(err),
     }
     model.close()
+    if not args.no_per_step:
+        out["per_step"] = per_step_roofline(ctx, args.per_step_steps)
     ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -32,6 +32,8 @@ struct bghip_ctx {
     // per-step runs replay captured graphs; BGHIP_GRAPHS=0 in the environment launches the same
     // kernels directly instead (for tools that do not follow graph launches)
     bool graphs;
+    // batches of this context: a destroyed network leaves every batch it is a member of
+    std::vector<bghip_batch *> batches;
 };
 
 struct bghip_net {
@@ -340,10 +342,13 @@ static void drop_graph(bghip_net *n)
     n->g_exec = nullptr;
 }
 
+static void batch_forget_net(bghip_batch *b, const bghip_net *n);
+
 extern "C" void bghip_net_destroy(bghip_net *n)
 {
     if (!n) return;
     (void)hipSetDevice(n->ctx->device);
+    for (bghip_batch *b : n->ctx->batches) batch_forget_net(b, n);
     (void)hipStreamSynchronize(n->stream);
     drop_graph(n);
     (void)hipEventDestroy(n->ev0);
@@ -1486,6 +1491,7 @@ struct bghip_batch {
     std::vector<BatchGroup> groups;
     std::vector<int> modes;        // net_mode of each member when the groups were formed
     std::vector<int> kmaxs;        // and its block length (a group's is the minimum of its members')
+    bool stale = false;            // a member was destroyed: the groups must be formed again
 };
 
 static void free_groups(bghip_batch *b)
@@ -1510,6 +1516,19 @@ static void free_groups(bghip_batch *b)
     b->groups.clear();
 }
 
+// A member being destroyed: the batch finishes its queued work, drops the network and its group
+// tables (which hold a copy of the network's device descriptor), and forms its groups again at
+// the next call.
+static void batch_forget_net(bghip_batch *b, const bghip_net *n)
+{
+    auto it = std::find(b->nets.begin(), b->nets.end(), n);
+    if (it == b->nets.end()) return;
+    (void)hipStreamSynchronize(b->stream);
+    b->nets.erase(it);
+    free_groups(b);
+    b->stale = true;
+}
+
 extern "C" int bghip_batch_create(bghip_ctx *ctx, bghip_batch **out)
 {
     if (!ctx || !out) return BGHIP_EINVAL;
@@ -1523,6 +1542,7 @@ extern "C" int bghip_batch_create(bghip_ctx *ctx, bghip_batch **out)
         delete b;
         return set_err(ctx, BGHIP_EDEVICE, "batch stream/event creation failed");
     }
+    ctx->batches.push_back(b);
     *out = b;
     return BGHIP_OK;
 }
@@ -1532,6 +1552,8 @@ extern "C" void bghip_batch_destroy(bghip_batch *b)
     if (!b) return;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->stream);
+    auto &bs = b->ctx->batches;
+    bs.erase(std::remove(bs.begin(), bs.end(), b), bs.end());
     free_groups(b);
     (void)hipEventDestroy(b->tev0);
     (void)hipEventDestroy(b->tev1);
@@ -1576,6 +1598,7 @@ static int regroup(bghip_batch *b)
 {
     HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
     free_groups(b);
+    b->stale = false;
     const int n = (int)b->nets.size();
     b->modes.assign(n, 0);
     b->kmaxs.assign(n, 0);
@@ -1685,6 +1708,7 @@ static int regroup(bghip_batch *b)
 // True if a member changed mode or block length since the groups were formed.
 static bool members_changed(const bghip_batch *b)
 {
+    if (b->stale || b->modes.size() != b->nets.size()) return true;
     for (size_t i = 0; i < b->nets.size(); i++)
         if (net_mode(b->nets[i]) != b->modes[i] || b->nets[i]->kmax != b->kmaxs[i]) return true;
     return false;

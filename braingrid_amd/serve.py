@@ -109,10 +109,13 @@ class DeviceEngine:
         return SW.SweepNet.restore(self.ctx, spec, path)
 
     def advance(self, handles, steps: int):
-        key = tuple(id(h) for h in handles)
-        if key != self._members:
+        # the members are compared by identity with references held here, so a released handle
+        # cannot be mistaken for a new one that happens to reuse its address
+        handles = list(handles)
+        if (self._members is None or len(handles) != len(self._members)
+                or any(a is not b for a, b in zip(handles, self._members))):
             self.batch.set([h.net for h in handles])
-            self._members = key
+            self._members = handles
         self.batch.advance(steps)
 
     def end_epoch(self, h):
@@ -131,6 +134,9 @@ class DeviceEngine:
         return out
 
     def release(self, h):
+        if self._members is not None and any(m is h for m in self._members):
+            # the batch drops a destroyed member itself; forget the cached membership too
+            self._members = None
         h.close()
 
     def close(self):

@@ -273,3 +273,41 @@ def test_batch_follows_a_member_whose_blocks_shorten(hip_ctx):
         batch.close()
         net.close()
         other.close()
+
+
+def test_destroyed_member_leaves_the_batch(hip_ctx):
+    """A network destroyed while it is a batch member leaves the batch: later batch calls advance
+    the remaining members (exactly as the oracle) and never touch the destroyed one, also when a
+    new network is created in its place before the next call."""
+    print("[batch] member destroyed between batch calls", flush=True)
+    cfgs, oras, nets = _grown_mix(hip_ctx)
+    batch = B.Batch(hip_ctx)
+    batch.set(nets)
+    for o in oras:
+        o.advance(300)
+    batch.advance(300)
+    gone = [1, 4]
+    for k in gone:
+        nets[k].close()
+    keep = [k for k in range(len(nets)) if k not in gone]
+    # a new network of another size takes the freed memory before the batch runs again
+    cnew = H.load_config("conn-lif-ds")
+    onew = O.OracleNet(cnew)
+    nnew = H.gpu_net_from_oracle(hip_ctx, onew, cnew)
+    assert len(batch.shapes()) >= 1
+    for k in keep:
+        oras[k].advance(500)
+    batch.advance(500)
+    bad = []
+    for k in keep:
+        nets[k].sync()
+        bad += H.compare_state(oras[k], nets[k], cfgs[k], f"{MIX[k]}-after-destroy", check_maps=False)
+    onew.advance(20)
+    nnew.advance(20)
+    nnew.sync()
+    bad += H.compare_state(onew, nnew, cnew, "new-net", check_maps=False)
+    batch.close()
+    for k in keep:
+        nets[k].close()
+    nnew.close()
+    assert not bad, bad[:10]

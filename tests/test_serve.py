@@ -220,3 +220,56 @@ def test_cancel_of_an_evicted_job_removes_its_checkpoint():
         assert big.state == S.DONE and a.state == S.DONE and a.result["epochs"] == 10
     finally:
         srv.close()
+
+
+class _FakeBatch:
+    def __init__(self):
+        self.sets = []
+        self.steps = 0
+
+    def set(self, nets):
+        self.sets.append(list(nets))
+
+    def advance(self, steps):
+        self.steps += steps
+
+
+class _FakeHandle:
+    def __init__(self, tag):
+        self.net = tag
+        self.closed = False
+
+    def close(self):
+        self.closed = True
+
+
+def test_device_engine_sets_batch_again_after_a_member_is_released():
+    """The served engine caches its batch membership between epochs. A job released after the
+    queue drained and a new job admitted later must form a new membership even when the new
+    handle object reuses the released one's address (the cache compares the objects themselves)."""
+    eng = S.DeviceEngine.__new__(S.DeviceEngine)
+    eng.batch = _FakeBatch()
+    eng._members = None
+    eng.fingerprint = False
+    h1 = _FakeHandle("net1")
+    eng.advance([h1], 10)
+    eng.advance([h1], 10)
+    assert len(eng.batch.sets) == 1
+    eng.release(h1)
+    assert h1.closed
+    del h1
+    # many new handles: CPython reuses freed addresses, so some of these share the old id()
+    for k in range(50):
+        h = _FakeHandle(f"net{k + 2}")
+        eng.advance([h], 10)
+        assert eng.batch.sets[-1] == [f"net{k + 2}"]
+        eng.release(h)
+        del h
+    assert len(eng.batch.sets) == 51
+    # the same members in the same order are not set again; another order or set is
+    a, b = _FakeHandle("a"), _FakeHandle("b")
+    eng.advance([a, b], 1)
+    eng.advance([a, b], 1)
+    eng.advance([b, a], 1)
+    eng.advance([b], 1)
+    assert [x for x in eng.batch.sets[51:]] == [["a", "b"], ["b", "a"], ["b"]]
