@@ -894,6 +894,20 @@ static int apply_weights(bghip_net *n, const float *Wd, bghip_growth_stats *stat
     return BGHIP_OK;
 }
 
+// The caller's matrix W(s, d) (row-major, s the source) in device memory at Wd is transposed in
+// place, so each destination's workgroup reads its row contiguously, then applied.
+static int apply_caller_weights(bghip_net *n, float *Wd, const float *W, bghip_growth_stats *stats)
+{
+    const int N = n->dn.N;
+    const size_t bytes = sizeof(float) * (size_t)N * (size_t)N;
+    if (hipMemcpyAsync(Wd, W, bytes, hipMemcpyHostToDevice, n->stream) != hipSuccess)
+        return set_err(n->ctx, BGHIP_EDEVICE, "weight matrix upload failed");
+    const unsigned T = (unsigned)((N + BG_TT - 1) / BG_TT);
+    hipLaunchKernelGGL(k_transpose_square, dim3(T, T), dim3(BG_BLOCK), 0, n->stream, Wd, N);
+    HIP_TRY(n->ctx, hipGetLastError());
+    return apply_weights(n, Wd, stats);
+}
+
 extern "C" int bghip_update_weights(bghip_net *n, const float *W, bghip_growth_stats *stats)
 {
     if (!n || !W) return BGHIP_EINVAL;
@@ -904,10 +918,7 @@ extern "C" int bghip_update_weights(bghip_net *n, const float *W, bghip_growth_s
     const size_t bytes = sizeof(float) * (size_t)n->dn.N * (size_t)n->dn.N;
     float *Wd = nullptr;
     HIP_TRY(n->ctx, hipMalloc((void **)&Wd, std::max<size_t>(bytes, 4)));
-    int rc = BGHIP_OK;
-    if (hipMemcpyAsync(Wd, W, bytes, hipMemcpyHostToDevice, n->stream) != hipSuccess)
-        rc = set_err(n->ctx, BGHIP_EDEVICE, "weight matrix upload failed");
-    if (rc == BGHIP_OK) rc = apply_weights(n, Wd, stats);
+    int rc = apply_caller_weights(n, Wd, W, stats);
     // the matrix is freed once the kernels reading it are done
     const hipError_t e = hipStreamSynchronize(n->stream);
     (void)hipFree(Wd);

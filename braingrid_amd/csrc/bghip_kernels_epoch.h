@@ -95,18 +95,47 @@ __device__ void create_synapse(const DevNet &n, const CreateCtx &c, uint64_t slo
 // connected-source set is an LDS bitmap; create decisions are compacted to slots with a
 // wavefront-ballot + workgroup prefix scan.
 // W(s, d) of the update: the circles' overlap area (the growth rule's W = area,
-// Connections/ConnGrowth.cpp:395), or entry [s * N + d] of a caller's dense matrix when Wd is set.
-__device__ __forceinline__ float pair_weight(const DevNet &n, const float *Wd, int s, int d, float xs, float ys,
+// Connections/ConnGrowth.cpp:395), or, when Wt is set, the caller's dense matrix W(s, d) held
+// transposed (Wt[d * N + s]), so that a destination's workgroup reads its sources contiguously.
+__device__ __forceinline__ float pair_weight(const DevNet &n, const float *Wt, int s, int d, float xs, float ys,
                                              float rs, float xd, float yd, float rd)
 {
-    return Wd ? Wd[(uint64_t)s * (uint64_t)n.N + (uint64_t)d] : overlap_area(xs, ys, rs, xd, yd, rd, s == d);
+    return Wt ? Wt[(uint64_t)d * (uint64_t)n.N + (uint64_t)s] : overlap_area(xs, ys, rs, xd, yd, rd, s == d);
+}
+
+// In-place transpose of a square N x N float matrix, one 32 x 32 tile pair per workgroup
+// (tile (i, j) with i <= j swaps with tile (j, i) through LDS). Grid: (T, T), T = ceil(N / 32),
+// 256 threads.
+#define BG_TT 32
+__global__ __launch_bounds__(BG_BLOCK) void k_transpose_square(float *W, int N)
+{
+    __shared__ float a[BG_TT][BG_TT + 1], b[BG_TT][BG_TT + 1];
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (ti > tj) return;
+    const int tx = threadIdx.x & (BG_TT - 1), ty = threadIdx.x / BG_TT;
+    const int rows = BG_BLOCK / BG_TT;
+    for (int r = ty; r < BG_TT; r += rows) {
+        const int i = ti * BG_TT + r, j = tj * BG_TT + tx;
+        if (i < N && j < N) a[r][tx] = W[(size_t)i * N + j];
+        const int i2 = tj * BG_TT + r, j2 = ti * BG_TT + tx;
+        if (ti != tj && i2 < N && j2 < N) b[r][tx] = W[(size_t)i2 * N + j2];
+    }
+    __syncthreads();
+    for (int r = ty; r < BG_TT; r += rows) {
+        // element (i, j) of the transpose is W(j, i): tile (ti, tj) entry (tx, r)
+        const int i = tj * BG_TT + r, j = ti * BG_TT + tx;
+        if (i < N && j < N) W[(size_t)i * N + j] = a[tx][r];
+        const int i2 = ti * BG_TT + r, j2 = tj * BG_TT + tx;
+        if (ti != tj && i2 < N && j2 < N) W[(size_t)i2 * N + j2] = b[tx][r];
+    }
 }
 
 __device__ void growth_serial(const DevNet &n, const CreateCtx &c, int d, const uint32_t *conn, uint32_t *stats,
                               const float *Wd)
 {
-    // Replays the reference's per-source order one source at a time. Used for rows that contain
-    // holes or need erasures (the growth rule itself never produces area < 0; a caller's matrix can).
+    // Replays the reference's per-source order one source at a time on one lane. Used only for rows
+    // with holes or erasures whose capacity M exceeds the workgroup path's slot bitmaps
+    // (growth_holed below).
     const int N = n.N, M = n.M;
     const uint64_t base = (uint64_t)d * M;
     const float xd = n.xloc[d], yd = n.yloc[d], rd = n.radii[d];
@@ -145,12 +174,170 @@ __device__ void growth_serial(const DevNet &n, const CreateCtx &c, int d, const 
     atomicAdd(&stats[1], erased);
 }
 
+// Rows with holes or erasures, whole workgroup: the reference visits the sources in ascending
+// order and, for destination d, erases a connected source's synapses when W < 0 and gives an
+// unconnected source with W > 0 a synapse in d's first free slot (Connections/ConnGrowth.cpp:
+// 388-463, AllSynapses::addSynapse at Synapses/AllSynapses.cpp:357-377), so an erasure frees a slot
+// that a later source may take. The workgroup evaluates W for 256 sources at a time; wavefront 0
+// then applies that chunk's erase and create events in source order, with the row's in-use slots
+// and the slots created in this update as LDS bitmaps: a create takes the lowest clear in-use bit
+// (a ballot over 64 bitmap words per round), an erase finds the source's slots with a ballot over
+// 64 slots per round. Same decisions, slots and order as the one-lane replay growth_serial.
+#define BG_HOLED_MAX_M 8192
+#define BG_HOLED_WORDS (BG_HOLED_MAX_M / 32)
+
+// LDS written by some lanes of a wavefront and read by others afterwards
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct HoledLds {
+    uint32_t inuse[BG_HOLED_WORDS];   // slot k of the row in use
+    uint32_t made[BG_HOLED_WORDS];    // slot k created by this update (its source is unconnected)
+    float area[BG_BLOCK];             // W(s0 + t, d) of the current chunk
+    uint64_t cre[BG_BLOCK / BG_WAVE], era[BG_BLOCK / BG_WAVE];
+};
+
+__device__ void growth_holed(const DevNet &n, const CreateCtx &c, int d, const uint32_t *conn, uint32_t *stats,
+                             const float *Wt, HoledLds &L, float xd, float yd, float rd, int td, uint32_t cnt0,
+                             uint32_t hi0)
+{
+    const int tid = threadIdx.x, lane = tid & (BG_WAVE - 1), wv = tid / BG_WAVE;
+    const int N = n.N, M = n.M;
+    const uint64_t base = (uint64_t)d * M;
+    const uint32_t words = (uint32_t)(M + 31) >> 5;
+    for (uint32_t w = tid; w < words; w += BG_BLOCK) {
+        L.inuse[w] = 0u;
+        L.made[w] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t k0 = 0; k0 < hi0; k0 += BG_BLOCK) {
+        const uint32_t k = k0 + tid;
+        const uint64_t b = __ballot(k < hi0 && (n.meta[base + k] & BG_META_IN_USE));
+        if (lane == 0 && b) {
+            const uint32_t w0 = (k0 + BG_WAVE * wv) >> 5;
+            L.inuse[w0] = (uint32_t)b;
+            L.inuse[w0 + 1] = (uint32_t)(b >> 32);
+        }
+    }
+    __syncthreads();
+    // wavefront 0's running state (uniform across its lanes)
+    uint32_t count = cnt0, hi = hi0, created = 0, erased = 0, fw = 0;
+    const uint32_t last_mask = (M & 31) ? ((1u << (M & 31)) - 1u) : 0xFFFFFFFFu;
+    for (int s0 = 0; s0 < N; s0 += BG_BLOCK) {
+        const int s = s0 + tid;
+        bool cr = false, er = false;
+        float area = 0.0f;
+        if (s < N) {
+            const bool connected = (conn[s >> 5] >> (s & 31)) & 1u;
+            area = pair_weight(n, Wt, s, d, n.xloc[s], n.yloc[s], n.radii[s], xd, yd, rd);
+            er = connected && area < 0;
+            cr = !connected && area > 0;
+        }
+        L.area[tid] = area;
+        const uint64_t bc = __ballot(cr), be = __ballot(er);
+        if (lane == 0) {
+            L.cre[wv] = bc;
+            L.era[wv] = be;
+        }
+        __syncthreads();
+        if (wv == 0) {
+            for (int w = 0; w < BG_BLOCK / BG_WAVE; w++) {
+                const uint64_t ew = L.era[w];
+                uint64_t ev = L.cre[w] | ew;
+                while (ev) {
+                    const int j = __ffsll((unsigned long long)ev) - 1;
+                    ev &= ev - 1;
+                    const int se = s0 + BG_WAVE * w + j;
+                    if ((ew >> j) & 1ull) {
+                        // erase: every in-use slot of source se that existed before this update
+                        for (uint32_t k0 = 0; k0 < hi; k0 += BG_WAVE) {
+                            const uint32_t k = k0 + lane;
+                            bool hit = false;
+                            if (k < hi) {
+                                const uint32_t wd = L.inuse[k >> 5] & ~L.made[k >> 5];
+                                if ((wd >> (k & 31)) & 1u) hit = (int)(n.meta[base + k] & BG_META_SRC_MASK) == se;
+                            }
+                            if (hit) n.meta[base + k] = 0u;
+                            const uint64_t hb = __ballot(hit);
+                            if (hb) {
+                                if (lane == 0) {
+                                    L.inuse[k0 >> 5] &= ~(uint32_t)hb;
+                                    if ((k0 >> 5) + 1 < words) L.inuse[(k0 >> 5) + 1] &= ~(uint32_t)(hb >> 32);
+                                }
+                                const uint32_t nh = (uint32_t)__popcll(hb);
+                                count -= nh;
+                                erased += nh;
+                                fw = min(fw, k0 >> 5);
+                                wave_lds_sync();
+                            }
+                        }
+                    } else {
+                        if (count >= (uint32_t)M) {
+                            if (lane == 0) dev_error(n, 4u);
+                            continue;
+                        }
+                        // create: the lowest free slot of the row
+                        uint32_t k = 0xFFFFFFFFu;
+                        for (uint32_t w0 = fw; w0 < words; w0 += BG_WAVE) {
+                            const uint32_t ww = w0 + lane;
+                            uint32_t fr = 0u;
+                            if (ww < words) fr = ~L.inuse[ww] & (ww == words - 1 ? last_mask : 0xFFFFFFFFu);
+                            const uint64_t bw = __ballot(fr != 0u);
+                            if (bw) {
+                                const int l = __ffsll((unsigned long long)bw) - 1;
+                                const uint32_t frl = (uint32_t)__shfl((int)fr, l);
+                                k = 32u * (w0 + (uint32_t)l) + (uint32_t)(__ffs((int)frl) - 1);
+                                break;
+                            }
+                        }
+                        if (k == 0xFFFFFFFFu) {   // unreachable while count < M
+                            if (lane == 0) dev_error(n, 4u);
+                            continue;
+                        }
+                        if (lane == 0) {
+                            L.inuse[k >> 5] |= 1u << (k & 31);
+                            L.made[k >> 5] |= 1u << (k & 31);
+                            const int type = syn_type_of(n.ntype[se], td);
+                            create_synapse(n, c, base + k, se, type,
+                                           L.area[BG_WAVE * w + j] * (float)syn_sign_of(type) * c.W_adj);
+                        }
+                        wave_lds_sync();
+                        fw = k >> 5;
+                        hi = max(hi, k + 1);
+                        count++;
+                        created++;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        uint32_t h = 0;
+        for (int w = (int)words - 1; w >= 0; w--)
+            if (L.inuse[w]) {
+                h = 32u * (uint32_t)w + 32u - (uint32_t)__clz((int)L.inuse[w]);
+                break;
+            }
+        n.syn_count[d] = count;
+        n.slot_hi[d] = h;
+        if (!(count < (uint32_t)M)) dev_error(n, 4u);
+        atomicAdd(&stats[0], created);
+        atomicAdd(&stats[1], erased);
+    }
+}
+
 __global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c, uint32_t *stats, const float *Wd)
 {
     __shared__ float sx[BG_GROWTH_TILE], sy[BG_GROWTH_TILE], sr[BG_GROWTH_TILE];
     __shared__ uint32_t conn[BG_CONN_WORDS];
     __shared__ uint32_t s_wave[BG_BLOCK / BG_WAVE];
     __shared__ uint32_t s_base, s_erase;
+    __shared__ HoledLds holed;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int N = n.N, M = n.M;
     const int nwords = (N + 31) >> 5;
@@ -176,7 +363,8 @@ __global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c,
         }
         __syncthreads();
         if (s_erase || cnt0 != hi0) {
-            if (tid == 0) growth_serial(n, c, d, conn, stats, Wd);
+            if (M <= BG_HOLED_MAX_M) growth_holed(n, c, d, conn, stats, Wd, holed, xd, yd, rd, td, cnt0, hi0);
+            else if (tid == 0) growth_serial(n, c, d, conn, stats, Wd);
             continue;
         }
         // rows without holes: the j-th creation (ascending source) takes slot hi0 + j

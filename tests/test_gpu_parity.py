@@ -69,20 +69,24 @@ def test_block_size_does_not_change_results(hip_ctx, block_steps):
     assert not bad, bad[:10]
 
 
-def test_dense_weight_update_with_erasures_matches_oracle(hip_ctx):
+@pytest.mark.parametrize("M", [30, 8200])
+def test_dense_weight_update_with_erasures_matches_oracle(hip_ctx, M):
     """bghip_update_weights (ConnGrowth::updateSynapsesWeights with a caller's W matrix): existing
     synapses erased (their slots reused by later sources' new synapses), reweighted or kept at zero
     weight, new ones created; slot state and index maps equal the oracle's, and so does the network
-    over the steps that follow and a second update."""
+    over the steps that follow and a second update. Rows of 30 slots go through the workgroup path
+    for rows with holes or erasures, rows of 8,200 slots (beyond its LDS slot bitmaps) through the
+    one-lane replay."""
+    print(f"[parity] weight update with erasures, M={M}", flush=True)
     cfg = O.default_config()
-    cfg.update(width=6, height=6, max_synapses_per_neuron=30, layout="DynamicLayout", frac_exc=0.8,
+    cfg.update(width=6, height=6, max_synapses_per_neuron=M, layout="DynamicLayout", frac_exc=0.8,
                frac_starter=0.1, noise_mode=1, math_mode=1, sum_mode=1, epoch_duration=0.05, num_epochs=2)
     ora = O.OracleNet(cfg)
     ora.set_radii(0.75)
     net = H.gpu_net_from_oracle(hip_ctx, ora, cfg)
     ora.update_connections()
     net.update_connections()
-    N, M = 36, 30
+    N = 36
     rng = np.random.default_rng(11)
     bad = []
     for rnd in range(2):
@@ -181,6 +185,62 @@ def test_benchmark_network_bit_exact_at_full_size(hip_ctx, case):
         bad += H.compare_state(ora, net, cfg, "grown-again")
         print(f"[parity] {case}: {spikes} spikes, {len(bad)} differences", flush=True)
         assert spikes > min_spikes
+        assert not bad, bad[:10]
+    finally:
+        net.close()
+        ora.close()
+
+
+def test_weight_update_with_erasures_at_full_size(hip_ctx):
+    """The 10,000-neuron benchmark network (8,440,432 synapses after the first growth update) given
+    a caller's matrix that erases about 5% of the existing synapses, re-weights the rest and creates
+    about 10,000 new ones in the freed and the free slots; then 270 steps over the holed rows and a
+    growth update (overlap areas) over them. Every state array equals the oracle's after each stage."""
+    side, M, radius, seed = 100, 1000, 8.9, 1
+    N = side * side
+    cfg = O.default_config()
+    cfg.update(width=side, height=side, max_synapses_per_neuron=M, max_firing_rate=1000, num_epochs=1,
+               epoch_duration=10.0, seed=seed, layout="DynamicLayout", frac_exc=0.9, frac_starter=0.1,
+               noise_mode=1, math_mode=1, sum_mode=1)
+    ora = O.OracleNet(cfg)
+    ora.set_radii(radius)
+    net = H.gpu_net_from_oracle(hip_ctx, ora, cfg)
+    try:
+        print("[parity] 10k erasures: growth update", flush=True)
+        ora.update_connections()
+        net.update_connections()
+        use = ora.array("in_use").astype(bool)
+        slots = np.flatnonzero(use)
+        src = ora.array("sourceNeuronIndex")[slots].astype(np.int64)
+        dst = slots // M
+        rng = np.random.default_rng(17)
+        Wm = np.zeros((N, N), np.float32)
+        new = rng.random((N, N), dtype=np.float32) < np.float32(1e-4)
+        Wm[new] = rng.uniform(0.5, 2.0, int(new.sum())).astype(np.float32)
+        Wm[src, dst] = rng.uniform(0.1, 3.0, slots.size).astype(np.float32)
+        gone = rng.random(slots.size) < 0.05
+        Wm[src[gone], dst[gone]] = -1.0
+        existing = np.zeros((N, N), bool)
+        existing[src, dst] = True
+        print("[parity] 10k erasures: weight update", flush=True)
+        ora.update_weights(Wm)
+        st = net.update_weights(Wm)
+        print(f"[parity] 10k erasures: {st.created} created, {st.erased} erased, {st.total_synapses} synapses",
+              flush=True)
+        assert st.erased == int(gone.sum()) > 0
+        assert st.created == int(((Wm > 0) & ~existing).sum()) > 0
+        assert st.total_synapses == ora.total_synapses
+        del Wm, new, existing
+        bad = H.compare_state(ora, net, cfg, "weights")
+        print("[parity] 10k erasures: 270 steps", flush=True)
+        ora.advance(270)
+        net.advance(270)
+        net.sync()
+        bad += H.compare_state(ora, net, cfg, "advanced")
+        ora.update_connections()
+        net.update_connections()
+        bad += H.compare_state(ora, net, cfg, "grown-again")
+        print(f"[parity] 10k erasures: {len(bad)} differences", flush=True)
         assert not bad, bad[:10]
     finally:
         net.close()
