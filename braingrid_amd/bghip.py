@@ -140,6 +140,7 @@ def load():
         "bghip_advance": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "bghip_update_connections": (ctypes.c_int, [vp, ctypes.POINTER(GrowthStats)]),
         "bghip_update_weights": (ctypes.c_int, [vp, _fp, ctypes.POINTER(GrowthStats)]),
+        "bghip_update_weights_static": (ctypes.c_int, [vp, _fp, ctypes.POINTER(GrowthStats)]),
         "bghip_download_spikes": (ctypes.c_int, [vp, _ip, _ip, _qp]),
         "bghip_clear_spike_counts": (ctypes.c_int, [vp]),
         "bghip_update_history": (ctypes.c_int, [vp]),
@@ -195,7 +196,8 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_upload_input", "bghip_download_input", "bghip_recorder_enable", "bghip_recorder_download",
             "bghip_net_save", "bghip_net_load", "bghip_net_get_desc", "bghip_recorder_info",
             "bghip_add_synapses", "bghip_add_synapses_ex", "bghip_batch_autotune", "bghip_batch_group_count", "bghip_batch_get_shape",
-            "bghip_batch_set_shape", "bghip_update_weights", "bghip_download_spikes", "bghip_clear_spike_counts")
+            "bghip_batch_set_shape", "bghip_update_weights", "bghip_download_spikes", "bghip_clear_spike_counts",
+            "bghip_update_weights_static")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -368,6 +370,14 @@ class Net:
         W = np.ascontiguousarray(W, dtype=np.float32)
         st = GrowthStats()
         self._c(load().bghip_update_weights(self.h, _ptr(W, _fp), ctypes.byref(st)), "update_weights")
+        return st
+
+    def update_weights_static(self, W: np.ndarray) -> GrowthStats:
+        """update_weights without device allocation (bghip_update_weights_static: the matrix goes
+        through the library's static per-GPU scratch, so N <= 2048)."""
+        W = np.ascontiguousarray(W, dtype=np.float32)
+        st = GrowthStats()
+        self._c(load().bghip_update_weights_static(self.h, _ptr(W, _fp), ctypes.byref(st)), "update_weights_static")
         return st
 
     def update_history(self):

@@ -11,11 +11,39 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 #include "../../include/bghip.h"
 #include "bghip_kernels_epoch.h"
 #include "bghip_launch.h"
+
+// Scratch memory of bghip_update_weights_static: the caller's N x N weight matrix for N up to
+// BG_SCRATCH_MAX_N. One array per GPU, held until the process ends. In host code `scratch` only
+// names the symbol; its device address on each GPU is looked up once per device
+// (hipGetSymbolAddress, at context creation with that device current) and only that address is
+// passed to copies and kernels.
+#define BG_SCRATCH_MAX_N 2048
+#define BG_SCRATCH_BYTES ((size_t)BG_SCRATCH_MAX_N * BG_SCRATCH_MAX_N * sizeof(float))
+#define BG_MAX_DEVICES 64
+static __device__ __align__(256) char scratch[BG_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[BG_MAX_DEVICES];
+static hipError_t g_scratch_err[BG_MAX_DEVICES];
+
+// The device address of `scratch` on the current device `device`, looked up on the first call.
+static void *scratch_address(int device)
+{
+    if (device < 0 || device >= BG_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device] && g_scratch_err[device] == hipSuccess) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+        else g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
 
 // A launch shape the batch autotuner picked, kept per device for groups of the same kind and
 // size class (work items within a factor of two), so re-formed groups do not repeat the trials.
@@ -34,6 +62,7 @@ struct bghip_ctx {
     bool graphs;
     // batches of this context: a destroyed network leaves every batch it is a member of
     std::vector<bghip_batch *> batches;
+    void *scratch;   // device address of the static scratch array on this device (null: unavailable)
 };
 
 struct bghip_net {
@@ -121,6 +150,7 @@ extern "C" int bghip_ctx_create(int device, bghip_ctx **out)
     c->err[0] = 0;
     const char *g = getenv("BGHIP_GRAPHS");
     c->graphs = !(g && strcmp(g, "0") == 0);
+    c->scratch = scratch_address(device);
     *out = c;
     return BGHIP_OK;
 }
@@ -922,6 +952,28 @@ extern "C" int bghip_update_weights(bghip_net *n, const float *W, bghip_growth_s
     // the matrix is freed once the kernels reading it are done
     const hipError_t e = hipStreamSynchronize(n->stream);
     (void)hipFree(Wd);
+    if (rc == BGHIP_OK && e != hipSuccess) rc = set_err(n->ctx, BGHIP_EDEVICE, "%s", hipGetErrorString(e));
+    return rc;
+}
+
+extern "C" int bghip_update_weights_static(bghip_net *n, const float *W, bghip_growth_stats *stats)
+{
+    if (!n || !W) return BGHIP_EINVAL;
+    net_begin(n);
+    if (!n->has_layout) return set_err(n->ctx, BGHIP_EINVAL, "layout not uploaded");
+    const size_t bytes = sizeof(float) * (size_t)n->dn.N * (size_t)n->dn.N;
+    if (bytes > BG_SCRATCH_BYTES)
+        return set_err(n->ctx, BGHIP_EUNSUPPORTED,
+                       "update_weights_static: a %d x %d weight matrix (%zu bytes) exceeds the static scratch of "
+                       "%u bytes (at most %d neurons)", n->dn.N, n->dn.N, bytes, (unsigned)BG_SCRATCH_BYTES,
+                       BG_SCRATCH_MAX_N);
+    float *Wd = (float *)n->ctx->scratch;
+    if (!Wd) return set_err(n->ctx, BGHIP_EDEVICE, "update_weights_static: static scratch address unavailable");
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    flush_pending(n);
+    int rc = apply_caller_weights(n, Wd, W, stats);
+    // the scratch is free for the next call (on any network of this device) once this returns
+    const hipError_t e = hipStreamSynchronize(n->stream);
     if (rc == BGHIP_OK && e != hipSuccess) rc = set_err(n->ctx, BGHIP_EDEVICE, "%s", hipGetErrorString(e));
     return rc;
 }

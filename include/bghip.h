@@ -233,6 +233,14 @@ int bghip_update_connections(bghip_net *net, bghip_growth_stats *stats);
  * included), sources in ascending order; then the incoming map is rebuilt. Radii and rates are not
  * touched. Needs the layout (neuron types); synchronises. */
 int bghip_update_weights(bghip_net *net, const float *W, bghip_growth_stats *stats);
+/* bghip_update_weights without any device allocation: no counterpart in the reference. It is for
+ * callers that cannot allocate device memory (tight serving loops, code that must not call
+ * hipMalloc/hipFree). The device copy of W goes into a static scratch array of the library
+ * (16 MiB per GPU, held until the process ends), so N is at most 2048; a larger matrix returns
+ * BGHIP_EUNSUPPORTED. Same kernels, results identical to bghip_update_weights. Every call on one GPU
+ * shares that GPU's scratch: calls on networks of the same GPU must not overlap (from different
+ * threads they race); each call synchronises the network's stream before it returns. */
+int bghip_update_weights_static(bghip_net *net, const float *W, bghip_growth_stats *stats);
 /* The model part of GPUSpikingModel::updateHistory (Core/GPUSpikingModel.cu:266-272): applies the
  * recorder's minRadius clamp (Recorders/XmlGrowthRecorder.cpp:89-90) and clears spike counts
  * (Neurons/AllSpikingNeurons.cpp:79-87). Download spikes and rates/radii before calling it. */
