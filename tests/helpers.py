@@ -193,3 +193,48 @@ def izh_static_oracle(neurons: dict, syn: dict, meta: dict, seed: int, epoch_s: 
     ora.view("Wex")[slot] = syn["Wex"][slot]
     ora.rebuild_index_map()
     return ora, cfg
+
+
+def oracle_for_spec(spec: dict) -> tuple[O.OracleNet, dict]:
+    """The oracle, in the deterministic mode the device computes in, holding the network of one
+    sweep or served job specification (braingrid_amd.sweep): the growth family through the oracle's
+    own setup from the same configuration as the host model's, the Izhikevich family through the
+    oracle's addSynapse/createSynapse chain from the builder's draws, with its 10 Hz Poisson input.
+    Recorder histories are sized for the job's whole duration."""
+    from braingrid_amd import networks as NW
+    from braingrid_amd import sweep as SW
+    epochs = int(round(spec["sim_s"] / SW.EPOCH_S))
+    if spec["kind"] == "lif_growth":
+        c = O.default_config()
+        c.update(width=spec["side"], height=spec["side"], layout="DynamicLayout", frac_exc=spec["fE"],
+                 frac_starter=0.1, target_rate=spec["tR"], seed=spec["seed"], epoch_duration=SW.EPOCH_S,
+                 num_epochs=epochs, max_synapses_per_neuron=SW.LIF_M, max_firing_rate=SW.LIF_MAX_RATE,
+                 noise_mode=1, math_mode=1, sum_mode=1)
+        return O.OracleNet(c), c
+    N = spec["side"] ** 2
+    neu, syn, meta = NW.izh_stdp_arrays(N, spec["seed"], k=min(100, N - 1), max_rate=SW.IZH_MAX_RATE,
+                                        epoch_s=SW.EPOCH_S)
+    return izh_static_oracle(neu, syn, meta, spec["seed"], epoch_s=SW.EPOCH_S, max_rate=SW.IZH_MAX_RATE,
+                             num_epochs=epochs,
+                             input=dict(kind="poisson", fr_mean=SW.POISSON_HZ, weight=SW.POISSON_WEIGHT))
+
+
+def run_spec_in_oracle(spec: dict) -> dict:
+    """A job of `spec` run start to end in the oracle, epoch by epoch as the served form and the
+    sweep run it (advance one epoch, growth update for the growth family, history step); returns
+    its recorder output (and final radii for the growth family), the served job's result fields."""
+    from braingrid_amd import sweep as SW
+    ora, _ = oracle_for_spec(spec)
+    try:
+        steps = int(round(SW.EPOCH_S / SW.DT))
+        for _ in range(int(round(spec["sim_s"] / SW.EPOCH_S))):
+            ora.advance(steps)
+            if spec["kind"] == "lif_growth":
+                ora.update_connections()
+            ora.update_history()
+        out = {k: ora.array(k).copy() for k in ("burstinessHist", "spikesHistory")}
+        if spec["kind"] == "lif_growth":
+            out["radii"] = ora.array("radii").copy()
+        return out
+    finally:
+        ora.close()
