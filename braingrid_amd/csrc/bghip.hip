@@ -16,6 +16,7 @@
 #include <vector>
 #include "../../include/bghip.h"
 #include "bghip_kernels_epoch.h"
+#include "bghip_implicit.h"
 #include "bghip_launch.h"
 
 // Scratch memory of bghip_update_weights_static: the caller's N x N weight matrix for N up to
@@ -62,6 +63,9 @@ struct bghip_ctx {
     bool graphs;
     // batches of this context: a destroyed network leaves every batch it is a member of
     std::vector<bghip_batch *> batches;
+    // Spiking/DS networks with 32-bit rings run the implicit-ring kernels; BGHIP_IMPLICIT=0 in the
+    // environment keeps them on the explicit ones (for comparisons)
+    bool implicit;
     void *scratch;   // device address of the static scratch array on this device (null: unavailable)
 };
 
@@ -108,7 +112,16 @@ struct bghip_net {
     // per-step STDP: BG_GRAPH_STEPS steps captured as one graph, valid while dn equals g_dn
     hipGraphExec_t g_exec = nullptr;
     DevNet g_dn;
+    // implicit rings: cohort ids handed out (coh_alloc), those handed out since the last usage scan
+    // was issued (coh_since), and that scan's result landing in pinned memory
+    uint32_t coh_alloc, coh_since;
+    bool coh_scan_pending;
+    uint32_t *h_coh_used, *d_coh_used;
+    hipEvent_t coh_ev;
 };
+
+static int to_explicit(bghip_net *n);
+static uint64_t net_total(bghip_net *n);
 
 // Orders the network's own stream after outstanding batch work on it.
 static void net_begin(bghip_net *n)
@@ -150,6 +163,8 @@ extern "C" int bghip_ctx_create(int device, bghip_ctx **out)
     c->err[0] = 0;
     const char *g = getenv("BGHIP_GRAPHS");
     c->graphs = !(g && strcmp(g, "0") == 0);
+    const char *im = getenv("BGHIP_IMPLICIT");
+    c->implicit = !(im && strcmp(im, "0") == 0);
     c->scratch = scratch_address(device);
     *out = c;
     return BGHIP_OK;
@@ -235,6 +250,14 @@ static bool ev_offsets(const std::vector<EvPart> &parts, std::vector<uint32_t> &
     return as < (1ull << 32);
 }
 
+// Networks whose synapses can run with implicit delay rings: Spiking or DS synapses (STDP needs
+// per-step kernels), the reference's 32-bit rings, fused blocks.
+static bool impl_capable(const bghip_net_desc &d)
+{
+    return (d.synapse_model == BGHIP_SYN_SPIKING || d.synapse_model == BGHIP_SYN_DS) && d.ring_bits == 32 &&
+           d.block_steps != 1;
+}
+
 static void plan_net(bghip_net *n, Carve &c)
 {
     const bghip_net_desc &d = n->desc;
@@ -269,6 +292,14 @@ static void plan_net(bghip_net *n, Carve &c)
     c.add(&dn.tbase, 1);
     c.add(&n->d_out_map, S); c.add(&n->d_out_begin, N); c.add(&n->d_out_count, N); c.add(&n->d_cursor, N);
     c.add(&n->d_stats, 4); c.add(&n->d_total, 1);
+    c.add(&n->d_coh_used, 1);
+    if (impl_capable(d)) {
+        c.add(&dn.rec[0], N); c.add(&dn.rec[1], N); c.add(&dn.coh_step, BG_COH_MAX);
+        if (ds) {
+            const size_t C2N = (size_t)BG_COH_MAX * 2 * N;
+            c.add(&dn.cst_r, C2N); c.add(&dn.cst_u, C2N); c.add(&dn.cst_last, C2N); c.add(&dn.cfac, 2 * C2N);
+        }
+    }
 }
 
 extern "C" int64_t bghip_net_footprint(const bghip_net_desc *desc)
@@ -290,6 +321,12 @@ static int block_kmax(const bghip_net *n)
     if (n->desc.block_steps > 0) k = std::min(k, (int)n->desc.block_steps);
     if (n->desc.synapse_model == BGHIP_SYN_STDP || n->desc.synapse_model == BGHIP_SYN_DYNSTDP) k = 1;
     return std::max(k, 1);
+}
+
+static void set_kmax(bghip_net *n)
+{
+    n->kmax = block_kmax(n);
+    n->dn.look = n->kmax;   // implicit rings: next-block window prepared by the source wavefronts
 }
 
 extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghip_net **out)
@@ -326,6 +363,7 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
         dn.tF[t] = desc->types.F[t];
     }
     dn.ds_typed = 1;
+    for (int t = 0; t < 4; t++) dn.tdelay[t] = desc->types.total_delay[t];
     dn.self = n->d_self;
     dn.out_begin = n->d_out_begin;
     dn.out_count = n->d_out_count;
@@ -337,7 +375,9 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
         delete n;
         return set_err(ctx, BGHIP_EDEVICE, "hipStreamCreate failed");
     }
-    if (hipHostMalloc((void **)&n->h_total, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&n->h_total, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&n->h_coh_used, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        (void)hipHostFree(n->h_total);
         (void)hipStreamDestroy(n->stream);
         (void)hipFree(n->arena);
         delete n;
@@ -348,6 +388,7 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
     HIP_TRY(ctx, hipEventCreate(&n->ev0));
     HIP_TRY(ctx, hipEventCreate(&n->ev1));
     HIP_TRY(ctx, hipEventCreateWithFlags(&n->ext_ev, hipEventDisableTiming));
+    HIP_TRY(ctx, hipEventCreateWithFlags(&n->coh_ev, hipEventDisableTiming));
     // zero everything (empty synapse table, zero rings/counters), then the sentinels
     HIP_TRY(ctx, hipMemsetAsync(n->arena, 0, n->arena_bytes, n->stream));
     HIP_TRY(ctx, hipMemsetAsync(dn.fired_step, 0xFF, sizeof(uint64_t) * dn.N, n->stream));
@@ -361,7 +402,8 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
                                     n->stream));
     }
     HIP_TRY(ctx, hipStreamSynchronize(n->stream));
-    n->kmax = block_kmax(n);
+    set_kmax(n);
+    dn.impl = impl_capable(*desc) && ctx->implicit && n->kmax >= 2 ? 1 : 0;
     *out = n;
     return BGHIP_OK;
 }
@@ -384,7 +426,9 @@ extern "C" void bghip_net_destroy(bghip_net *n)
     (void)hipEventDestroy(n->ev0);
     (void)hipEventDestroy(n->ev1);
     (void)hipEventDestroy(n->ext_ev);
+    if (n->coh_ev) (void)hipEventDestroy(n->coh_ev);
     (void)hipStreamDestroy(n->stream);
+    (void)hipHostFree(n->h_coh_used);
     free(n->in_idx_host);
     free(n->in_mask_host);
     (void)hipHostFree(n->h_total);
@@ -445,6 +489,8 @@ extern "C" int bghip_upload_neurons(bghip_net *n, const bghip_neurons *h)
     if (izh && (!h->Aconst || !h->Bconst || !h->Cconst || !h->Dconst || !h->u || !h->C3))
         return set_err(n->ctx, BGHIP_EINVAL, "upload_neurons: missing Izhikevich field");
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    // the synapses' rings must keep their spikes while the neurons' histories are replaced
+    if (n->dn.impl && net_total(n) > 0) CHK(to_explicit(n));
     // refractory length, AllLIFNeurons::fire (Neurons/AllLIFNeurons.cpp:73), computed on the host
     std::vector<int32_t> rs(N);
     for (size_t i = 0; i < N; i++) rs[i] = (int32_t)((double)(h->Trefract[i] / n->desc.dt) + 0.5);
@@ -594,6 +640,10 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
     }
     n->min_delay_uploaded = cnt.empty() ? 0 : min_delay;
     dn.ds_typed = typed;
+    // uploaded synapse state (rings, DS state) is taken as it is: the explicit kernels
+    bool any = false;
+    for (int d = 0; d < N && !any; d++) any = cnt[d] != 0;
+    if (any) dn.impl = 0;
     CHK(h2d(n, dn.meta, meta.data(), S)); CHK(h2d(n, dn.phase, phase.data(), S));
     CHK(h2d(n, dn.ring, ring.data(), (size_t)W32 * S)); CHK(h2d(n, dn.psr, h->psr, S)); CHK(h2d(n, dn.W, h->W, S));
     CHK(h2d(n, dn.syn_count, cnt.data(), N)); CHK(h2d(n, dn.slot_hi, hi.data(), N));
@@ -610,7 +660,7 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
     }
     n->step = step;
     n->pending = false;
-    n->kmax = block_kmax(n);
+    set_kmax(n);
     CHK(rebuild_incoming(n));
     HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));   // the staging vectors above go out of scope
     return BGHIP_OK;
@@ -654,6 +704,84 @@ static void flush_pending(bghip_net *n)
     n->pending = false;
 }
 
+// Implicit rings: writes the per-synapse rings and DS state the network's fire words and cohort
+// states stand for at the current step into the slot arrays (for downloads and before the explicit
+// kernels take over).
+static int materialize(bghip_net *n)
+{
+    const DevNet &dn = n->dn;
+    const unsigned blocks = (unsigned)((dn.S + BG_BLOCK - 1) / BG_BLOCK);
+    if (blocks) hipLaunchKernelGGL(k_materialize, dim3(blocks), dim3(BG_BLOCK), 0, n->stream, dn, n->step, n->sel);
+    HIP_TRY(n->ctx, hipGetLastError());
+    return BGHIP_OK;
+}
+
+// Moves an implicit-ring network to the explicit kernels for the rest of its life (an upload of
+// synapse or neuron state, per-synapse delays, more live cohorts than the meta bits hold, per-step
+// blocks): its slot arrays then hold the complete rings, so the next block has nothing to pull.
+static int to_explicit(bghip_net *n)
+{
+    if (!n->dn.impl) return BGHIP_OK;
+    CHK(materialize(n));
+    n->dn.impl = 0;
+    n->pending = false;
+    return BGHIP_OK;
+}
+
+// Issues a pass over the slots for the cohort ids still in use; its result is read at the next
+// cohort allocation.
+static int coh_scan(bghip_net *n)
+{
+    if (!n->dn.impl) return BGHIP_OK;
+    HIP_TRY(n->ctx, hipMemsetAsync(n->d_coh_used, 0, sizeof(uint32_t), n->stream));
+    const unsigned blocks = (unsigned)std::min<uint64_t>(1024, (n->dn.S + BG_BLOCK - 1) / BG_BLOCK + 1);
+    hipLaunchKernelGGL(k_coh_usage, dim3(blocks), dim3(BG_BLOCK), 0, n->stream, n->dn, n->d_coh_used);
+    HIP_TRY(n->ctx, hipGetLastError());
+    HIP_TRY(n->ctx, hipMemcpyAsync(n->h_coh_used, n->d_coh_used, sizeof(uint32_t), hipMemcpyDeviceToHost, n->stream));
+    HIP_TRY(n->ctx, hipEventRecord(n->coh_ev, n->stream));
+    n->coh_scan_pending = true;
+    n->coh_since = 0;
+    return BGHIP_OK;
+}
+
+static int coh_collect(bghip_net *n)
+{
+    if (!n->coh_scan_pending) return BGHIP_OK;
+    HIP_TRY(n->ctx, hipEventSynchronize(n->coh_ev));
+    n->coh_alloc = *n->h_coh_used | n->coh_since;
+    n->coh_scan_pending = false;
+    n->dn.coh_mask = n->coh_alloc;
+    return BGHIP_OK;
+}
+
+// Before a call that creates synapses on an implicit-ring network: a free cohort id for them, its
+// creation step and createSynapse state (queued on the network's stream), in cc.coh. Ids of
+// cohorts whose synapses are all gone come back through the usage scans; with every id live, the
+// network moves to the explicit kernels.
+static int coh_begin(bghip_net *n, CreateCtx &cc)
+{
+    cc.coh = 0;
+    if (!n->dn.impl) return BGHIP_OK;
+    CHK(coh_collect(n));
+    const uint32_t all = (1u << BG_COH_MAX) - 1u;
+    if (!(~n->coh_alloc & all)) {
+        CHK(coh_scan(n));
+        CHK(coh_collect(n));
+        if (!(~n->coh_alloc & all)) return to_explicit(n);
+    }
+    const int c = __builtin_ffs((int)(~n->coh_alloc & all)) - 1;
+    n->coh_alloc |= 1u << c;
+    n->coh_since |= 1u << c;
+    DevNet &dn = n->dn;
+    dn.coh_mask = n->coh_alloc;
+    dn.young_until = std::max<uint64_t>(dn.young_until, n->step + 64);
+    const unsigned blocks = (unsigned)((2 * (uint64_t)dn.N + BG_BLOCK - 1) / BG_BLOCK);
+    hipLaunchKernelGGL(k_coh_init, dim3(blocks), dim3(BG_BLOCK), 0, n->stream, dn, c, n->step, cc.r0, cc.u0);
+    HIP_TRY(n->ctx, hipGetLastError());
+    cc.coh = c;
+    return BGHIP_OK;
+}
+
 // Captures BG_GRAPH_STEPS per-step STDP steps (offsets 0..S-1 from *tbase) and the tbase bump as
 // one executable graph. Capture records only the kernels; work already queued on the stream stays
 // ordered before each replay.
@@ -685,6 +813,7 @@ extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
     DevNet &dn = n->dn;
     const int nk = n->desc.neuron_model, sk = n->desc.synapse_model;
     const bool per_step = n->kmax <= 1 || sk == BGHIP_SYN_STDP || sk == BGHIP_SYN_DYNSTDP;
+    if (dn.impl && per_step) CHK(to_explicit(n));
     if (net_total(n) == 0) {
         // no synapses: neurons only, many steps per launch
         flush_pending(n);
@@ -730,6 +859,20 @@ extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
             }
         }
         n->step += n_steps;
+    } else if (dn.impl) {
+        // implicit rings: nothing to pull or flush between blocks or calls
+        for (uint64_t done = 0; done < n_steps;) {
+            const int K = (int)std::min<uint64_t>((uint64_t)n->kmax, n_steps - done);
+            const uint64_t t0 = n->step + done;
+            if (nk == BGHIP_NEURON_LIF) launch_fused_impl_lif(dn, n->stream, sk, n->kmax, t0, K, n->sel);
+            else launch_fused_impl_izh(dn, n->stream, sk, n->kmax, t0, K, n->sel);
+            n->sel ^= 1;
+            n->last_t0 = t0;
+            done += (uint64_t)K;
+            n->timed_launches += 1;
+        }
+        n->step += n_steps;
+        n->pending = false;
     } else {
         uint64_t done = 0;
         while (done < n_steps) {
@@ -846,6 +989,10 @@ extern "C" int bghip_add_synapses_ex(bghip_net *n, int64_t count, const int32_t 
     }
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     flush_pending(n);
+    // per-synapse delays break the shared delivery times of a cohort: explicit rings
+    if (total_delay) CHK(to_explicit(n));
+    CreateCtx cc = create_ctx(n);
+    CHK(coh_begin(n, cc));
     const int nd = (int)dests.size();
     char *buf = nullptr;
     const size_t o_dest = 0, o_begin = align256(sizeof(int32_t) * nd),
@@ -864,7 +1011,7 @@ extern "C" int bghip_add_synapses_ex(bghip_net *n, int64_t count, const int32_t 
         rc = set_err(n->ctx, BGHIP_EDEVICE, "add_synapses: upload");
     } else {
         const unsigned grid = (unsigned)(((int64_t)nd * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
-        hipLaunchKernelGGL(k_add_synapses, dim3(grid), dim3(BG_BLOCK), 0, n->stream, n->dn, create_ctx(n),
+        hipLaunchKernelGGL(k_add_synapses, dim3(grid), dim3(BG_BLOCK), 0, n->stream, n->dn, cc,
                            (const int32_t *)(buf + o_dest), (const int64_t *)(buf + o_begin),
                            (const int32_t *)(buf + o_src), (const float *)(buf + o_w),
                            total_delay ? (const int32_t *)(buf + o_d) : nullptr,
@@ -876,9 +1023,10 @@ extern "C" int bghip_add_synapses_ex(bghip_net *n, int64_t count, const int32_t 
     if (rc) return rc;
     if (dmin_new) {   // per-synapse delays bound the fused block length like uploaded ones
         n->min_delay_uploaded = n->min_delay_uploaded ? std::min(n->min_delay_uploaded, dmin_new) : dmin_new;
-        n->kmax = block_kmax(n);
+        set_kmax(n);
     }
     CHK(rebuild_incoming(n));
+    CHK(coh_scan(n));
     return BGHIP_OK;
 }
 
@@ -905,7 +1053,8 @@ extern "C" int bghip_update_connections(bghip_net *n, bghip_growth_stats *stats)
 static int apply_weights(bghip_net *n, const float *Wd, bghip_growth_stats *stats)
 {
     DevNet &dn = n->dn;
-    const CreateCtx cc = create_ctx(n);
+    CreateCtx cc = create_ctx(n);
+    CHK(coh_begin(n, cc));
     HIP_TRY(n->ctx, hipMemsetAsync(n->d_stats, 0, 16, n->stream));
     if (dn.M > 0) {
         unsigned grid = (unsigned)std::min(dn.N, 4096);
@@ -913,6 +1062,7 @@ static int apply_weights(bghip_net *n, const float *Wd, bghip_growth_stats *stat
     }
     HIP_TRY(n->ctx, hipGetLastError());
     CHK(rebuild_incoming(n));
+    CHK(coh_scan(n));
     if (stats) {
         uint32_t st[4];
         HIP_TRY(n->ctx, hipMemcpyAsync(st, n->d_stats, 16, hipMemcpyDeviceToHost, n->stream));
@@ -1129,6 +1279,7 @@ extern "C" int bghip_download_synapses(bghip_net *n, bghip_synapses *h)
     if (!n || !h) return BGHIP_EINVAL;
     net_begin(n);
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    if (n->dn.impl) CHK(materialize(n));
     DevNet &dn = n->dn;
     const int N = dn.N, M = dn.M, R = dn.R, W32 = dn.W32;
     const size_t S = dn.S;
@@ -1355,7 +1506,7 @@ extern "C" int bghip_download_input(bghip_net *n, bghip_input_state *st)
 // Philox noise and Poisson draws are functions of (seed, neuron, step), so the step counter is the
 // whole random-number state.
 static const char kCkptMagic[8] = {'B', 'G', 'H', 'I', 'P', 'N', 'E', 'T'};
-static const uint32_t kCkptVersion = 1;
+static const uint32_t kCkptVersion = 2;
 
 struct CkptHost {   // fixed-size record of the host-side state
     uint64_t step, last_t0, total, hist_step0, in_cyc_off;
@@ -1366,6 +1517,9 @@ struct CkptHost {   // fixed-size record of the host-side state
     bghip_growth growth;
     float in_W, in_lambda, in_decay[2], in_U[2], in_D[2], in_F[2];
     int32_t in_delay[2];
+    int32_t impl;            // implicit rings (the fire words and cohort states are in the arena)
+    uint32_t coh_alloc;
+    uint64_t young_until;
 };
 
 template <typename T>
@@ -1380,9 +1534,11 @@ extern "C" int bghip_net_save(bghip_net *n, const char *path)
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     CHK(bghip_sync(n));
     flush_pending(n);   // a pending block's spikes go into the rings first
+    CHK(coh_collect(n));
     const DevNet &dn = n->dn;
     CkptHost h;
     memset(&h, 0, sizeof h);
+    h.impl = dn.impl; h.coh_alloc = n->coh_alloc; h.young_until = dn.young_until;
     h.step = n->step; h.last_t0 = n->last_t0; h.total = net_total(n); h.hist_step0 = n->hist_step0;
     h.in_cyc_off = dn.in_cyc_off; h.n_burst = n->n_burst; h.n_shist = n->n_shist;
     h.kmax = n->kmax; h.sel = n->sel; h.min_delay_uploaded = n->min_delay_uploaded; h.ds_typed = dn.ds_typed;
@@ -1466,7 +1622,8 @@ extern "C" int bghip_net_load(bghip_ctx *ctx, const char *path, bghip_net **out)
         return set_err(ctx, BGHIP_EDEVICE, "net_load: upload failed");
     }
     n->step = h.step; n->last_t0 = h.last_t0; n->total = h.total; n->total_pending = false;
-    n->hist_step0 = h.hist_step0; n->kmax = h.kmax; n->sel = h.sel; n->min_delay_uploaded = h.min_delay_uploaded;
+    dn.impl = h.impl; n->coh_alloc = h.coh_alloc; dn.coh_mask = h.coh_alloc; dn.young_until = h.young_until;
+    n->hist_step0 = h.hist_step0; n->kmax = h.kmax; dn.look = h.kmax; n->sel = h.sel; n->min_delay_uploaded = h.min_delay_uploaded;
     n->pending = h.pending; n->has_layout = h.has_layout; n->has_growth = h.has_growth; n->growth = h.growth;
     n->hist_range_known = h.hist_range_known; n->in_phase = h.in_phase; n->out_valid = false;
     dn.ds_typed = h.ds_typed; dn.in_kind = h.in_kind; dn.in_dur = h.in_dur; dn.in_cycle = h.in_cycle;
@@ -1512,6 +1669,7 @@ static int net_mode(const bghip_net *x)
 }
 
 struct BatchGroup {
+    int impl = 0;   // fused groups: implicit-ring members
     int nk, sk, w32;
     int mode;
     std::vector<int> members;      // indices into bghip_batch::nets
@@ -1554,6 +1712,7 @@ struct bghip_batch {
     std::vector<BatchGroup> groups;
     std::vector<int> modes;        // net_mode of each member when the groups were formed
     std::vector<int> kmaxs;        // and its block length (a group's is the minimum of its members')
+    std::vector<int> impls;        // and whether it ran with implicit rings
     bool stale = false;            // a member was destroyed: the groups must be formed again
 };
 
@@ -1665,18 +1824,23 @@ static int regroup(bghip_batch *b)
     const int n = (int)b->nets.size();
     b->modes.assign(n, 0);
     b->kmaxs.assign(n, 0);
+    b->impls.assign(n, 0);
     for (int i = 0; i < n; i++) {
         bghip_net *x = b->nets[i];
         const int mode = net_mode(x);
         b->modes[i] = mode;
         b->kmaxs[i] = x->kmax;
+        b->impls[i] = x->dn.impl;
         const int sk = mode == BG_MODE_QUIET ? -1 : x->desc.synapse_model;
         const int w32 = mode == BG_MODE_QUIET ? 0 : x->dn.W32;
+        const int impl = mode == BG_MODE_FUSED ? x->dn.impl : 0;
         BatchGroup *grp = nullptr;
         for (auto &g : b->groups)
-            if (g.nk == x->desc.neuron_model && g.sk == sk && g.w32 == w32 && g.mode == mode) grp = &g;
+            if (g.nk == x->desc.neuron_model && g.sk == sk && g.w32 == w32 && g.mode == mode && g.impl == impl)
+                grp = &g;
         if (!grp) {
             BatchGroup g;
+            g.impl = impl;
             g.nk = x->desc.neuron_model;
             g.sk = sk;
             g.w32 = w32;
@@ -1773,7 +1937,9 @@ static bool members_changed(const bghip_batch *b)
 {
     if (b->stale || b->modes.size() != b->nets.size()) return true;
     for (size_t i = 0; i < b->nets.size(); i++)
-        if (net_mode(b->nets[i]) != b->modes[i] || b->nets[i]->kmax != b->kmaxs[i]) return true;
+        if (net_mode(b->nets[i]) != b->modes[i] || b->nets[i]->kmax != b->kmaxs[i] ||
+            b->nets[i]->dn.impl != b->impls[i])
+            return true;
     return false;
 }
 
@@ -1995,23 +2161,26 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
         } else {
             const int Kfull = std::max(1, g.kmax);
             const uint64_t L = (n_steps + (uint64_t)Kfull - 1) / (uint64_t)Kfull;
+            auto fused_launch = [&](const BatchMap &m, const BatchShape &sh, int K, int launch) {
+                if (g.impl) launch_fused_impl_batch(m, b->stream, sh, g.nk, g.sk, g.kmax, K, Kfull, launch);
+                else launch_fused_batch(m, b->stream, sh, g.nk, g.sk, g.w32, g.kmax, K, Kfull, launch);
+            };
             uint64_t l = 0;
             const uint64_t TL = std::max<uint64_t>(4, (T + (uint64_t)Kfull - 1) / (uint64_t)Kfull);
             if (tune && L > 4 * (TL + 1)) {
                 // trial: one launch (first of the shape), then TL timed launches of Kfull steps
                 CHK(tune_group(b, g, fused_space(), [&](const BatchShape &sh, float &us) {
                     const BatchMap m = batch_map(g, sh);
-                    launch_fused_batch(m, b->stream, sh, g.nk, g.sk, g.w32, g.kmax, Kfull, Kfull, (int)l++);
+                    fused_launch(m, sh, Kfull, (int)l++);
                     return time_steps(b, us, TL * (uint64_t)Kfull, [&] {
-                        for (uint64_t i = 0; i < TL; i++)
-                            launch_fused_batch(m, b->stream, sh, g.nk, g.sk, g.w32, g.kmax, Kfull, Kfull, (int)l++);
+                        for (uint64_t i = 0; i < TL; i++) fused_launch(m, sh, Kfull, (int)l++);
                     });
                 }));
             }
             const BatchMap bm = batch_map(g, g.shape);
             for (; l < L; l++) {
                 const int K = (int)std::min<uint64_t>((uint64_t)Kfull, n_steps - l * (uint64_t)Kfull);
-                launch_fused_batch(bm, b->stream, g.shape, g.nk, g.sk, g.w32, g.kmax, K, Kfull, (int)l);
+                fused_launch(bm, g.shape, K, (int)l);
             }
             uint64_t *tprev = g.h_tprev;
             int *sel = g.h_sel;
@@ -2028,7 +2197,7 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
             HIP_TRY(b->ctx, hipMemcpyAsync(g.d_sel, sel, sizeof(int) * nn, hipMemcpyHostToDevice, b->stream));
             uint64_t slots = 0;
             for (int k = 0; k < nn; k++) slots += b->nets[g.members[k]]->dn.S;
-            if (slots) {
+            if (slots && !g.impl) {   // implicit rings have nothing to flush
                 unsigned blocks = (unsigned)((slots + BG_BLOCK - 1) / BG_BLOCK);
                 hipLaunchKernelGGL(k_flush_batch, dim3(blocks), dim3(BG_BLOCK), 0, b->stream, g.d_nets, g.d_sbase, nn,
                                    g.d_tprev, g.d_sel);

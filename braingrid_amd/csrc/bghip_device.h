@@ -18,6 +18,10 @@
 #define BG_META_TYPE_SHIFT 18
 #define BG_META_DELAY_SHIFT 20
 #define BG_META_IN_USE 0x80000000u
+// Implicit-ring networks keep each synapse's cohort (the growth update or synapse-list call that
+// created it) in the three meta bits between total_delay and in_use.
+#define BG_META_COH_SHIFT 28
+#define BG_COH_MAX 8
 #define BG_MAX_NEURONS 262144
 #define BG_NO_SPIKE 0xFFFFFFFFFFFFFFFFull
 
@@ -87,6 +91,23 @@ struct DevNet {
     const uint32_t *out_begin, *out_count, *out_map;   // outgoing map (synapses of each source)
     const DevNet *self;          // this descriptor in device memory (for kernels that call out of line)
     uint64_t *tbase;             // per-step STDP launches: step = *tbase + the launch's offset (graph replays)
+    // Implicit delay rings (Spiking/DS networks with 32-bit rings, fused blocks). A synapse's ring
+    // holds exactly the spikes its source fired in the last total_delay steps since the synapse was
+    // created, so it is read from the source's recent fire bits instead of being stored; and every
+    // synapse of one (source, type, cohort) sees the same deliveries, so their Tsodyks-Markram state
+    // (r, u, lastSpike) is one value, advanced by the source's wavefront.
+    int impl;                    // 1: this network runs the implicit-ring kernels
+    int look;                    // steps of the next block whose first delivery factor is prepared
+    uint32_t coh_mask;           // cohort ids in use
+    uint64_t young_until;        // blocks starting earlier may see spikes from before a cohort's creation
+    int32_t tdelay[4];           // total delay of each synapse type (every synapse has its type's)
+    uint64_t *rec[2];            // N: fire bits of the 64 steps before a block (bit j = step t0 - 64 + j),
+                                 // by block parity like mask[]
+    uint64_t *coh_step;          // BG_COH_MAX: creation step of each cohort
+    float *cst_r, *cst_u;        // DS: state per (cohort, target class, source) after the last block,
+    uint64_t *cst_last;          //     index (c * 2 + class) * N + source; class = type & 1
+    float2 *cfac;                // DS: (u, r) after the first delivery of the next block, by block
+                                 //     parity: [(parity * BG_COH_MAX + c) * 2 + class] * N + source
 };
 
 enum { BG_INPUT_NONE = 0, BG_INPUT_POISSON = 1, BG_INPUT_REGULAR = 2 };

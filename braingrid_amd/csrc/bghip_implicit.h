@@ -1,0 +1,383 @@
+// Fused block kernels with implicit delay rings, for Spiking/DS networks with 32-bit rings.
+//
+// The reference keeps a delay ring per synapse (AllSpikingSynapses::delayQueue,
+// Synapses/AllSpikingSynapses.h:354-365): preSpikeHit sets bit delay ahead when the source fires
+// (Synapses/AllSpikingSynapses.cpp:261-279) and advanceSynapse tests and clears the current bit
+// (:242-254, 297-319). At a step boundary T the ring of a synapse therefore holds exactly the spikes
+// its source fired at steps s in [T - total_delay, T) with s at or after the synapse's creation
+// step (a new synapse starts with an empty ring). Here each neuron keeps the fire bits of its last
+// 64 steps (rec[], written once per block by the neuron's own wavefront) and a synapse reads its
+// deliveries of a block from its source's word: no ring is stored, read or written.
+//
+// AllDSSynapses::changePSR (Synapses/AllDSSynapses.cpp:227-247) advances r, u and lastSpike at
+// each delivery from the interval since the previous one. All synapses of one source, one type
+// (so one delay and one U, D, F) and one creation step start from the same createSynapse values
+// and see the same deliveries, so their r, u, lastSpike are the same numbers at every step. The
+// source's wavefront advances that state once per (cohort, target class) at the end of each block,
+// and prepares (u, r) after the first delivery of the next block; a delivering synapse then adds
+// ((W / decay) * u) * r exactly as changePSR does, with the same operands. A synapse that receives
+// a second spike inside one block (a source refractory period shorter than the block) advances a
+// private copy of that state for the later ones, with the same arithmetic.
+//
+// Downloads and every switch to the explicit kernels first write the per-synapse rings and DS
+// state these representations stand for into the slot arrays (k_materialize), bit for bit the
+// reference's.
+#pragma once
+#include "bghip_kernels.h"
+
+__device__ __forceinline__ int meta_coh(uint32_t meta) { return (int)((meta >> BG_META_COH_SHIFT) & (BG_COH_MAX - 1)); }
+
+// Bits k of a block starting at t0 whose delivery (at t0 + k, delay steps after the spike) comes
+// from a spike at or after step E.
+__device__ __forceinline__ uint32_t since_mask(uint64_t E, uint32_t delay, uint64_t t0)
+{
+    const uint64_t first = E + (uint64_t)delay;
+    if (first <= t0) return 0xFFFFFFFFu;
+    const uint64_t lim = first - t0;
+    return lim >= 32 ? 0u : ~((1u << (uint32_t)lim) - 1u);
+}
+
+// The r, u, lastSpike update of AllDSSynapses::changePSR at a delivery at step t (the same
+// expressions as psr_increment_regs).
+__device__ __forceinline__ void ds_deliver(const DevNet &n, float &r, float &u, uint64_t &last, uint64_t t, float U,
+                                           float D, float F)
+{
+    if (last != BG_NO_SPIKE) {
+        const uint64_t gap = t - last;
+        const float isi = ((gap >> 32) ? (float)gap : (float)(uint32_t)gap) * n.dt;
+        r = 1.0f + (r * (1.0f - u) - 1.0f) * bg_expf(-isi / D);
+        u = U + u * (1.0f - U) * bg_expf(-isi / F);
+    }
+    last = t;
+}
+
+struct ImplRank {
+    uint64_t slot;
+    uint32_t meta;
+    float psr;
+};
+
+__device__ __forceinline__ void impl_hot_load(const DevNet &n, bool dense, uint64_t row, const uint32_t *imap,
+                                              uint32_t q, uint32_t cnt, ImplRank &h)
+{
+    h.slot = 0;
+    h.meta = 0;
+    h.psr = 0.0f;
+    if (q < cnt) {
+        h.slot = dense ? row + q : (uint64_t)imap[q];
+        h.meta = n.meta[h.slot];
+        h.psr = n.psr[h.slot];
+    }
+}
+
+// The state a delivering synapse needs: W and, for DS, the prepared (u, r) of its cohort.
+struct ImplCold {
+    float W;
+    float2 f;
+    bool valid;
+};
+
+template <int SK>
+__device__ __forceinline__ void impl_cold_load(const DevNet &n, const float2 *fac, const ImplRank &h, ImplCold &c)
+{
+    c.W = n.W[h.slot];
+    if (SK == SK_DS) {
+        const uint32_t src = h.meta & BG_META_SRC_MASK;
+        const int type = (h.meta >> BG_META_TYPE_SHIFT) & 3;
+        c.f = fac[((size_t)meta_coh(h.meta) * 2 + (size_t)(type & 1)) * (size_t)n.N + src];
+    }
+    c.valid = true;
+}
+
+// One synapse through the K steps of a block: its deliveries are bits (64 - delay) .. of its
+// source's fire word (bit k = delivery at t0 + k), masked to spikes after its creation in blocks
+// that may reach back before it; then the PSR with the canonical per-step accumulation.
+template <int SK, int KMAX>
+__device__ __forceinline__ void impl_synapse(const DevNet &n, const TypeRegs &tr, const float2 *fac,
+                                             const ImplRank &h, float &psr, uint64_t recent, ImplCold &c,
+                                             uint64_t t0, int K, bool young, float (&acc)[KMAX], uint32_t &hit)
+{
+    const int Kb = BG_K_EXACT(KMAX) ? KMAX : K;
+    const uint32_t kmask = Kb >= 32 ? 0xFFFFFFFFu : ((1u << Kb) - 1u);
+    const uint32_t delay = (h.meta >> BG_META_DELAY_SHIFT) & 0xFF;
+    const int type = (h.meta >> BG_META_TYPE_SHIFT) & 3;
+    uint32_t hm = (uint32_t)(recent >> (64u - delay)) & kmask;
+    if (young && hm) hm &= since_mask(n.coh_step[meta_coh(h.meta)], delay, t0);
+    hit = hm;
+    const float decay = pick4(tr.dec, type);
+    int k1 = -1;
+    float c1 = 0.0f;
+    ColdDS cd;
+    cd.valid = true;
+    if (hm) {
+        if (!c.valid) impl_cold_load<SK>(n, fac, h, c);
+        k1 = __ffs(hm) - 1;
+        if (SK == SK_DS) {
+            c1 = ((c.W / decay) * c.f.x) * c.f.y;
+            cd.W = c.W;
+            cd.u = c.f.x;
+            cd.r = c.f.y;
+            cd.last = t0 + (uint64_t)k1;
+        } else {
+            c1 = c.W / decay;
+            cd.W = c.W;
+        }
+        hm &= hm - 1;
+    }
+    float p = psr;
+    if (__ballot(hm != 0) == 0ull) {
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            if (BG_K_EXACT(KMAX) || k < K) {
+                p = (k == k1) ? p + c1 : p;
+                p = p * decay;
+                acc[k] += p;
+            }
+        }
+    } else {
+        bool ru_changed = false;
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            if (BG_K_EXACT(KMAX) || k < K) {
+                p = (k == k1) ? p + c1 : p;
+                if ((hm >> k) & 1u)
+                    p = p + psr_increment_regs<SK>(n, tr, h.slot, type, t0 + (uint64_t)k, decay, cd, ru_changed);
+                p = p * decay;
+                acc[k] += p;
+            }
+        }
+    }
+    psr = p;
+}
+
+template <int NK, int SK, int KMAX>
+__device__ __forceinline__ void fused_impl_body(const DevNet &n, int d, uint64_t t0, int K, int sel)
+{
+    __shared__ float s_noise[BG_BLOCK / BG_WAVE][KMAX];
+    __shared__ float s_in[BG_BLOCK / BG_WAVE][KMAX];
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    const int wv = threadIdx.x / BG_WAVE;
+    const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
+    const bool dense = (cnt == hi);
+    const uint64_t row = (uint64_t)d * n.M;
+    const uint32_t *imap = n.in_map + (dense ? 0 : n.in_begin[d]);
+    const uint64_t *rec = n.rec[sel];
+    const float2 *fac = n.cfac + (size_t)sel * BG_COH_MAX * 2 * (size_t)n.N;
+    const bool young = t0 < n.young_until;
+    if (lane < K) s_noise[wv][lane] = bg_noise_normal(n.seed, (uint32_t)d, t0 + (uint64_t)lane);
+    float acc[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) acc[k] = 0.0f;
+
+    // Lane-strided ranks in a three-stage pipeline, as in fused_block_body: while rank q is
+    // processed, rank q+64 has its source's fire word (and, while deliveries are frequent, its W
+    // and prepared DS factors) in flight and rank q+128 its meta and psr.
+    TypeRegs tr;
+    type_regs(n, tr);
+    ImplRank x, y, z;
+    ImplCold cx, cy, cz;
+    uint64_t fx = 0, fy = 0, fz = 0;
+    bool spec = false;
+    uint32_t q = lane;
+    impl_hot_load(n, dense, row, imap, q, cnt, x);
+    impl_hot_load(n, dense, row, imap, q + BG_WAVE, cnt, y);
+    cx.valid = false;
+    if (q < cnt) fx = rec[x.meta & BG_META_SRC_MASK];
+    auto pass = [&](ImplRank &cur, ImplCold &ccur, uint64_t &fcur, ImplRank &nxt, ImplCold &cnxt, uint64_t &fnxt,
+                    ImplRank &far) {
+        impl_hot_load(n, dense, row, imap, q + 2 * BG_WAVE, cnt, far);
+        cnxt.valid = false;
+        fnxt = 0;
+        if (q + BG_WAVE < cnt) {
+            fnxt = rec[nxt.meta & BG_META_SRC_MASK];
+            if (spec) impl_cold_load<SK>(n, fac, nxt, cnxt);
+        }
+        uint32_t hit = 0;
+        float p = cur.psr;
+        impl_synapse<SK, KMAX>(n, tr, fac, cur, p, fcur, ccur, t0, K, young, acc, hit);
+        n.psr[cur.slot] = p;
+        spec = __popcll(__ballot(hit != 0)) >= 4;
+        q += BG_WAVE;
+    };
+    while (true) {
+        if (q >= cnt) break;
+        pass(x, cx, fx, y, cy, fy, z);
+        if (q >= cnt) break;
+        pass(y, cy, fy, z, cz, fz, x);
+        if (q >= cnt) break;
+        pass(z, cz, fz, x, cx, fx, y);
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; k++)
+        if (BG_K_EXACT(KMAX) || k < K) acc[k] = wave_sum(acc[k]);
+
+    // this neuron as a source: its fire word, and for DS the state of its outgoing synapses per
+    // (cohort, target class) on lanes 0 .. 2 * cohorts - 1 (loaded now, used after the neuron loop)
+    const uint64_t rec_old = rec[d];
+    const uint32_t cm = n.coh_mask;
+    const int npair = SK == SK_DS ? 2 * __popc(cm) : 0;
+    int pc = 0;
+    size_t pidx = 0;
+    float pr = 0.0f, pu = 0.0f;
+    uint64_t plast = 0, pE = 0;
+    if (lane < npair) {
+        uint32_t m = cm;
+        for (int j = 0; j < (lane >> 1); j++) m &= m - 1;
+        pc = __ffs(m) - 1;
+        pidx = ((size_t)pc * 2 + (size_t)(lane & 1)) * (size_t)n.N + (size_t)d;
+        pr = n.cst_r[pidx];
+        pu = n.cst_u[pidx];
+        plast = n.cst_last[pidx];
+        pE = n.coh_step[pc];
+    }
+
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    uint32_t mask = 0;
+    if (lane == 0) {
+        uint32_t inbits = 0;
+        if (n.in_kind != BG_INPUT_NONE) {
+            for (int k = 0; k < K; k++) {
+                float add;
+                if (input_step(n, d, t0 + (uint64_t)k, &add)) {
+                    s_in[wv][k] = add;
+                    inbits |= 1u << k;
+                }
+            }
+        }
+        NeuronRegs s;
+        neuron_load<NK>(n, d, s);
+        float sp = n.sp[d];
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            if (BG_K_EXACT(KMAX) || k < K) {
+                if ((inbits >> k) & 1u) sp = sp + s_in[wv][k];
+                if (neuron_advance<NK>(n, d, t0 + (uint64_t)k, sp, s_noise[wv][k], s)) mask |= 1u << k;
+                sp = 0.0f + acc[k];
+            }
+        }
+        neuron_store<NK>(n, d, s);
+        n.sp[d] = sp;
+        n.mask[sel ^ 1][d] = mask;
+        if (mask) n.fired_step[d] = t0 + (uint64_t)(31 - __clz(mask));
+    }
+    mask = (uint32_t)__shfl((int)mask, 0);
+    const uint64_t rec_new = (rec_old >> K) | ((uint64_t)mask << (64 - K));
+    if (lane == 0) n.rec[sel ^ 1][d] = rec_new;
+    if (SK == SK_DS && lane < npair) {
+        const int tsub = lane & 1;
+        const int type = (n.ntype[d] == 1 ? 0 : 2) + tsub;
+        const uint32_t delay = (uint32_t)n.tdelay[type];
+        const float U = pick4(tr.U, type), D = pick4(tr.D, type), F = pick4(tr.F, type);
+        // this block's deliveries, as every synapse of the pair saw them
+        const uint32_t kmK = K >= 32 ? 0xFFFFFFFFu : ((1u << K) - 1u);
+        uint32_t hr = (uint32_t)(rec_old >> (64u - delay)) & kmK & since_mask(pE, delay, t0);
+        while (hr) {
+            const int k = __ffs(hr) - 1;
+            hr &= hr - 1;
+            ds_deliver(n, pr, pu, plast, t0 + (uint64_t)k, U, D, F);
+        }
+        n.cst_r[pidx] = pr;
+        n.cst_u[pidx] = pu;
+        n.cst_last[pidx] = plast;
+        // the first delivery of the next block (of up to `look` steps)
+        const uint64_t T = t0 + (uint64_t)K;
+        const uint32_t kmL = n.look >= 32 ? 0xFFFFFFFFu : ((1u << n.look) - 1u);
+        const uint32_t hl = (uint32_t)(rec_new >> (64u - delay)) & kmL & since_mask(pE, delay, T);
+        if (hl) {
+            float r2 = pr, u2 = pu;
+            uint64_t l2 = plast;
+            ds_deliver(n, r2, u2, l2, T + (uint64_t)(__ffs(hl) - 1), U, D, F);
+            n.cfac[(((size_t)(sel ^ 1) * BG_COH_MAX + (size_t)pc) * 2 + (size_t)tsub) * (size_t)n.N + (size_t)d] =
+                make_float2(u2, r2);
+        }
+    }
+}
+
+template <int NK, int SK, int KMAX>
+__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_FUSED_WAVES))) void k_fused_impl(
+    DevNet n, uint64_t t0, int K, int sel)
+{
+    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
+    if (d >= n.N) return;
+    fused_impl_body<NK, SK, KMAX>(n, __builtin_amdgcn_readfirstlane(d), t0, K, sel);
+}
+
+template <int NK, int SK, int KMAX>
+__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_FUSED_WAVES))) void k_fused_impl_batch(
+    BatchMap bm, int K, int Kfull, int launch)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(
+        (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
+    if (w >= bm.total) return;
+    const int i = __builtin_amdgcn_readfirstlane((int)bm.nof[w]);
+    const BatchNet &b = bm.nets[i];
+    const uint64_t t0 = b.t0 + (uint64_t)launch * (uint64_t)Kfull;
+    const int sel = b.sel ^ (launch & 1);
+    fused_impl_body<NK, SK, KMAX>(b.n, (int)(w - bm.base[i]), t0, K, sel);
+}
+
+// The explicit slot state an implicit network stands for at step boundary T (fire words in
+// rec[sel]): each synapse's 32-bit ring in absolute orientation (bit (s + delay) mod 32 for each
+// spike s of its source in [T - delay, T) at or after its creation) and, for DS, its cohort's r, u,
+// lastSpike. One thread per slot.
+static __global__ __launch_bounds__(BG_BLOCK) void k_materialize(DevNet n, uint64_t T, int sel)
+{
+    const uint64_t slot = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (slot >= n.S) return;
+    const uint32_t meta = n.meta[slot];
+    if (!(meta & BG_META_IN_USE)) return;
+    const uint32_t src = meta & BG_META_SRC_MASK;
+    const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
+    const uint32_t delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
+    const int c = meta_coh(meta);
+    const uint64_t E = n.coh_step[c];
+    const uint64_t rec = n.rec[sel][src];
+    uint32_t ring = 0;
+    for (uint32_t j = 1; j <= delay && j <= 64; j++) {
+        if (T < (uint64_t)j) break;
+        const uint64_t s = T - j;
+        if (s < E) break;
+        if ((rec >> (64u - j)) & 1ull) ring |= 1u << ((uint32_t)(s + delay) & 31u);
+    }
+    n.ring[slot] = ring;
+    if (n.cst_r) {
+        const size_t i = ((size_t)c * 2 + (size_t)(type & 1)) * (size_t)n.N + src;
+        n.r[slot] = n.cst_r[i];
+        n.us[slot] = n.cst_u[i];
+        n.last[slot] = n.cst_last[i];
+    }
+}
+
+// A new cohort c created at `step`: its creation step, and for DS the createSynapse state
+// (r0, u0, no spike yet) for every (target class, source). Threads cover 2 N entries.
+static __global__ void k_coh_init(DevNet n, int c, uint64_t step, float r0, float u0)
+{
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i == 0) n.coh_step[c] = step;
+    if (!n.cst_r || i >= 2 * (uint64_t)n.N) return;
+    const size_t k = (size_t)c * 2 * (size_t)n.N + i;
+    n.cst_r[k] = r0;
+    n.cst_u[k] = u0;
+    n.cst_last[k] = BG_NO_SPIKE;
+}
+
+// Cohort ids of the synapses in use (bit c of *out), over a grid-strided pass of the slots.
+static __global__ __launch_bounds__(BG_BLOCK) void k_coh_usage(DevNet n, uint32_t *out)
+{
+    __shared__ uint32_t s_bits;
+    if (threadIdx.x == 0) s_bits = 0;
+    __syncthreads();
+    uint32_t bits = 0;
+    for (uint64_t slot = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; slot < n.S;
+         slot += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t meta = n.meta[slot];
+        if (meta & BG_META_IN_USE) bits |= 1u << meta_coh(meta);
+    }
+    for (int off = 32; off >= 1; off >>= 1) bits |= (uint32_t)__shfl_xor((int)bits, off);
+    if ((threadIdx.x & (BG_WAVE - 1)) == 0 && bits) atomicOr(&s_bits, bits);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_bits) atomicOr(out, s_bits);
+}

@@ -53,6 +53,7 @@ struct CreateCtx {
     int32_t useFD;
     int syn_kind;
     uint64_t step;          // creation step (delay-ring phase)
+    int coh;                // cohort of the created synapses (implicit-ring networks)
 };
 
 __device__ __forceinline__ int syn_type_of(int ts, int td)
@@ -66,7 +67,8 @@ __device__ __forceinline__ int syn_sign_of(int type) { return type <= 1 ? -1 : 1
 __device__ void create_synapse(const DevNet &n, const CreateCtx &c, uint64_t slot, int src, int type, float w)
 {
     n.meta[slot] = (uint32_t)src | ((uint32_t)type << BG_META_TYPE_SHIFT) |
-                   ((uint32_t)c.total_delay[type] << BG_META_DELAY_SHIFT) | BG_META_IN_USE;
+                   ((uint32_t)c.total_delay[type] << BG_META_DELAY_SHIFT) |
+                   ((uint32_t)(c.coh & (BG_COH_MAX - 1)) << BG_META_COH_SHIFT) | BG_META_IN_USE;
     n.phase[slot] = (uint8_t)(c.step & (uint64_t)(n.R - 1));
     n.psr[slot] = 0.0f;
     n.W[slot] = w;

@@ -8,6 +8,9 @@ void launch_fused_lif(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_
                       uint64_t tprev0, int sel);
 void launch_fused_izh(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_t t0, int K, int pull,
                       uint64_t tprev0, int sel);
+// implicit-ring fused blocks (Spiking/DS, 32-bit rings): sel = parity of the block's fire words
+void launch_fused_impl_lif(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_t t0, int K, int sel);
+void launch_fused_impl_izh(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_t t0, int K, int sel);
 void launch_neuron_step(const DevNet &dn, hipStream_t s, int nk, uint64_t t);
 // steps per neuron-only launch
 #define BG_QUIET_STEPS 10000
@@ -47,6 +50,8 @@ struct BatchShape {
 };
 void launch_fused_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, int sk, int w32, int kmax,
                         int K, int Kfull, int launch);
+void launch_fused_impl_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, int sk, int kmax,
+                             int K, int Kfull, int launch);
 void launch_neuron_only_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, uint64_t step,
                               int steps);
 void launch_step_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, int sk, int w32,
