@@ -42,6 +42,14 @@ DS_SYN_STEP_BYTES = 24            # per-step kernel: psr r/w 8, decay 4, ring r/
 LIF_NEURON_STEP_BYTES = 44        # Vm, Vthresh, sp, I0, Inoise, C1, C2, refr read; Vm, sp, refr written
 DS_SYN_BLOCK_BYTES = 20           # blocked kernel, per synapse and block: meta 4, ring r/w 8, psr r/w 8
 DS_DELIVERY_BYTES = 36            # per delivered spike: W, r, u (12) + lastSpike (8) read; r, u, lastSpike written
+# implicit-ring blocked kernel: per synapse and block meta 4 and psr r/w 8 (no ring); per delivered
+# spike W 4 (the r, u, lastSpike of a delivery are shared by the synapses of one source, type and
+# cohort); per neuron and block its state (44) and fire word r/w (16), and per live cohort and target
+# class its DS state r/w (32) and prepared first-delivery factor (8)
+IMPL_SYN_BLOCK_BYTES = 12
+IMPL_DELIVERY_BYTES = 4
+IMPL_NEURON_BLOCK_BYTES = LIF_NEURON_STEP_BYTES + 16
+IMPL_COHORT_CLASS_BYTES = 40
 GRID = 100
 R0 = 8.9
 M = 1000
@@ -149,7 +157,7 @@ def forms_cpu_baseline(specs: list, sample: list, steps: int, threads: int) -> d
                       f"from the sample per family by neuron-seconds"}
 
 
-KERNEL_SOURCES = ("bghip_kernels.h", "bghip_device.h", "bghip_launch.h", "bghip_fused_lif.hip")
+KERNEL_SOURCES = ("bghip_kernels.h", "bghip_implicit.h", "bghip_device.h", "bghip_launch.h", "bghip_fused_lif.hip")
 
 
 def kernel_source_sha() -> str:
@@ -175,6 +183,61 @@ def load_traffic():
     if t.get("kernel_source_sha") != kernel_source_sha():
         return None
     return t
+
+
+def fused_phase(model, steps: int) -> dict:
+    """Times `steps` steps of back-to-back fused launches with HIP events on the network's stream
+    and counts the synapses that received a spike over them (spike counts before and after, times
+    the out-degrees; the phase must lie inside one epoch)."""
+    import numpy as np
+    outdeg = model.net.download_index_map()["outgoingSynapseCount"].astype(np.int64)
+    counts0 = model.net.download_spikes()["spikeCount"].astype(np.int64)
+    model.net.timing(True)
+    model.advance(steps)
+    avg_ms, launches = model.net.timing(False)
+    delivered = deliveries_between(model.net, counts0, outdeg)
+    return {"avg_ms": avg_ms, "launches": launches, "delivered": delivered, "synapses": model.net.total_synapses,
+            "implicit": model.net.implicit}
+
+
+def blocked_bytes(S: int, delivered_per_launch: float, N: int, implicit: bool, cohorts: int = 1) -> float:
+    """Minimum HBM bytes of one fused launch of the blocked kernel (implicit or per-synapse rings)."""
+    if implicit:
+        return (IMPL_SYN_BLOCK_BYTES * S + IMPL_DELIVERY_BYTES * delivered_per_launch
+                + (IMPL_NEURON_BLOCK_BYTES + 2 * cohorts * IMPL_COHORT_CLASS_BYTES) * N)
+    return DS_SYN_BLOCK_BYTES * S + DS_DELIVERY_BYTES * delivered_per_launch + LIF_NEURON_STEP_BYTES * N
+
+
+def explicit_fused_roofline(steps: int) -> dict:
+    """The same pre-grown network on the per-synapse ring kernels (a context created with
+    BGHIP_IMPLICIT=0): its fused launch time, priced with the explicit blocked byte model."""
+    from braingrid_amd import bghip as B
+    from braingrid_amd import host
+    old = os.environ.get("BGHIP_IMPLICIT")
+    os.environ["BGHIP_IMPLICIT"] = "0"
+    try:
+        ctx = B.Context(0)
+    finally:
+        if old is None:
+            del os.environ["BGHIP_IMPLICIT"]
+        else:
+            os.environ["BGHIP_IMPLICIT"] = old
+    m = host.Model(ctx, workload_cfg())
+    m.set_radii(R0)
+    m.update_connections()
+    m.advance(900)
+    m.net.sync()
+    ph = fused_phase(m, steps)
+    m.close()
+    ctx.close()
+    N = GRID * GRID
+    per = ph["delivered"] / max(1, ph["launches"])
+    b = blocked_bytes(ph["synapses"], per, N, False)
+    ach = b / (ph["avg_ms"] * 1e-3) if ph["avg_ms"] > 0 else 0.0
+    return {"kernel": "k_fused_block<LIF,DS,1,9> (per-synapse rings)", "implicit": ph["implicit"],
+            "avg_launch_ms": ph["avg_ms"], "launches": ph["launches"], "algorithmic_bytes_per_launch": b,
+            "achieved": ach / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
+            "steps_per_launch": steps / max(1, ph["launches"])}
 
 
 def deliveries_between(net, counts0, outdeg):
@@ -450,6 +513,10 @@ def main():
                     help="batched/served forms: oracle steps per sampled network (plus one epoch-end update)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="batched/served forms: host threads of the CPU baseline (one network each)")
+    ap.add_argument("--no-explicit", action="store_true",
+                    help="single form: skip timing the per-synapse-ring fused kernel on the same network")
+    ap.add_argument("--explicit-steps", type=int, default=9000,
+                    help="single form: steps of the per-synapse-ring fused kernel's timing")
     ap.add_argument("--no-per-step", action="store_true",
                     help="single form: skip the per-step kernels' roofline on the same network")
     ap.add_argument("--per-step-steps", type=int, default=2000,
@@ -507,15 +574,10 @@ def main():
     # roofline sub-phase: back-to-back fused launches only, timed with HIP events on the net stream;
     # the spike counts before and after give the synapses that received a spike in these blocks
     rl_steps = min(args.roofline_steps, steps_per_epoch - done % steps_per_epoch - 1)
-    import numpy as np
-    outdeg = model.net.download_index_map()["outgoingSynapseCount"].astype(np.int64)
-    counts0 = model.net.download_spikes()["spikeCount"].astype(np.int64)
-    model.net.timing(True)
-    model.advance(rl_steps)
-    avg_ms, launches = model.net.timing(False)
-    delivered = deliveries_between(model.net, counts0, outdeg)
+    ph = fused_phase(model, rl_steps)
+    avg_ms, launches, delivered, implicit = ph["avg_ms"], ph["launches"], ph["delivered"], ph["implicit"]
     done += rl_steps
-    S_rl = model.net.total_synapses
+    S_rl = ph["synapses"]
     K = max(1, round(rl_steps / max(1, launches)))
 
     # timed region: whole hot path including growth/history at epoch boundaries
@@ -545,12 +607,14 @@ def main():
 
     value = synapse_updates / wall
     per_launch_delivered = delivered / max(1, launches)
-    launch_bytes = (DS_SYN_BLOCK_BYTES * S_rl + DS_DELIVERY_BYTES * per_launch_delivered
-                    + LIF_NEURON_STEP_BYTES * N)
+    # the pre-growth update created every synapse: one cohort
+    launch_bytes = blocked_bytes(S_rl, per_launch_delivered, N, implicit, cohorts=1)
+    ref_bytes = blocked_bytes(S_rl, per_launch_delivered, N, False)
     achieved = launch_bytes / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
     traffic = load_traffic()
     # the instantiation the full blocks run: exactly 9 steps for a 32-bit ring, else up to 16 / 32
-    kernel_name = "k_fused_block<LIF,DS,1,%d>" % (9 if K == 9 else (16 if K <= 16 else 32))
+    kb = 9 if K == 9 else (16 if K <= 16 else 32)
+    kernel_name = ("k_fused_impl<LIF,DS,%d>" % kb) if implicit else ("k_fused_block<LIF,DS,1,%d>" % kb)
     out = {
         "metric": "synapse-updates/s at 10k-LIF growth; batched vs per-net sims/hr; p50/p99",
         "value": value,
@@ -580,10 +644,20 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic.get("bytes_per_launch") if traffic else None,
                      "kernel": kernel_name, "avg_launch_ms": avg_ms, "launches": launches,
                      "steps_per_launch": rl_steps / max(1, launches),
-                     "byte_model": f"per launch: {DS_SYN_BLOCK_BYTES} B x synapses (meta, ring r/w, psr r/w once "
-                                   f"per block) + {DS_DELIVERY_BYTES} B x synapses receiving a spike in the block "
-                                   f"(W, r, u, lastSpike read; r, u, lastSpike written) + "
-                                   f"{LIF_NEURON_STEP_BYTES} B x neurons",
+                     "byte_model": (f"per launch: {IMPL_SYN_BLOCK_BYTES} B x synapses (meta, psr r/w once per block; "
+                                    f"the delay rings are the sources' fire words) + {IMPL_DELIVERY_BYTES} B x synapses "
+                                    f"receiving a spike in the block (W) + {IMPL_NEURON_BLOCK_BYTES} B x neurons "
+                                    f"(state, fire word) + {2 * IMPL_COHORT_CLASS_BYTES} B x neurons per creation "
+                                    f"cohort (DS state per source and target class, 1 cohort here)") if implicit else
+                                   (f"per launch: {DS_SYN_BLOCK_BYTES} B x synapses (meta, ring r/w, psr r/w once "
+                                    f"per block) + {DS_DELIVERY_BYTES} B x synapses receiving a spike in the block "
+                                    f"(W, r, u, lastSpike read; r, u, lastSpike written) + "
+                                    f"{LIF_NEURON_STEP_BYTES} B x neurons"),
+                     "reference_blocked_bytes_per_launch": ref_bytes,
+                     "reference_blocked_rate_frac": (ref_bytes / (avg_ms * 1e-3) / HBM_PEAK) if avg_ms > 0 else None,
+                     "reference_blocked_note": "the bytes a per-synapse-ring blocked kernel moves for the same launch "
+                                               "(20 B/synapse + 36 B/delivery + 44 B/neuron) over this launch's time: "
+                                               "a rate of delivered work in those units, not a bandwidth",
                      "synapses": int(S_rl), "delivered_per_launch": per_launch_delivered,
                      "delivered_fraction": per_launch_delivered / max(1, S_rl),
                      "algorithmic_bytes_per_launch": launch_bytes,
@@ -595,6 +669,8 @@ def main():
         "device_error_bits": int(err),
     }
     model.close()
+    if implicit and not args.no_explicit:
+        out["explicit_fused"] = explicit_fused_roofline(args.explicit_steps)
     if not args.no_per_step:
         out["per_step"] = per_step_roofline(ctx, args.per_step_steps)
     ctx.close()

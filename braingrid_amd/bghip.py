@@ -141,6 +141,7 @@ def load():
         "bghip_update_connections": (ctypes.c_int, [vp, ctypes.POINTER(GrowthStats)]),
         "bghip_update_weights": (ctypes.c_int, [vp, _fp, ctypes.POINTER(GrowthStats)]),
         "bghip_update_weights_static": (ctypes.c_int, [vp, _fp, ctypes.POINTER(GrowthStats)]),
+        "bghip_net_implicit": (ctypes.c_int, [vp]),
         "bghip_download_spikes": (ctypes.c_int, [vp, _ip, _ip, _qp]),
         "bghip_clear_spike_counts": (ctypes.c_int, [vp]),
         "bghip_update_history": (ctypes.c_int, [vp]),
@@ -197,7 +198,7 @@ EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_
             "bghip_net_save", "bghip_net_load", "bghip_net_get_desc", "bghip_recorder_info",
             "bghip_add_synapses", "bghip_add_synapses_ex", "bghip_batch_autotune", "bghip_batch_group_count", "bghip_batch_get_shape",
             "bghip_batch_set_shape", "bghip_update_weights", "bghip_download_spikes", "bghip_clear_spike_counts",
-            "bghip_update_weights_static")
+            "bghip_update_weights_static", "bghip_net_implicit")
 
 
 def _ptr(a: np.ndarray | None, ct):
@@ -392,6 +393,11 @@ class Net:
     @property
     def step(self) -> int:
         return load().bghip_net_step(self.h)
+
+    @property
+    def implicit(self) -> bool:
+        """True when the network runs the implicit-ring kernels (bghip_net_implicit)."""
+        return bool(load().bghip_net_implicit(self.h))
 
     @property
     def total_synapses(self) -> int:

@@ -294,7 +294,7 @@ static void plan_net(bghip_net *n, Carve &c)
     c.add(&n->d_stats, 4); c.add(&n->d_total, 1);
     c.add(&n->d_coh_used, 1);
     if (impl_capable(d)) {
-        c.add(&dn.rec[0], N); c.add(&dn.rec[1], N); c.add(&dn.coh_step, BG_COH_MAX);
+        c.add(&dn.rec[0], N); c.add(&dn.rec[1], N); c.add(&dn.coh_step, BG_COH_MAX); c.add(&dn.Wd, S);
         if (ds) {
             const size_t C2N = (size_t)BG_COH_MAX * 2 * N;
             c.add(&dn.cst_r, C2N); c.add(&dn.cst_u, C2N); c.add(&dn.cst_last, C2N); c.add(&dn.cfac, 2 * C2N);
@@ -403,7 +403,8 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
     }
     HIP_TRY(ctx, hipStreamSynchronize(n->stream));
     set_kmax(n);
-    dn.impl = impl_capable(*desc) && ctx->implicit && n->kmax >= 2 ? 1 : 0;
+    // (32-bit byte offsets address the slot arrays in the implicit kernels)
+    dn.impl = impl_capable(*desc) && ctx->implicit && n->kmax >= 2 && dn.S < (1ull << 30) ? 1 : 0;
     *out = n;
     return BGHIP_OK;
 }
@@ -441,6 +442,7 @@ extern "C" void bghip_net_destroy(bghip_net *n)
 }
 
 extern "C" uint64_t bghip_net_step(const bghip_net *n) { return n ? n->step : 0; }
+extern "C" int bghip_net_implicit(const bghip_net *n) { return n ? n->dn.impl : 0; }
 extern "C" int bghip_net_get_desc(const bghip_net *n, bghip_net_desc *out)
 {
     if (!n || !out) return BGHIP_EINVAL;

@@ -108,6 +108,8 @@ struct DevNet {
     uint64_t *cst_last;          //     index (c * 2 + class) * N + source; class = type & 1
     float2 *cfac;                // DS: (u, r) after the first delivery of the next block, by block
                                  //     parity: [(parity * BG_COH_MAX + c) * 2 + class] * N + source
+    float *Wd;                   // S: W / decay of the synapse's type (the quotient changePSR forms at
+                                 //    every delivery), written wherever W is
 };
 
 enum { BG_INPUT_NONE = 0, BG_INPUT_POISSON = 1, BG_INPUT_REGULAR = 2 };

@@ -51,80 +51,87 @@ __device__ __forceinline__ void ds_deliver(const DevNet &n, float &r, float &u, 
     last = t;
 }
 
+// Loads and stores at a 32-bit byte offset from a wave-uniform base: the address is the scalar
+// base plus a zero-extended 32-bit lane offset, which global memory instructions take as is (the
+// implicit kernels run only for networks whose slot arrays stay below 4 GiB).
+template <typename T>
+__device__ __forceinline__ T ld_off(const T *base, uint32_t byteoff)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byteoff);
+}
+template <typename T>
+__device__ __forceinline__ void st_off(T *base, uint32_t byteoff, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byteoff) = v;
+}
+
 struct ImplRank {
-    uint64_t slot;
+    uint32_t slot;
     uint32_t meta;
     float psr;
 };
 
-__device__ __forceinline__ void impl_hot_load(const DevNet &n, bool dense, uint64_t row, const uint32_t *imap,
-                                              uint32_t q, uint32_t cnt, ImplRank &h)
+// Rank q of the row (clamped to its last rank, so every lane loads a valid slot).
+template <bool DENSE>
+__device__ __forceinline__ void impl_hot_load(const DevNet &n, uint32_t row, const uint32_t *imap, uint32_t q,
+                                              ImplRank &h)
 {
-    h.slot = 0;
-    h.meta = 0;
-    h.psr = 0.0f;
-    if (q < cnt) {
-        h.slot = dense ? row + q : (uint64_t)imap[q];
-        h.meta = n.meta[h.slot];
-        h.psr = n.psr[h.slot];
-    }
+    h.slot = DENSE ? row + q : imap[q];
+    h.meta = ld_off(n.meta, h.slot << 2);
+    h.psr = ld_off(n.psr, h.slot << 2);
 }
 
-// The state a delivering synapse needs: W and, for DS, the prepared (u, r) of its cohort.
+// What a delivering synapse needs, fetched for every rank ahead of its pass (at the usual delivery
+// rates every cache line of these arrays is needed anyway): W / decay and, for DS, the prepared
+// (u, r) of its cohort.
 struct ImplCold {
-    float W;
+    float Wd;
     float2 f;
-    bool valid;
 };
 
 template <int SK>
 __device__ __forceinline__ void impl_cold_load(const DevNet &n, const float2 *fac, const ImplRank &h, ImplCold &c)
 {
-    c.W = n.W[h.slot];
+    c.Wd = ld_off(n.Wd, h.slot << 2);
     if (SK == SK_DS) {
         const uint32_t src = h.meta & BG_META_SRC_MASK;
-        const int type = (h.meta >> BG_META_TYPE_SHIFT) & 3;
-        c.f = fac[((size_t)meta_coh(h.meta) * 2 + (size_t)(type & 1)) * (size_t)n.N + src];
+        const uint32_t pair = ((h.meta >> (BG_META_COH_SHIFT - 1)) & ((BG_COH_MAX - 1) << 1)) |
+                              ((h.meta >> BG_META_TYPE_SHIFT) & 1u);   // cohort * 2 + target class
+        c.f = ld_off(fac, (pair * (uint32_t)n.N + src) << 3);
+    } else {
+        c.f = make_float2(0.0f, 0.0f);
     }
-    c.valid = true;
 }
 
 // One synapse through the K steps of a block: its deliveries are bits (64 - delay) .. of its
 // source's fire word (bit k = delivery at t0 + k), masked to spikes after its creation in blocks
-// that may reach back before it; then the PSR with the canonical per-step accumulation.
+// that may reach back before it; then the PSR with the canonical per-step accumulation. tab holds
+// each type's decay and 64 - total delay, cstep each cohort's creation step. A lane past the end
+// of the row (valid false) adds exactly +0.0 to its partial sums, which leaves them bit-identical
+// (they start at +0.0 and so are never -0.0).
 template <int SK, int KMAX>
-__device__ __forceinline__ void impl_synapse(const DevNet &n, const TypeRegs &tr, const float2 *fac,
-                                             const ImplRank &h, float &psr, uint64_t recent, ImplCold &c,
-                                             uint64_t t0, int K, bool young, float (&acc)[KMAX], uint32_t &hit)
+__device__ __forceinline__ void impl_synapse(const DevNet &n, const TypeRegs &tr, const uint2 *tab,
+                                             const uint64_t *cstep, const ImplRank &h, float &psr, uint64_t recent,
+                                             const ImplCold &c, bool valid, uint64_t t0, int K, bool young,
+                                             float (&acc)[KMAX], uint32_t &hit)
 {
     const int Kb = BG_K_EXACT(KMAX) ? KMAX : K;
     const uint32_t kmask = Kb >= 32 ? 0xFFFFFFFFu : ((1u << Kb) - 1u);
-    const uint32_t delay = (h.meta >> BG_META_DELAY_SHIFT) & 0xFF;
     const int type = (h.meta >> BG_META_TYPE_SHIFT) & 3;
-    uint32_t hm = (uint32_t)(recent >> (64u - delay)) & kmask;
-    if (young && hm) hm &= since_mask(n.coh_step[meta_coh(h.meta)], delay, t0);
+    const uint2 td = tab[type];
+    const float decay = __uint_as_float(td.x);
+    uint32_t hm = valid ? (uint32_t)(recent >> td.y) & kmask : 0u;
+    if (young && hm) hm &= since_mask(cstep[meta_coh(h.meta)], 64u - td.y, t0);
     hit = hm;
-    const float decay = pick4(tr.dec, type);
     int k1 = -1;
     float c1 = 0.0f;
-    ColdDS cd;
-    cd.valid = true;
     if (hm) {
-        if (!c.valid) impl_cold_load<SK>(n, fac, h, c);
         k1 = __ffs(hm) - 1;
-        if (SK == SK_DS) {
-            c1 = ((c.W / decay) * c.f.x) * c.f.y;
-            cd.W = c.W;
-            cd.u = c.f.x;
-            cd.r = c.f.y;
-            cd.last = t0 + (uint64_t)k1;
-        } else {
-            c1 = c.W / decay;
-            cd.W = c.W;
-        }
+        // changePSR's ((W / decay) * u) * r, or W / decay for Spiking synapses
+        c1 = SK == SK_DS ? (c.Wd * c.f.x) * c.f.y : c.Wd;
         hm &= hm - 1;
     }
-    float p = psr;
+    float p = valid ? psr : 0.0f;
     if (__ballot(hm != 0) == 0ull) {
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
@@ -135,7 +142,16 @@ __device__ __forceinline__ void impl_synapse(const DevNet &n, const TypeRegs &tr
             }
         }
     } else {
+        // a second delivery within the block: the later ones from a private copy of the state
+        ColdDS cd;
+        cd.valid = true;
         bool ru_changed = false;
+        if (hm) {
+            cd.W = ld_off(n.W, h.slot << 2);
+            cd.u = c.f.x;
+            cd.r = c.f.y;
+            cd.last = t0 + (uint64_t)k1;
+        }
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
             if (BG_K_EXACT(KMAX) || k < K) {
@@ -150,6 +166,49 @@ __device__ __forceinline__ void impl_synapse(const DevNet &n, const TypeRegs &tr
     psr = p;
 }
 
+// The row of one destination in passes of 64 ranks (lane l takes ranks l, l+64, ...), in a three
+// stage software pipeline: while pass p is processed, pass p+1 has its sources' fire words and its
+// cold state in flight and pass p+2 its meta and psr. Every load is unconditional, so the waits
+// before a pass cover only the loads it needs.
+template <int SK, int KMAX, bool DENSE>
+__device__ __forceinline__ void impl_row(const DevNet &n, const TypeRegs &tr, const uint2 *tab, const uint64_t *cstep,
+                                         const uint64_t *rec, const float2 *fac, uint32_t row, const uint32_t *imap,
+                                         uint32_t cnt, uint64_t t0, int K, bool young, float (&acc)[KMAX])
+{
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    const uint32_t P = (cnt + BG_WAVE - 1) / BG_WAVE;
+    const uint32_t last = cnt - 1;
+    auto rank = [&](uint32_t pass) {
+        const uint32_t q = pass * BG_WAVE + (uint32_t)lane;
+        return q < cnt ? q : last;
+    };
+    ImplRank x, y, z;
+    ImplCold cx, cy, cz;
+    uint64_t fx, fy, fz;
+    impl_hot_load<DENSE>(n, row, imap, rank(0), x);
+    impl_hot_load<DENSE>(n, row, imap, rank(1), y);
+    fx = ld_off(rec, (x.meta & BG_META_SRC_MASK) << 3);
+    impl_cold_load<SK>(n, fac, x, cx);
+    auto pass = [&](uint32_t p, ImplRank &cur, ImplCold &ccur, uint64_t &fcur, ImplRank &nxt, ImplCold &cnxt,
+                    uint64_t &fnxt, ImplRank &far) {
+        impl_hot_load<DENSE>(n, row, imap, rank(p + 2), far);
+        fnxt = ld_off(rec, (nxt.meta & BG_META_SRC_MASK) << 3);
+        impl_cold_load<SK>(n, fac, nxt, cnxt);
+        const bool valid = p * BG_WAVE + (uint32_t)lane < cnt;
+        uint32_t hit = 0;
+        float ps = cur.psr;
+        impl_synapse<SK, KMAX>(n, tr, tab, cstep, cur, ps, fcur, ccur, valid, t0, K, young, acc, hit);
+        if (valid) st_off(n.psr, cur.slot << 2, ps);
+    };
+    for (uint32_t p = 0; p < P; p += 3) {
+        pass(p, x, cx, fx, y, cy, fy, z);
+        if (p + 1 >= P) break;
+        pass(p + 1, y, cy, fy, z, cz, fz, x);
+        if (p + 2 >= P) break;
+        pass(p + 2, z, cz, fz, x, cx, fx, y);
+    }
+}
+
 template <int NK, int SK, int KMAX>
 __device__ __forceinline__ void fused_impl_body(const DevNet &n, int d, uint64_t t0, int K, int sel)
 {
@@ -157,11 +216,15 @@ __device__ __forceinline__ void fused_impl_body(const DevNet &n, int d, uint64_t
     __shared__ float s_in[BG_BLOCK / BG_WAVE][KMAX];
     const int lane = threadIdx.x & (BG_WAVE - 1);
     const int wv = threadIdx.x / BG_WAVE;
+    __shared__ uint2 s_tab[BG_BLOCK / BG_WAVE][4];
     const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
     const bool dense = (cnt == hi);
-    const uint64_t row = (uint64_t)d * n.M;
+    const uint32_t row = (uint32_t)d * (uint32_t)n.M;
     const uint32_t *imap = n.in_map + (dense ? 0 : n.in_begin[d]);
     const uint64_t *rec = n.rec[sel];
+    __shared__ uint64_t s_coh[BG_BLOCK / BG_WAVE][BG_COH_MAX];
+    if (lane < 4) s_tab[wv][lane] = make_uint2(__float_as_uint(n.decay[lane]), 64u - (uint32_t)n.tdelay[lane]);
+    if (lane < BG_COH_MAX) s_coh[wv][lane] = n.coh_step[lane];
     const float2 *fac = n.cfac + (size_t)sel * BG_COH_MAX * 2 * (size_t)n.N;
     const bool young = t0 < n.young_until;
     if (lane < K) s_noise[wv][lane] = bg_noise_normal(n.seed, (uint32_t)d, t0 + (uint64_t)lane);
@@ -169,43 +232,14 @@ __device__ __forceinline__ void fused_impl_body(const DevNet &n, int d, uint64_t
 #pragma unroll
     for (int k = 0; k < KMAX; k++) acc[k] = 0.0f;
 
-    // Lane-strided ranks in a three-stage pipeline, as in fused_block_body: while rank q is
-    // processed, rank q+64 has its source's fire word (and, while deliveries are frequent, its W
-    // and prepared DS factors) in flight and rank q+128 its meta and psr.
     TypeRegs tr;
     type_regs(n, tr);
-    ImplRank x, y, z;
-    ImplCold cx, cy, cz;
-    uint64_t fx = 0, fy = 0, fz = 0;
-    bool spec = false;
-    uint32_t q = lane;
-    impl_hot_load(n, dense, row, imap, q, cnt, x);
-    impl_hot_load(n, dense, row, imap, q + BG_WAVE, cnt, y);
-    cx.valid = false;
-    if (q < cnt) fx = rec[x.meta & BG_META_SRC_MASK];
-    auto pass = [&](ImplRank &cur, ImplCold &ccur, uint64_t &fcur, ImplRank &nxt, ImplCold &cnxt, uint64_t &fnxt,
-                    ImplRank &far) {
-        impl_hot_load(n, dense, row, imap, q + 2 * BG_WAVE, cnt, far);
-        cnxt.valid = false;
-        fnxt = 0;
-        if (q + BG_WAVE < cnt) {
-            fnxt = rec[nxt.meta & BG_META_SRC_MASK];
-            if (spec) impl_cold_load<SK>(n, fac, nxt, cnxt);
-        }
-        uint32_t hit = 0;
-        float p = cur.psr;
-        impl_synapse<SK, KMAX>(n, tr, fac, cur, p, fcur, ccur, t0, K, young, acc, hit);
-        n.psr[cur.slot] = p;
-        spec = __popcll(__ballot(hit != 0)) >= 4;
-        q += BG_WAVE;
-    };
-    while (true) {
-        if (q >= cnt) break;
-        pass(x, cx, fx, y, cy, fy, z);
-        if (q >= cnt) break;
-        pass(y, cy, fy, z, cz, fz, x);
-        if (q >= cnt) break;
-        pass(z, cz, fz, x, cx, fx, y);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (cnt) {
+        if (dense) impl_row<SK, KMAX, true>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc);
+        else impl_row<SK, KMAX, false>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc);
     }
 #pragma unroll
     for (int k = 0; k < KMAX; k++)

@@ -72,6 +72,7 @@ __device__ void create_synapse(const DevNet &n, const CreateCtx &c, uint64_t slo
     n.phase[slot] = (uint8_t)(c.step & (uint64_t)(n.R - 1));
     n.psr[slot] = 0.0f;
     n.W[slot] = w;
+    if (n.Wd) n.Wd[slot] = w / n.decay[type];
     for (int q = 0; q < n.W32; q++) n.ring[(uint64_t)q * n.S + slot] = 0u;
     if (c.syn_kind == SK_DS || c.syn_kind == SK_DYNSTDP) {
         n.r[slot] = c.r0;
@@ -360,8 +361,13 @@ __global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c,
             atomicOr(&conn[s >> 5], 1u << (s & 31));
             float area = pair_weight(n, Wd, s, d, n.xloc[s], n.yloc[s], n.radii[s], xd, yd, rd);
             int type = syn_type_of(n.ntype[s], td);
-            if (area < 0) atomicAdd(&s_erase, 1u);
-            else n.W[base + k] = area * (float)syn_sign_of(type) * c.W_adj;
+            if (area < 0) {
+                atomicAdd(&s_erase, 1u);
+            } else {
+                const float w = area * (float)syn_sign_of(type) * c.W_adj;
+                n.W[base + k] = w;
+                if (n.Wd) n.Wd[base + k] = w / n.decay[type];
+            }
         }
         __syncthreads();
         if (s_erase || cnt0 != hi0) {

@@ -260,6 +260,13 @@ int bghip_recorder_download(bghip_net *net, float *burstinessHist, float *spikes
 int bghip_sync(bghip_net *net);
 uint32_t bghip_net_error_bits(bghip_net *net);
 uint64_t bghip_net_step(const bghip_net *net);
+/* 1 when the network's Spiking/DS synapses advance with implicit delay rings (their rings read from
+ * the sources' recent fire bits, DS state shared per source, type and creation cohort), 0 when they
+ * use the per-synapse ring kernels. No counterpart in the reference (it reports which of this
+ * library's kernel families GPUSpikingModel::advance, Core/GPUSpikingModel.cu:188-225, maps to);
+ * results are identical either way. BGHIP_IMPLICIT=0 in the environment at bghip_ctx_create keeps
+ * a context's networks on the per-synapse ring kernels. */
+int bghip_net_implicit(const bghip_net *net);
 /* The description the network was created with (or loaded from a checkpoint). */
 int bghip_net_get_desc(const bghip_net *net, bghip_net_desc *out);
 /* The on-device recorder's sizes as enabled, and `epoch`: the epochs it has stored (history rows
