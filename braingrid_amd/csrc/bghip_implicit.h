@@ -25,6 +25,11 @@
 #pragma once
 #include "bghip_kernels.h"
 
+// Occupancy target of the implicit-ring fused kernels (waves per SIMD): bounds their registers.
+#ifndef BG_IMPL_WAVES
+#define BG_IMPL_WAVES 6
+#endif
+
 __device__ __forceinline__ int meta_coh(uint32_t meta) { return (int)((meta >> BG_META_COH_SHIFT) & (BG_COH_MAX - 1)); }
 
 // Bits k of a block starting at t0 whose delivery (at t0 + k, delay steps after the spike) comes
@@ -187,6 +192,9 @@ __device__ __forceinline__ void impl_row(const DevNet &n, const TypeRegs &tr, co
     uint64_t fx, fy, fz;
     impl_hot_load<DENSE>(n, row, imap, rank(0), x);
     impl_hot_load<DENSE>(n, row, imap, rank(1), y);
+    // keep y's loads older than x's gathers, as in every pass, so that the loop header's wait for
+    // the first pass covers the same loads whether it is entered here or from the loop's end
+    __builtin_amdgcn_sched_barrier(0);
     fx = ld_off(rec, (x.meta & BG_META_SRC_MASK) << 3);
     impl_cold_load<SK>(n, fac, x, cx);
     auto pass = [&](uint32_t p, ImplRank &cur, ImplCold &ccur, uint64_t &fcur, ImplRank &nxt, ImplCold &cnxt,
@@ -331,7 +339,7 @@ __device__ __forceinline__ void fused_impl_body(const DevNet &n, int d, uint64_t
 }
 
 template <int NK, int SK, int KMAX>
-__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_FUSED_WAVES))) void k_fused_impl(
+__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_IMPL_WAVES))) void k_fused_impl(
     DevNet n, uint64_t t0, int K, int sel)
 {
     const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
@@ -340,7 +348,7 @@ __global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_FUS
 }
 
 template <int NK, int SK, int KMAX>
-__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_FUSED_WAVES))) void k_fused_impl_batch(
+__global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_IMPL_WAVES))) void k_fused_impl_batch(
     BatchMap bm, int K, int Kfull, int launch)
 {
     const uint32_t w = __builtin_amdgcn_readfirstlane(
