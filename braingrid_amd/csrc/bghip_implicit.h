@@ -102,7 +102,7 @@ __device__ __forceinline__ void impl_cold_load(const DevNet &n, const float2 *fa
         const uint32_t src = h.meta & BG_META_SRC_MASK;
         const uint32_t pair = ((h.meta >> (BG_META_COH_SHIFT - 1)) & ((BG_COH_MAX - 1) << 1)) |
                               ((h.meta >> BG_META_TYPE_SHIFT) & 1u);   // cohort * 2 + target class
-        c.f = ld_off(fac, (pair * (uint32_t)n.N + src) << 3);
+        c.f = ld_off(fac, (__umul24(pair, (uint32_t)n.N) + src) << 3);
     } else {
         c.f = make_float2(0.0f, 0.0f);
     }
@@ -114,12 +114,13 @@ __device__ __forceinline__ void impl_cold_load(const DevNet &n, const float2 *fa
 // each type's decay and 64 - total delay, cstep each cohort's creation step. A lane past the end
 // of the row (valid false) adds exactly +0.0 to its partial sums, which leaves them bit-identical
 // (they start at +0.0 and so are never -0.0).
-template <int SK, int KMAX>
+template <int SK, int KMAX, bool TAIL>
 __device__ __forceinline__ void impl_synapse(const DevNet &n, const TypeRegs &tr, const uint2 *tab,
                                              const uint64_t *cstep, const ImplRank &h, float &psr, uint64_t recent,
                                              const ImplCold &c, bool valid, uint64_t t0, int K, bool young,
                                              float (&acc)[KMAX], uint32_t &hit)
 {
+    if (!TAIL) valid = true;
     const int Kb = BG_K_EXACT(KMAX) ? KMAX : K;
     const uint32_t kmask = Kb >= 32 ? 0xFFFFFFFFu : ((1u << Kb) - 1u);
     const int type = (h.meta >> BG_META_TYPE_SHIFT) & 3;
@@ -197,16 +198,24 @@ __device__ __forceinline__ void impl_row(const DevNet &n, const TypeRegs &tr, co
     __builtin_amdgcn_sched_barrier(0);
     fx = ld_off(rec, (x.meta & BG_META_SRC_MASK) << 3);
     impl_cold_load<SK>(n, fac, x, cx);
+    // full passes (every lane has a rank of the row) and, when the row length is not a multiple
+    // of 64, a last partial pass in which the lanes past its end add +0.0
+    const uint32_t PF = cnt / BG_WAVE;
     auto pass = [&](uint32_t p, ImplRank &cur, ImplCold &ccur, uint64_t &fcur, ImplRank &nxt, ImplCold &cnxt,
                     uint64_t &fnxt, ImplRank &far) {
         impl_hot_load<DENSE>(n, row, imap, rank(p + 2), far);
         fnxt = ld_off(rec, (nxt.meta & BG_META_SRC_MASK) << 3);
         impl_cold_load<SK>(n, fac, nxt, cnxt);
-        const bool valid = p * BG_WAVE + (uint32_t)lane < cnt;
         uint32_t hit = 0;
         float ps = cur.psr;
-        impl_synapse<SK, KMAX>(n, tr, tab, cstep, cur, ps, fcur, ccur, valid, t0, K, young, acc, hit);
-        if (valid) st_off(n.psr, cur.slot << 2, ps);
+        if (p < PF) {
+            impl_synapse<SK, KMAX, false>(n, tr, tab, cstep, cur, ps, fcur, ccur, true, t0, K, young, acc, hit);
+            st_off(n.psr, cur.slot << 2, ps);
+        } else {
+            const bool valid = p * BG_WAVE + (uint32_t)lane < cnt;
+            impl_synapse<SK, KMAX, true>(n, tr, tab, cstep, cur, ps, fcur, ccur, valid, t0, K, young, acc, hit);
+            if (valid) st_off(n.psr, cur.slot << 2, ps);
+        }
     };
     for (uint32_t p = 0; p < P; p += 3) {
         pass(p, x, cx, fx, y, cy, fy, z);
@@ -235,7 +244,9 @@ __device__ __forceinline__ void fused_impl_body(const DevNet &n, int d, uint64_t
     if (lane < BG_COH_MAX) s_coh[wv][lane] = n.coh_step[lane];
     const float2 *fac = n.cfac + (size_t)sel * BG_COH_MAX * 2 * (size_t)n.N;
     const bool young = t0 < n.young_until;
-    if (lane < K) s_noise[wv][lane] = bg_noise_normal(n.seed, (uint32_t)d, t0 + (uint64_t)lane);
+    // noise of the block's steps, unless the neuron stays refractory through all of them (then
+    // neuron_advance never reads it)
+    if (n.refr[d] < K && lane < K) s_noise[wv][lane] = bg_noise_normal(n.seed, (uint32_t)d, t0 + (uint64_t)lane);
     float acc[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; k++) acc[k] = 0.0f;
