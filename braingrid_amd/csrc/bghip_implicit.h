@@ -56,20 +56,6 @@ __device__ __forceinline__ void ds_deliver(const DevNet &n, float &r, float &u, 
     last = t;
 }
 
-// Loads and stores at a 32-bit byte offset from a wave-uniform base: the address is the scalar
-// base plus a zero-extended 32-bit lane offset, which global memory instructions take as is (the
-// implicit kernels run only for networks whose slot arrays stay below 4 GiB).
-template <typename T>
-__device__ __forceinline__ T ld_off(const T *base, uint32_t byteoff)
-{
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byteoff);
-}
-template <typename T>
-__device__ __forceinline__ void st_off(T *base, uint32_t byteoff, T v)
-{
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byteoff) = v;
-}
-
 struct ImplRank {
     uint32_t slot;
     uint32_t meta;

@@ -7,6 +7,21 @@
 #define BG_BLOCK 256
 #define BG_KMAX 32
 
+// Loads and stores at a 32-bit byte offset from a wave-uniform base: the address is the scalar
+// base plus a zero-extended 32-bit lane offset, which global memory instructions take as is (the
+// kernels that use them run only for networks whose slot arrays stay below 4 GiB).
+template <typename T>
+__device__ __forceinline__ T ld_off(const T *base, uint32_t byteoff)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byteoff);
+}
+template <typename T>
+__device__ __forceinline__ void st_off(T *base, uint32_t byteoff, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byteoff) = v;
+}
+
+
 // ---------------------------------------------------------------------------------------------
 // Per-step path (used for STDP synapses, which need every neuron's step-t spike before any
 // synapse of step t): one thread per neuron. AllSpikingNeurons::advanceNeurons
@@ -82,14 +97,6 @@ __device__ __forceinline__ void synapse_step_body(const DevNet &n, int d, int la
     if (lane == 0) n.sp[d] = n.sp[d] + acc;
 }
 
-template <int SK, int W32>
-__global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
-{
-    const int lane = threadIdx.x & (BG_WAVE - 1);
-    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
-    if (d >= n.N) return;
-    synapse_step_body<SK, W32>(n, d, lane, t);
-}
 
 // ---------------------------------------------------------------------------------------------
 // Networks without synapses (growth networks before their first connection, for instance): one
@@ -558,6 +565,153 @@ __device__ __forceinline__ float psr_increment_regs(const DevNet &n, const TypeR
     }
 }
 
+// synapse_step_body for Spiking/DS synapses with 32-bit rings, as a software pipeline over the
+// row's passes of 64 ranks: while pass p is processed, pass p+1 has its sources' last firing steps
+// and (for its delivering lanes) its cold state in flight, and pass p+2 its meta, ring word and psr.
+// Every load instruction is unconditional (ranks past the row's end are clamped to its last rank;
+// lanes without a delivery read the cold state of the row's first slot, one shared cache line), so
+// the waits before a pass cover only its own operands. Same per-slot operations, same order.
+struct StepRank {
+    uint32_t slot, meta, ring;
+    float psr;
+};
+struct StepCold {
+    float W, r, u;
+    uint64_t last;
+};
+
+template <int SK, bool DENSE>
+__device__ __forceinline__ void step_hot_load(const DevNet &n, uint32_t row, const uint32_t *imap, uint32_t q,
+                                              StepRank &h)
+{
+    h.slot = DENSE ? row + q : imap[q];
+    const uint32_t off = h.slot << 2;
+    h.meta = ld_off(n.meta, off);
+    h.ring = ld_off(n.ring, off);
+    h.psr = ld_off(n.psr, off);
+}
+
+template <int SK>
+__device__ __forceinline__ void step_cold_load(const DevNet &n, uint32_t row, const StepRank &h, uint32_t tb,
+                                               StepCold &c)
+{
+    const uint32_t off = (((h.ring >> tb) & 1u) ? h.slot : row) << 2;
+    c.W = ld_off(n.W, off);
+    if (SK == SK_DS) {
+        c.r = ld_off(n.r, off);
+        c.u = ld_off(n.us, off);
+        c.last = ld_off(n.last, off << 1);
+    }
+}
+
+template <int SK, bool DENSE>
+__device__ __forceinline__ void synapse_step_row(const DevNet &n, const TypeRegs &tr, int d, uint32_t cnt, uint64_t t,
+                                                 float &acc)
+{
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    const uint32_t row = (uint32_t)d * (uint32_t)n.M;
+    const uint32_t *imap = n.in_map + (DENSE ? 0 : n.in_begin[d]);
+    const uint32_t tb = (uint32_t)(t & 31u);
+    const uint32_t P = (cnt + BG_WAVE - 1) / BG_WAVE, last = cnt - 1;
+    auto rank = [&](uint32_t pass) {
+        const uint32_t q = pass * BG_WAVE + (uint32_t)lane;
+        return q < cnt ? q : last;
+    };
+    StepRank x, y, z;
+    StepCold cx, cy, cz;
+    uint64_t fx, fy, fz;
+    step_hot_load<SK, DENSE>(n, row, imap, rank(0), x);
+    step_hot_load<SK, DENSE>(n, row, imap, rank(1), y);
+    __builtin_amdgcn_sched_barrier(0);
+    fx = ld_off(n.fired_step, (x.meta & BG_META_SRC_MASK) << 3);
+    step_cold_load<SK>(n, row, x, tb, cx);
+    auto pass = [&](uint32_t p, StepRank &cur, StepCold &ccur, uint64_t &fcur, StepRank &nxt, StepCold &cnxt,
+                    uint64_t &fnxt, StepRank &far) {
+        step_hot_load<SK, DENSE>(n, row, imap, rank(p + 2), far);
+        fnxt = ld_off(n.fired_step, (nxt.meta & BG_META_SRC_MASK) << 3);
+        step_cold_load<SK>(n, row, nxt, tb, cnxt);
+        const bool valid = p * BG_WAVE + (uint32_t)lane < cnt;
+        const int type = (cur.meta >> BG_META_TYPE_SHIFT) & 3;
+        const uint32_t delay = (cur.meta >> BG_META_DELAY_SHIFT) & 0xFF;
+        uint32_t w = cur.ring;
+        bool changed = false;
+        if (fcur == t) {   // preSpikeHit at the firing step
+            const uint32_t m = 1u << ((tb + delay) & 31u);
+            if (valid && (w & m)) dev_error(n, 1u);
+            w |= m;
+            changed = true;
+        }
+        const bool pre = (w >> tb) & 1u;
+        if (pre) {
+            w &= ~(1u << tb);
+            changed = true;
+        }
+        const float decay = pick4(tr.dec, type);
+        float psr = cur.psr;
+        if (pre) {
+            ColdDS c;
+            c.W = ccur.W;
+            c.r = ccur.r;
+            c.u = ccur.u;
+            c.last = ccur.last;
+            c.valid = true;
+            bool ru_changed = false;
+            psr = psr + psr_increment_regs<SK>(n, tr, cur.slot, type, t, decay, c, ru_changed);
+            if (valid && SK == SK_DS) {
+                const uint32_t off = cur.slot << 2;
+                if (ru_changed) {
+                    st_off(n.r, off, c.r);
+                    st_off(n.us, off, c.u);
+                }
+                st_off(n.last, off << 1, c.last);
+            }
+        }
+        psr = psr * decay;
+        if (valid) {
+            st_off(n.psr, cur.slot << 2, psr);
+            if (changed) st_off(n.ring, cur.slot << 2, w);
+            acc += psr;
+        }
+    };
+    for (uint32_t p = 0; p < P; p += 3) {
+        pass(p, x, cx, fx, y, cy, fy, z);
+        if (p + 1 >= P) break;
+        pass(p + 1, y, cy, fy, z, cz, fz, x);
+        if (p + 2 >= P) break;
+        pass(p + 2, z, cz, fz, x, cx, fx, y);
+    }
+}
+
+template <int SK, int W32>
+__device__ __forceinline__ void synapse_step_any(const DevNet &n, int d, int lane, uint64_t t)
+{
+    if (W32 == 1 && (SK == SK_SPIKING || SK == SK_DS)) {
+        if (n.S < (1ull << 29)) {   // 32-bit byte offsets (8-byte lastSpike included)
+            TypeRegs tr;
+            type_regs(n, tr);
+            const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
+            float acc = 0.0f;
+            if (cnt) {
+                if (cnt == hi) synapse_step_row<SK, true>(n, tr, d, cnt, t, acc);
+                else synapse_step_row<SK, false>(n, tr, d, cnt, t, acc);
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) n.sp[d] = n.sp[d] + acc;
+            return;
+        }
+    }
+    synapse_step_body<SK, W32>(n, d, lane, t);
+}
+
+template <int SK, int W32>
+__global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
+{
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    const int d = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
+    if (d >= n.N) return;
+    synapse_step_any<SK, W32>(n, d, lane, t);
+}
+
 // A synapse through the K steps of a block: pull the spikes its source fired in the previous block
 // (fire mask fm) into the ring, take the block's deliveries out of it, and run the PSR with the
 // canonical per-step accumulation. The first delivery's increment is evaluated before the step
@@ -889,7 +1043,7 @@ __global__ __launch_bounds__(BG_BLOCK) void k_synapse_batch(BatchMap bm, uint64_
     if (w >= bm.total) return;
     const int i = __builtin_amdgcn_readfirstlane((int)bm.nof[w]);
     const BatchNet &b = bm.nets[i];
-    synapse_step_body<SK, W32>(b.n, (int)(w - bm.base[i]), threadIdx.x & (BG_WAVE - 1), b.t0 + step);
+    synapse_step_any<SK, W32>(b.n, (int)(w - bm.base[i]), threadIdx.x & (BG_WAVE - 1), b.t0 + step);
 }
 
 // The per-step STDP passes over a batch; event lists hold batch work-item indices.
