@@ -295,6 +295,7 @@ static void plan_net(bghip_net *n, Carve &c)
     c.add(&n->d_coh_used, 1);
     if (impl_capable(d)) {
         c.add(&dn.rec[0], N); c.add(&dn.rec[1], N); c.add(&dn.coh_step, BG_COH_MAX); c.add(&dn.Wd, S);
+        c.add(&dn.lpt, N);
         if (ds) {
             const size_t C2N = (size_t)BG_COH_MAX * 2 * N;
             c.add(&dn.cst_r, C2N); c.add(&dn.cst_u, C2N); c.add(&dn.cst_last, C2N); c.add(&dn.cfac, 2 * C2N);
@@ -531,6 +532,7 @@ static int rebuild_incoming(bghip_net *n)
 {
     DevNet &dn = n->dn;
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, n->stream, dn.syn_count, dn.in_begin, dn.N, n->d_total);
+    if (dn.lpt) hipLaunchKernelGGL(k_lpt_order, dim3(1), dim3(1024), 0, n->stream, dn.syn_count, dn.lpt, dn.N);
     unsigned blocks = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
     hipLaunchKernelGGL(k_fill_incoming, dim3(blocks), dim3(BG_BLOCK), 0, n->stream, dn);
     HIP_TRY(n->ctx, hipGetLastError());

@@ -101,6 +101,8 @@ struct DevNet {
     uint32_t coh_mask;           // cohort ids in use
     uint64_t young_until;        // blocks starting earlier may see spikes from before a cohort's creation
     int32_t tdelay[4];           // total delay of each synapse type (every synapse has its type's)
+    uint32_t *lpt;               // N: destinations by row length, longest first (implicit-ring kernels'
+                                 // workgroup order; any order gives the same results)
     uint64_t *rec[2];            // N: fire bits of the 64 steps before a block (bit j = step t0 - 64 + j),
                                  // by block parity like mask[]
     uint64_t *coh_step;          // BG_COH_MAX: creation step of each cohort

@@ -1,5 +1,5 @@
 // Fused block kernels for LIF neurons (see k_fused_block in bghip_kernels.h).
-#include "bghip_implicit.h"
+#include "bghip_kernels.h"
 #include "bghip_launch.h"
 
 template <int SK, int W32>
@@ -33,23 +33,4 @@ void launch_fused_lif(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_
 {
     if (sk == SK_DS) go_w<SK_DS>(dn, s, kmax, t0, K, pull, tprev0, sel);
     else go_w<SK_SPIKING>(dn, s, kmax, t0, K, pull, tprev0, sel);
-}
-
-// implicit-ring blocks (k_fused_impl in bghip_implicit.h): 32-bit rings only
-template <int SK>
-static void go_impl(const DevNet &dn, hipStream_t s, int kmax, uint64_t t0, int K, int sel)
-{
-    unsigned blocks = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
-    if (K == BG_K_COMMON)
-        hipLaunchKernelGGL((k_fused_impl<NK_LIF, SK, BG_K_COMMON>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t0, K, sel);
-    else if (kmax <= 16)
-        hipLaunchKernelGGL((k_fused_impl<NK_LIF, SK, 16>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t0, K, sel);
-    else
-        hipLaunchKernelGGL((k_fused_impl<NK_LIF, SK, 32>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t0, K, sel);
-}
-
-void launch_fused_impl_lif(const DevNet &dn, hipStream_t s, int sk, int kmax, uint64_t t0, int K, int sel)
-{
-    if (sk == SK_DS) go_impl<SK_DS>(dn, s, kmax, t0, K, sel);
-    else go_impl<SK_SPIKING>(dn, s, kmax, t0, K, sel);
 }

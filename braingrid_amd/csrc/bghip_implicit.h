@@ -162,10 +162,11 @@ __device__ __forceinline__ void impl_synapse(const DevNet &n, const TypeRegs &tr
 // stage software pipeline: while pass p is processed, pass p+1 has its sources' fire words and its
 // cold state in flight and pass p+2 its meta and psr. Every load is unconditional, so the waits
 // before a pass cover only the loads it needs.
-template <int SK, int KMAX, bool DENSE>
+template <int SK, int KMAX, bool DENSE, class Prologue>
 __device__ __forceinline__ void impl_row(const DevNet &n, const TypeRegs &tr, const uint2 *tab, const uint64_t *cstep,
                                          const uint64_t *rec, const float2 *fac, uint32_t row, const uint32_t *imap,
-                                         uint32_t cnt, uint64_t t0, int K, bool young, float (&acc)[KMAX])
+                                         uint32_t cnt, uint64_t t0, int K, bool young, float (&acc)[KMAX],
+                                         Prologue &&prologue)
 {
     const int lane = threadIdx.x & (BG_WAVE - 1);
     const uint32_t P = (cnt + BG_WAVE - 1) / BG_WAVE;
@@ -184,6 +185,8 @@ __device__ __forceinline__ void impl_row(const DevNet &n, const TypeRegs &tr, co
     __builtin_amdgcn_sched_barrier(0);
     fx = ld_off(rec, (x.meta & BG_META_SRC_MASK) << 3);
     impl_cold_load<SK>(n, fac, x, cx);
+    // work of the caller's that needs none of the row, done while the first passes' loads fly
+    prologue();
     // full passes (every lane has a rank of the row) and, when the row length is not a multiple
     // of 64, a last partial pass in which the lanes past its end add +0.0
     const uint32_t PF = cnt / BG_WAVE;
@@ -243,8 +246,9 @@ __device__ __forceinline__ void fused_impl_body(const DevNet &n, int d, uint64_t
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (cnt) {
-        if (dense) impl_row<SK, KMAX, true>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc);
-        else impl_row<SK, KMAX, false>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc);
+        auto none = [] {};
+        if (dense) impl_row<SK, KMAX, true>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc, none);
+        else impl_row<SK, KMAX, false>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc, none);
     }
 #pragma unroll
     for (int k = 0; k < KMAX; k++)
@@ -335,13 +339,222 @@ __device__ __forceinline__ void fused_impl_body(const DevNet &n, int d, uint64_t
     }
 }
 
+// The butterfly of wave_sum -- lanes l and l + 32 first, then l and l + 16, ..., l and l + 1, the
+// same pairs and therefore the same bits -- with moves that need no address registers: one
+// permute across the halves, a swizzle within 32 lanes, then row shifts inside the first row.
+// The total is valid in lane 15 only.
+__device__ __forceinline__ float wave_sum_lane15(float v)
+{
+    v = v + __shfl_xor(v, 32, 64);
+    v = v + __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));   // lane ^ 16
+    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, false));
+    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, false));
+    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, false));
+    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+    return v;
+}
+
+// One workgroup of W = BG_BLOCK / 64 wavefronts advances W destinations through one block: those at
+// positions d0 .. d0 + W - 1 of the network's longest-row-first order (k_lpt_order). Each wavefront walks its destination's row (impl_row) and
+// leaves the K summation-point sums in LDS. While its row's first loads are in flight it stages
+// what the per-neuron work needs into LDS -- its neuron's state, summation point and fire word,
+// the block's noise (lanes k), the neuron's outgoing (cohort, target class) states as a source --
+// so that nothing after the row waits on memory; after the row it applies this block's deliveries
+// to those source states. The work that is serial in the steps -- the stimulus input, the neuron
+// update and the next block's first delivery -- then runs once for all W neurons on wavefront 0,
+// lane w taking neuron d0 + w. Every value is computed by the same operations as in
+// fused_impl_body, so the results are the same bits.
+template <int NK, int SK, int KMAX>
+__device__ __forceinline__ void fused_impl_group(const DevNet &n, int d0, uint64_t t0, int K, int sel)
+{
+    constexpr int W = BG_BLOCK / BG_WAVE;
+    constexpr int NP = 2 * BG_COH_MAX;
+    __shared__ float s_acc[W][KMAX];
+    __shared__ float s_noise[W][KMAX];
+    __shared__ float s_in[W][KMAX];
+    __shared__ uint2 s_tab[W][4];
+    __shared__ uint64_t s_coh[W][BG_COH_MAX];
+    __shared__ float s_pr[W][NP], s_pu[W][NP];
+    __shared__ uint64_t s_plast[W][NP];
+    __shared__ NeuronRegs s_nr[W];
+    __shared__ float s_sp[W];
+    __shared__ uint64_t s_rec[W];
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / BG_WAVE));
+    // the workgroup's destinations: positions d0 .. d0 + W - 1 of the longest-row-first order
+    auto dest = [&](int g) { return g < n.N ? (int)n.lpt[g] : n.N; };
+    const int d = __builtin_amdgcn_readfirstlane(dest(d0 + wv));
+    const uint64_t *rec = n.rec[sel];
+    const uint32_t cm = n.coh_mask;
+    const int npair = SK == SK_DS ? 2 * __popc(cm) : 0;
+    TypeRegs tr;
+    type_regs(n, tr);
+
+    if (d < n.N) {
+        const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
+        const bool dense = (cnt == hi);
+        const uint32_t row = (uint32_t)d * (uint32_t)n.M;
+        const uint32_t *imap = n.in_map + (dense ? 0 : n.in_begin[d]);
+        if (lane < 4) s_tab[wv][lane] = make_uint2(__float_as_uint(n.decay[lane]), 64u - (uint32_t)n.tdelay[lane]);
+        if (lane < BG_COH_MAX) s_coh[wv][lane] = n.coh_step[lane];
+        const float2 *fac = n.cfac + (size_t)sel * BG_COH_MAX * 2 * (size_t)n.N;
+        const bool young = t0 < n.young_until;
+        const int refr_d = n.refr[d];
+        // lane j < npair: outgoing state j of this neuron as a source, (cohort j / 2, class j % 2)
+        int pc = 0;
+        size_t pidx = 0;
+        if (SK == SK_DS && lane < npair) {
+            uint32_t m = cm;
+            for (int j = 0; j < (lane >> 1); j++) m &= m - 1;
+            pc = __ffs(m) - 1;
+            pidx = ((size_t)pc * 2 + (size_t)(lane & 1)) * (size_t)n.N + (size_t)d;
+        }
+        auto stage = [&] {
+            // loads first, then the noise while they fly, then the LDS copies
+            NeuronRegs s;
+            float spv = 0.0f, pr = 0.0f, pu = 0.0f;
+            uint64_t rv = 0, pl = 0;
+            if (lane == 0) {
+                neuron_load<NK>(n, d, s);
+                spv = n.sp[d];
+                rv = rec[d];
+            }
+            if (SK == SK_DS && lane < npair) {
+                pr = n.cst_r[pidx];
+                pu = n.cst_u[pidx];
+                pl = n.cst_last[pidx];
+            }
+            // the block's noise, unless the neuron stays refractory through the block (then
+            // neuron_advance never reads it)
+            if (refr_d < K && lane < K) s_noise[wv][lane] = bg_noise_normal(n.seed, (uint32_t)d, t0 + (uint64_t)lane);
+            if (lane == 0) {
+                s_nr[wv] = s;
+                s_sp[wv] = spv;
+                s_rec[wv] = rv;
+            }
+            if (SK == SK_DS && lane < npair) {
+                s_pr[wv][lane] = pr;
+                s_pu[wv][lane] = pu;
+                s_plast[wv][lane] = pl;
+            }
+        };
+        float acc[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) acc[k] = 0.0f;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (cnt) {
+            if (dense) impl_row<SK, KMAX, true>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc, stage);
+            else impl_row<SK, KMAX, false>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc, stage);
+        } else {
+            stage();
+        }
+#pragma unroll
+        for (int k = 0; k < KMAX; k++)
+            if (BG_K_EXACT(KMAX) || k < K) acc[k] = wave_sum_lane15(acc[k]);
+        if (lane == 15) {
+#pragma unroll
+            for (int k = 0; k < KMAX; k++)
+                if (BG_K_EXACT(KMAX) || k < K) s_acc[wv][k] = acc[k];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // this block's deliveries to the neuron's outgoing synapses, from its fire word before the
+        // block
+        if (SK == SK_DS && lane < npair) {
+            float pr = s_pr[wv][lane], pu = s_pu[wv][lane];
+            uint64_t plast = s_plast[wv][lane];
+            const uint64_t pE = s_coh[wv][pc];
+            const int type = (n.ntype[d] == 1 ? 0 : 2) + (lane & 1);
+            const uint32_t delay = (uint32_t)n.tdelay[type];
+            const float U = pick4(tr.U, type), D = pick4(tr.D, type), F = pick4(tr.F, type);
+            const uint32_t kmK = K >= 32 ? 0xFFFFFFFFu : ((1u << K) - 1u);
+            uint32_t hr = (uint32_t)(s_rec[wv] >> (64u - delay)) & kmK & since_mask(pE, delay, t0);
+            if (hr) {
+                do {
+                    const int k = __ffs(hr) - 1;
+                    hr &= hr - 1;
+                    ds_deliver(n, pr, pu, plast, t0 + (uint64_t)k, U, D, F);
+                } while (hr);
+                n.cst_r[pidx] = pr;
+                n.cst_u[pidx] = pu;
+                n.cst_last[pidx] = plast;
+                s_pr[wv][lane] = pr;
+                s_pu[wv][lane] = pu;
+                s_plast[wv][lane] = plast;
+            }
+        }
+    }
+    __syncthreads();
+    if (wv != 0) return;
+
+    uint32_t mask = 0;
+    uint64_t rec_new = 0;
+    const int dn = lane < W ? dest(d0 + lane) : n.N;
+    if (dn < n.N) {
+        uint32_t inbits = 0;
+        if (n.in_kind != BG_INPUT_NONE) {
+            for (int k = 0; k < K; k++) {
+                float add;
+                if (input_step(n, dn, t0 + (uint64_t)k, &add)) {
+                    s_in[lane][k] = add;
+                    inbits |= 1u << k;
+                }
+            }
+        }
+        NeuronRegs s = s_nr[lane];
+        float sp = s_sp[lane];
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            if (BG_K_EXACT(KMAX) || k < K) {
+                if ((inbits >> k) & 1u) sp = sp + s_in[lane][k];
+                if (neuron_advance<NK>(n, dn, t0 + (uint64_t)k, sp, s_noise[lane][k], s)) mask |= 1u << k;
+                sp = 0.0f + s_acc[lane][k];
+            }
+        }
+        neuron_store<NK>(n, dn, s);
+        n.sp[dn] = sp;
+        n.mask[sel ^ 1][dn] = mask;
+        if (mask) n.fired_step[dn] = t0 + (uint64_t)(31 - __clz(mask));
+        rec_new = (s_rec[lane] >> K) | ((uint64_t)mask << (64 - K));
+        n.rec[sel ^ 1][dn] = rec_new;
+    }
+    // the first delivery of the next block (of up to `look` steps) to the outgoing synapses of
+    // neuron w, lane w * npair + j
+    if (SK == SK_DS && npair) {
+        const int w = lane / npair, j = lane - w * npair;
+        const uint64_t rw = __shfl(rec_new, w < W ? w : 0);
+        const int dd = __shfl(dn, w < W ? w : 0);
+        if (w < W && dd < n.N) {
+            uint32_t m = cm;
+            for (int q = 0; q < (j >> 1); q++) m &= m - 1;
+            const int pc = __ffs(m) - 1;
+            const int tsub = j & 1;
+            const int type = (n.ntype[dd] == 1 ? 0 : 2) + tsub;
+            const uint32_t delay = (uint32_t)n.tdelay[type];
+            const uint64_t pE = s_coh[0][pc];
+            const uint64_t T = t0 + (uint64_t)K;
+            const uint32_t kmL = n.look >= 32 ? 0xFFFFFFFFu : ((1u << n.look) - 1u);
+            const uint32_t hl = (uint32_t)(rw >> (64u - delay)) & kmL & since_mask(pE, delay, T);
+            if (hl) {
+                const float U = pick4(tr.U, type), D = pick4(tr.D, type), F = pick4(tr.F, type);
+                float r2 = s_pr[w][j], u2 = s_pu[w][j];
+                uint64_t l2 = s_plast[w][j];
+                ds_deliver(n, r2, u2, l2, T + (uint64_t)(__ffs(hl) - 1), U, D, F);
+                n.cfac[(((size_t)(sel ^ 1) * BG_COH_MAX + (size_t)pc) * 2 + (size_t)tsub) * (size_t)n.N + (size_t)dd] =
+                    make_float2(u2, r2);
+            }
+        }
+    }
+}
+
 template <int NK, int SK, int KMAX>
 __global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_IMPL_WAVES))) void k_fused_impl(
     DevNet n, uint64_t t0, int K, int sel)
 {
-    const int d = (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE);
-    if (d >= n.N) return;
-    fused_impl_body<NK, SK, KMAX>(n, __builtin_amdgcn_readfirstlane(d), t0, K, sel);
+    fused_impl_group<NK, SK, KMAX>(n, (int)(blockIdx.x * (BG_BLOCK / BG_WAVE)), t0, K, sel);
 }
 
 template <int NK, int SK, int KMAX>

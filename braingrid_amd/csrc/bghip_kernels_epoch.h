@@ -562,6 +562,30 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t *counts, ui
     if (tid == 0) *total = s_carry;
 }
 
+// Destinations ordered by row length, longest first (a counting sort over lengths capped at 1023,
+// one workgroup): the implicit-ring fused kernels hand out their workgroups in this order, so the
+// longest rows start first and the short ones fill the end of a launch. Within a length the order
+// is whatever the atomics give; no result depends on it.
+__global__ __launch_bounds__(1024) void k_lpt_order(const uint32_t *counts, uint32_t *order, int N)
+{
+    __shared__ uint32_t s_hist[1024];
+    const int tid = threadIdx.x;
+    s_hist[tid] = 0;
+    __syncthreads();
+    for (int i = tid; i < N; i += 1024) atomicAdd(&s_hist[1023u - min(counts[i], 1023u)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int b = 0; b < 1024; b++) {
+            const uint32_t c = s_hist[b];
+            s_hist[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += 1024) order[atomicAdd(&s_hist[1023u - min(counts[i], 1023u)], 1u)] = (uint32_t)i;
+}
+
 // incoming map: in-use slots of each destination in ascending slot order (wave per dest).
 __global__ __launch_bounds__(BG_BLOCK) void k_fill_incoming(DevNet n)
 {
