@@ -69,6 +69,67 @@ def workload_cfg():
     return c
 
 
+REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "bgref_dump")
+REF_CONFIG_TEMPLATE = os.path.join(ROOT, "tests", "golden", "configs", "conn-lif-ds-dynamic.xml")
+
+
+def reference_config_xml() -> str:
+    """The benchmark workload as a reference parameter file: the committed 8 x 8 conn-lif-ds-dynamic
+    config (test-small parameter distributions, DS synapses, ConnGrowth) at 100 x 100, M = 1000,
+    maxFiringRate 1000, 10 s epochs, seed 1 (DynamicLayout 0.9 / 0.1 comes from the harness)."""
+    with open(REF_CONFIG_TEMPLATE) as f:
+        x = f.read()
+    reps = [
+        ('<x name="x">8</x><y name="y">8</y>', f'<x name="x">{GRID}</x><y name="y">{GRID}</y>'),
+        ('<Tsim name="Tsim">0.5</Tsim><numSims name="numSims">3</numSims>',
+         f'<Tsim name="Tsim">{EPOCH}</Tsim><numSims name="numSims">10</numSims>'),
+        ('<maxFiringRate name="maxFiringRate">200</maxFiringRate><maxSynapsesPerNeuron name="maxSynapsesPerNeuron">40'
+         '</maxSynapsesPerNeuron>',
+         f'<maxFiringRate name="maxFiringRate">{MAX_RATE}</maxFiringRate><maxSynapsesPerNeuron '
+         f'name="maxSynapsesPerNeuron">{M}</maxSynapsesPerNeuron>'),
+        ('<Seed name="Seed"><value name="value">7</value></Seed>', '<Seed name="Seed"><value name="value">1</value></Seed>'),
+    ]
+    for a, b in reps:
+        if a not in x:
+            raise RuntimeError(f"reference config template changed: {a!r} not found")
+        x = x.replace(a, b)
+    return x
+
+
+class ReferenceBaseline:
+    """Times the reference's own CPU step (oracle/_ref/bgref_dump, BrainGrid's single-threaded model
+    compiled from its sources) on the same pre-grown network: the reference's growth update from
+    radius 8.9, then `steps` timed calls of IModel::advance, on one core. Started before the GPU
+    phases (its setup is the reference's O(N^2) growth update on dense matrices, about a minute)
+    and collected after them; only its timed steps enter the figure."""
+
+    def __init__(self, steps: int):
+        import subprocess
+        import tempfile
+        self.td = tempfile.TemporaryDirectory()
+        cfg = os.path.join(self.td.name, "bench10k.xml")
+        with open(cfg, "w") as f:
+            f.write(reference_config_xml())
+        self.t0 = time.perf_counter()
+        self.proc = subprocess.Popen([REF_DUMP, cfg, os.devnull, "--dynamic-layout", "0.9", "0.1", "--time-pregrown",
+                                      str(R0), str(steps)], cwd=self.td.name, stdout=subprocess.PIPE,
+                                     stderr=subprocess.DEVNULL, text=True)
+
+    def result(self) -> dict:
+        out, _ = self.proc.communicate(timeout=900)
+        total = time.perf_counter() - self.t0
+        self.td.cleanup()
+        if self.proc.returncode != 0:
+            raise RuntimeError(f"bgref_dump exited with {self.proc.returncode}")
+        res = json.loads(out.strip().splitlines()[-1])
+        S, dt = res["synapses"], res["seconds"]
+        return {"value": S * res["steps"] / dt, "unit": "synapse-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"the reference's CPU model (BrainGrid's SingleThreadedSpikingModel compiled from its "
+                          f"sources, Norm noise), {res['steps']} timed steps of the same pre-grown {GRID * GRID}-neuron "
+                          f"network ({S} synapses) after its own growth update from radius {R0}, {dt:.1f} s of "
+                          f"steps ({total:.0f} s with its setup, run beside the GPU phases)"}
+
+
 def cpu_baseline(budget_s: float):
     """Times the oracle (reference algorithm, reference noise mode, one core) on the same network."""
     from oracle import oracle_py as O
@@ -103,8 +164,12 @@ def oracle_sweep_net(spec: dict):
         c.update(width=spec["side"], height=spec["side"], layout="DynamicLayout", frac_exc=spec["fE"],
                  frac_starter=0.1, target_rate=spec["tR"], seed=spec["seed"], epoch_duration=SW.EPOCH_S,
                  num_epochs=int(round(spec["sim_s"] / SW.EPOCH_S)), max_synapses_per_neuron=SW.LIF_M,
-                 max_firing_rate=SW.LIF_MAX_RATE)
-        return O.OracleNet(c)
+                 max_firing_rate=SW.lif_max_rate(spec))
+        net = O.OracleNet(c)
+        if spec.get("r0"):
+            net.set_radii(spec["r0"])
+            net.update_connections()
+        return net
     from tests import helpers as H
     N = spec["side"] ** 2
     neu, syn, meta = NW.izh_stdp_arrays(N, spec["seed"], k=min(100, N - 1), max_rate=SW.IZH_MAX_RATE,
@@ -157,7 +222,7 @@ def forms_cpu_baseline(specs: list, sample: list, steps: int, threads: int) -> d
                       f"from the sample per family by neuron-seconds"}
 
 
-KERNEL_SOURCES = ("bghip_kernels.h", "bghip_implicit.h", "bghip_device.h", "bghip_launch.h", "bghip_fused_lif.hip")
+KERNEL_SOURCES = ("bghip_kernels.h", "bghip_implicit.h", "bghip_device.h", "bghip_launch.h", "bghip_impl_lif.hip")
 
 
 def kernel_source_sha() -> str:
@@ -268,7 +333,7 @@ def batched_result(args):
     from braingrid_amd import sweep as SW
 
     ctx = B.Context(0)
-    specs = SW.sweep_specs(args.nets, args.seed, sim_s=args.sim_seconds)
+    specs = SW.sweep_specs(args.nets, args.seed, sim_s=args.sim_seconds, lif_r0=args.lif_r0)
     N_total = sum(s["side"] ** 2 for s in specs)
     sim_total = sum(s["sim_s"] for s in specs)
     log(f"sweep: {len(specs)} networks, {N_total} neurons, {sim_total} simulated s")
@@ -284,6 +349,7 @@ def batched_result(args):
     t_alone = t_alone_sample * len(specs) / len(sample)
     nets = SW.build(ctx, specs, log)
     S_total = sum(x.synapses for x in nets)
+    S_lif = sum(x.synapses for x in nets if x.kind == "lif_growth")
     shapes = []
     t_batched = SW.run_batched(ctx, nets, log, tune_steps=args.tune_steps, shapes=shapes)
     fp_batched = {x.spec["id"]: x.fingerprint() for x in nets if x.spec["id"] in fp_alone}
@@ -292,6 +358,7 @@ def batched_result(args):
         x.close()
     same = sum(fp_alone[k] == fp_batched[k] for k in fp_alone)
     value = sim_total / t_batched * 3600.0
+    fused = fused_group_roofline(specs, S_lif, shapes) if args.lif_r0 else None
     out = {
         "metric": "sims/hr (simulated network-seconds per wall hour) of a batched sweep",
         "value": value,
@@ -306,7 +373,8 @@ def batched_result(args):
         "dtype": "f32",
         "data": "synthetic (seeded sweep; deterministic Philox noise and Poisson input)",
         "config": {"workload": f"batched sweep: {len(specs)} networks, sides log-uniform 10..120, "
-                               f"LIF+DS+growth / Izhikevich+DynamicSTDP+Poisson, {args.sim_seconds} s each",
+                               f"LIF+DS+growth{f' pre-grown to r0={args.lif_r0}' if args.lif_r0 else ''} / "
+                               f"Izhikevich+DynamicSTDP+Poisson, {args.sim_seconds} s each",
                    "networks": len(specs), "neurons": N_total, "synapses": int(S_total),
                    "lif_growth": sum(s["kind"] == "lif_growth" for s in specs),
                    "izh_stdp": sum(s["kind"] == "izh_stdp" for s in specs), "seed": args.seed},
@@ -321,12 +389,36 @@ def batched_result(args):
         "autotune_steps": args.tune_steps,
         "launch_shapes": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in s.items()} for s in shapes],
     }
+    if fused is not None:
+        out["fused_group"] = fused
     ctx.close()
     if not args.no_cpu:
         cs = specs[:: max(1, len(specs) // max(1, args.cpu_networks))]
         out["cpu_baseline"] = forms_cpu_baseline(specs, cs, args.cpu_steps, args.cpu_threads)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     return out
+
+
+def fused_group_roofline(specs, S_lif: int, shapes: list) -> dict | None:
+    """The batch's fused Spiking/DS group (the pre-grown growth networks, one shared implicit-ring
+    launch per 9-step block): its time per step as the launch-shape autotuner measured it (HIP
+    events on the batch stream over its trial steps), priced with the implicit byte model without
+    the per-delivery term (a lower bound on the bytes)."""
+    g = [x for x in shapes if x.get("mode") == 1]
+    if not g or not g[0].get("step_us"):
+        return None
+    g = g[0]
+    N = g["work_items"]
+    K = 9
+    launch_bytes = IMPL_SYN_BLOCK_BYTES * S_lif + (IMPL_NEURON_BLOCK_BYTES + 2 * IMPL_COHORT_CLASS_BYTES) * N
+    step_s = g["step_us"] * 1e-6
+    ach = launch_bytes / K / step_s
+    return {"kernel": "k_fused_impl_batch<LIF,DS,9>", "networks": g["networks"], "neurons": N, "synapses": int(S_lif),
+            "block": g["block"], "step_us": g["step_us"], "synapse_updates_per_s": S_lif / step_s,
+            "byte_model": f"per 9-step launch: {IMPL_SYN_BLOCK_BYTES} B x synapses + "
+                          f"{IMPL_NEURON_BLOCK_BYTES + 2 * IMPL_COHORT_CLASS_BYTES} B x neurons (deliveries not counted)",
+            "achieved": ach / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
+            "timing": "the autotuner's trial steps of the group's chosen shape (HIP events on the batch stream)"}
 
 
 def served_result(args):
@@ -492,6 +584,9 @@ def main():
     ap.add_argument("--nets", type=int, default=256, help="batched form: networks in the sweep")
     ap.add_argument("--sim-seconds", type=float, default=5.0, help="batched form: simulated seconds per network")
     ap.add_argument("--seed", type=int, default=2024, help="batched form: sweep seed")
+    ap.add_argument("--lif-r0", type=float, default=None,
+                    help="batched form: start the growth networks pre-grown to this radius (2.8: about 100 "
+                         "synapses per neuron), so they run the fused kernels")
     ap.add_argument("--requests", type=int, default=2000, help="served form: requests in the stream")
     ap.add_argument("--rate", type=float, default=20.0, help="served form: mean arrival rate (requests/s)")
     ap.add_argument("--hbm-fraction", type=float, default=0.9,
@@ -506,6 +601,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--roofline-steps", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-ref-steps", type=int, default=300,
+                    help="single form: timed steps of the reference's own CPU model (oracle/_ref/bgref_dump)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-networks", type=int, default=12,
                     help="batched/served forms: networks of the workload timed in the CPU oracle")
@@ -554,6 +651,9 @@ def main():
     from braingrid_amd import bghip as B
     from braingrid_amd import host
 
+    # the reference's own CPU step on the same network, when its harness is built (it travels with
+    # the tree): started now, collected after the GPU phases of this network
+    ref_base = ReferenceBaseline(args.cpu_ref_steps) if not args.no_cpu and os.path.exists(REF_DUMP) else None
     ctx = B.Context(0)
     cfg = workload_cfg()
     model = host.Model(ctx, cfg)
@@ -675,7 +775,18 @@ def main():
         out["per_step"] = per_step_roofline(ctx, args.per_step_steps)
     ctx.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        ref = None
+        if ref_base is not None:
+            try:
+                ref = ref_base.result()
+            except Exception as e:   # the port's figure stands; say why
+                out["cpu_reference_error"] = str(e)
+        port = cpu_baseline(args.cpu_seconds)
+        if ref is not None:
+            out["cpu_baseline"] = ref
+            out["cpu_port"] = port
+        else:
+            out["cpu_baseline"] = port
     if not args.no_forms:
         out.update(forms_summary(args))
     print(json.dumps(out), flush=True)

@@ -97,6 +97,7 @@ struct bghip_net {
     int32_t *in_idx_host;  // N: delayIdx of neurons without input (their synapse never advances)
     uint8_t *in_mask_host; // N: copy of dn.in_mask
     uint64_t *h_total;     // pinned landing slot of the synapse total after an index-map rebuild
+    uint32_t *h_err;       // pinned landing slot of the device error bits (read by bghip_sync)
     bool total_pending;    // h_total not read yet (the rebuild is still in flight)
     // on-device recorder (XmlRecorder / XmlGrowthRecorder histories)
     uint32_t *d_burst, *d_shist;    // spike counts per 1 s and 10 ms bin
@@ -376,9 +377,15 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
         delete n;
         return set_err(ctx, BGHIP_EDEVICE, "hipStreamCreate failed");
     }
+    n->h_total = nullptr;
+    n->h_err = nullptr;
+    n->h_coh_used = nullptr;
     if (hipHostMalloc((void **)&n->h_total, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&n->h_err, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&n->h_coh_used, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
-        (void)hipHostFree(n->h_total);
+        if (n->h_total) (void)hipHostFree(n->h_total);
+        if (n->h_err) (void)hipHostFree(n->h_err);
+        if (n->h_coh_used) (void)hipHostFree(n->h_coh_used);
         (void)hipStreamDestroy(n->stream);
         (void)hipFree(n->arena);
         delete n;
@@ -434,6 +441,7 @@ extern "C" void bghip_net_destroy(bghip_net *n)
     free(n->in_idx_host);
     free(n->in_mask_host);
     (void)hipHostFree(n->h_total);
+    (void)hipHostFree(n->h_err);
     (void)hipFree(n->d_burst);
     (void)hipFree(n->d_shist);
     (void)hipFree(n->d_rates_hist);
@@ -900,24 +908,29 @@ extern "C" int bghip_advance(bghip_net *n, uint64_t n_steps)
     return BGHIP_OK;
 }
 
+// The error word copied behind everything queued on the stream into a pinned slot, then one wait.
+static int read_error_bits(bghip_net *n, uint32_t *e)
+{
+    net_begin(n);
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    HIP_TRY(n->ctx, hipMemcpyAsync(n->h_err, n->dn.err, 4, hipMemcpyDeviceToHost, n->stream));
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    *e = *n->h_err;
+    return BGHIP_OK;
+}
+
 extern "C" uint32_t bghip_net_error_bits(bghip_net *n)
 {
     uint32_t e = 0;
-    if (!n) return 0;
-    net_begin(n);
-    (void)hipSetDevice(n->ctx->device);
-    (void)hipMemcpyAsync(&e, n->dn.err, 4, hipMemcpyDeviceToHost, n->stream);
-    (void)hipStreamSynchronize(n->stream);
+    if (!n || read_error_bits(n, &e)) return 0;
     return e;
 }
 
 extern "C" int bghip_sync(bghip_net *n)
 {
     if (!n) return BGHIP_EINVAL;
-    net_begin(n);
-    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
-    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
-    uint32_t e = bghip_net_error_bits(n);
+    uint32_t e = 0;
+    CHK(read_error_bits(n, &e));
     if (e) return set_err(n->ctx, BGHIP_ESTATE, "device error bits 0x%x", e);
     return BGHIP_OK;
 }
@@ -1684,6 +1697,8 @@ struct BatchGroup {
     uint32_t nchunks[3] = {};
     uint2 *d_nwaves = nullptr;     // thread-per-neuron kernels: {member, first neuron} per wavefront
     uint32_t nnwaves = 0;
+    uint2 *d_quads = nullptr;      // implicit-ring fused kernels: {member, first position} per workgroup
+    uint32_t nquads = 0;
     BatchShape shape = {};         // launch shape in use
     bool tuned = false;            // shape picked by the autotuner (or set by the caller)
     float step_us = 0.0f;          // measured time per step of that shape
@@ -1729,6 +1744,7 @@ static void free_groups(bghip_batch *b)
         (void)hipFree(g.d_nof);
         for (uint2 *c : g.d_chunks) (void)hipFree(c);
         (void)hipFree(g.d_nwaves);
+        (void)hipFree(g.d_quads);
         (void)hipFree(g.d_sbase);
         (void)hipFree(g.d_tprev);
         (void)hipFree(g.d_sel);
@@ -1877,6 +1893,15 @@ static int regroup(bghip_batch *b)
         g.nnwaves = (uint32_t)nwaves.size();
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_nwaves, sizeof(uint2) * std::max<size_t>(nwaves.size(), 1)));
         HIP_TRY(b->ctx, hipMemcpy(g.d_nwaves, nwaves.data(), sizeof(uint2) * nwaves.size(), hipMemcpyHostToDevice));
+        std::vector<uint2> quads;
+        if (g.impl)
+            for (int k = 0; k < nn; k++)
+                for (uint32_t first = 0; first < base[k + 1] - base[k]; first += BG_BLOCK / BG_WAVE)
+                    quads.push_back(make_uint2(k, first));
+        g.nquads = (uint32_t)quads.size();
+        HIP_TRY(b->ctx, hipMalloc((void **)&g.d_quads, sizeof(uint2) * std::max<size_t>(quads.size(), 1)));
+        if (!quads.empty())
+            HIP_TRY(b->ctx, hipMemcpy(g.d_quads, quads.data(), sizeof(uint2) * quads.size(), hipMemcpyHostToDevice));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_nets, sizeof(BatchNet) * nn));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_base, sizeof(uint32_t) * (nn + 1)));
         HIP_TRY(b->ctx, hipMalloc((void **)&g.d_sbase, sizeof(uint64_t) * (nn + 1)));
@@ -1951,7 +1976,7 @@ static BatchMap batch_map(const BatchGroup &g, const BatchShape &sh)
 {
     const int v = sh.dpw == 1 ? 0 : (sh.dpw == 2 ? 1 : 2);
     return BatchMap{g.d_nets, g.d_base,      g.d_nof,    g.total, (int)g.members.size(), g.d_chunks[v],
-                    g.nchunks[v], g.d_nwaves, g.nnwaves};
+                    g.nchunks[v], g.d_nwaves, g.nnwaves, g.d_quads, g.nquads};
 }
 
 // ---- batch autotuner -----------------------------------------------------------------------
@@ -1963,6 +1988,9 @@ struct ShapeAxis {
 
 static std::vector<ShapeAxis> quiet_space() { return {{&BatchShape::blk, {256, 128, 64}}}; }
 static std::vector<ShapeAxis> fused_space() { return {{&BatchShape::blk, {256, 128, 64}}}; }
+// implicit-ring fused groups run one workgroup of BG_BLOCK threads per four destinations: the one
+// shape is still timed, so the group reports its time per step
+static std::vector<ShapeAxis> fused_impl_space() { return {{&BatchShape::blk, {BG_BLOCK}}}; }
 static std::vector<ShapeAxis> step_space(const BatchGroup &g)
 {
     if (g.sk != BGHIP_SYN_STDP && g.sk != BGHIP_SYN_DYNSTDP) return {{&BatchShape::blk, {256, 128, 64}}};
@@ -2173,7 +2201,7 @@ extern "C" int bghip_batch_advance(bghip_batch *b, uint64_t n_steps)
             const uint64_t TL = std::max<uint64_t>(4, (T + (uint64_t)Kfull - 1) / (uint64_t)Kfull);
             if (tune && L > 4 * (TL + 1)) {
                 // trial: one launch (first of the shape), then TL timed launches of Kfull steps
-                CHK(tune_group(b, g, fused_space(), [&](const BatchShape &sh, float &us) {
+                CHK(tune_group(b, g, g.impl ? fused_impl_space() : fused_space(), [&](const BatchShape &sh, float &us) {
                     const BatchMap m = batch_map(g, sh);
                     fused_launch(m, sh, Kfull, (int)l++);
                     return time_steps(b, us, TL * (uint64_t)Kfull, [&] {

@@ -1,7 +1,7 @@
 // Multi-network launchers of the fused, per-step Spiking/DS and neuron-only kernels
 // (k_fused_batch / k_neuron_batch / k_synapse_batch / k_neuron_only_batch in bghip_kernels.h).
 // Every kernel takes the batch's BatchMap by value; block sizes come from the group's BatchShape.
-#include "bghip_implicit.h"
+#include "bghip_kernels.h"
 #include "bghip_launch.h"
 #include <algorithm>
 
@@ -42,30 +42,6 @@ void launch_fused_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh,
     } else {
         if (sk == SK_DS) fused_w<NK_IZH, SK_DS>(bm, s, sh.blk, w32, kmax, K, Kfull, launch);
         else fused_w<NK_IZH, SK_SPIKING>(bm, s, sh.blk, w32, kmax, K, Kfull, launch);
-    }
-}
-
-template <int NK, int SK>
-static void fused_impl(const BatchMap &bm, hipStream_t s, int blk, int kmax, int K, int Kfull, int launch)
-{
-    const dim3 grid(wave_blocks(bm.total, blk)), block(blk);
-    if (K == BG_K_COMMON)
-        hipLaunchKernelGGL((k_fused_impl_batch<NK, SK, BG_K_COMMON>), grid, block, 0, s, bm, K, Kfull, launch);
-    else if (kmax <= 16)
-        hipLaunchKernelGGL((k_fused_impl_batch<NK, SK, 16>), grid, block, 0, s, bm, K, Kfull, launch);
-    else
-        hipLaunchKernelGGL((k_fused_impl_batch<NK, SK, 32>), grid, block, 0, s, bm, K, Kfull, launch);
-}
-
-void launch_fused_impl_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, int sk, int kmax,
-                             int K, int Kfull, int launch)
-{
-    if (nk == NK_LIF) {
-        if (sk == SK_DS) fused_impl<NK_LIF, SK_DS>(bm, s, sh.blk, kmax, K, Kfull, launch);
-        else fused_impl<NK_LIF, SK_SPIKING>(bm, s, sh.blk, kmax, K, Kfull, launch);
-    } else {
-        if (sk == SK_DS) fused_impl<NK_IZH, SK_DS>(bm, s, sh.blk, kmax, K, Kfull, launch);
-        else fused_impl<NK_IZH, SK_SPIKING>(bm, s, sh.blk, kmax, K, Kfull, launch);
     }
 }
 

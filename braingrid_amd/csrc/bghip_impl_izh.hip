@@ -10,6 +10,8 @@ static void go_impl(const DevNet &dn, hipStream_t s, int kmax, uint64_t t0, int 
     unsigned blocks = (unsigned)(((uint64_t)dn.N * BG_WAVE + BG_BLOCK - 1) / BG_BLOCK);
     if (K == BG_K_COMMON)
         hipLaunchKernelGGL((k_fused_impl<NK_IZH, SK, BG_K_COMMON>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t0, K, sel);
+    else if (K < BG_K_COMMON)
+        hipLaunchKernelGGL((k_fused_impl<NK_IZH, SK, 8>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t0, K, sel);
     else if (kmax <= 16)
         hipLaunchKernelGGL((k_fused_impl<NK_IZH, SK, 16>), dim3(blocks), dim3(BG_BLOCK), 0, s, dn, t0, K, sel);
     else

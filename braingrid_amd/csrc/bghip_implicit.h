@@ -215,130 +215,6 @@ __device__ __forceinline__ void impl_row(const DevNet &n, const TypeRegs &tr, co
     }
 }
 
-template <int NK, int SK, int KMAX>
-__device__ __forceinline__ void fused_impl_body(const DevNet &n, int d, uint64_t t0, int K, int sel)
-{
-    __shared__ float s_noise[BG_BLOCK / BG_WAVE][KMAX];
-    __shared__ float s_in[BG_BLOCK / BG_WAVE][KMAX];
-    const int lane = threadIdx.x & (BG_WAVE - 1);
-    const int wv = threadIdx.x / BG_WAVE;
-    __shared__ uint2 s_tab[BG_BLOCK / BG_WAVE][4];
-    const uint32_t cnt = n.syn_count[d], hi = n.slot_hi[d];
-    const bool dense = (cnt == hi);
-    const uint32_t row = (uint32_t)d * (uint32_t)n.M;
-    const uint32_t *imap = n.in_map + (dense ? 0 : n.in_begin[d]);
-    const uint64_t *rec = n.rec[sel];
-    __shared__ uint64_t s_coh[BG_BLOCK / BG_WAVE][BG_COH_MAX];
-    if (lane < 4) s_tab[wv][lane] = make_uint2(__float_as_uint(n.decay[lane]), 64u - (uint32_t)n.tdelay[lane]);
-    if (lane < BG_COH_MAX) s_coh[wv][lane] = n.coh_step[lane];
-    const float2 *fac = n.cfac + (size_t)sel * BG_COH_MAX * 2 * (size_t)n.N;
-    const bool young = t0 < n.young_until;
-    // noise of the block's steps, unless the neuron stays refractory through all of them (then
-    // neuron_advance never reads it)
-    if (n.refr[d] < K && lane < K) s_noise[wv][lane] = bg_noise_normal(n.seed, (uint32_t)d, t0 + (uint64_t)lane);
-    float acc[KMAX];
-#pragma unroll
-    for (int k = 0; k < KMAX; k++) acc[k] = 0.0f;
-
-    TypeRegs tr;
-    type_regs(n, tr);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (cnt) {
-        auto none = [] {};
-        if (dense) impl_row<SK, KMAX, true>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc, none);
-        else impl_row<SK, KMAX, false>(n, tr, s_tab[wv], s_coh[wv], rec, fac, row, imap, cnt, t0, K, young, acc, none);
-    }
-#pragma unroll
-    for (int k = 0; k < KMAX; k++)
-        if (BG_K_EXACT(KMAX) || k < K) acc[k] = wave_sum(acc[k]);
-
-    // this neuron as a source: its fire word, and for DS the state of its outgoing synapses per
-    // (cohort, target class) on lanes 0 .. 2 * cohorts - 1 (loaded now, used after the neuron loop)
-    const uint64_t rec_old = rec[d];
-    const uint32_t cm = n.coh_mask;
-    const int npair = SK == SK_DS ? 2 * __popc(cm) : 0;
-    int pc = 0;
-    size_t pidx = 0;
-    float pr = 0.0f, pu = 0.0f;
-    uint64_t plast = 0, pE = 0;
-    if (lane < npair) {
-        uint32_t m = cm;
-        for (int j = 0; j < (lane >> 1); j++) m &= m - 1;
-        pc = __ffs(m) - 1;
-        pidx = ((size_t)pc * 2 + (size_t)(lane & 1)) * (size_t)n.N + (size_t)d;
-        pr = n.cst_r[pidx];
-        pu = n.cst_u[pidx];
-        plast = n.cst_last[pidx];
-        pE = n.coh_step[pc];
-    }
-
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-    uint32_t mask = 0;
-    if (lane == 0) {
-        uint32_t inbits = 0;
-        if (n.in_kind != BG_INPUT_NONE) {
-            for (int k = 0; k < K; k++) {
-                float add;
-                if (input_step(n, d, t0 + (uint64_t)k, &add)) {
-                    s_in[wv][k] = add;
-                    inbits |= 1u << k;
-                }
-            }
-        }
-        NeuronRegs s;
-        neuron_load<NK>(n, d, s);
-        float sp = n.sp[d];
-#pragma unroll
-        for (int k = 0; k < KMAX; k++) {
-            if (BG_K_EXACT(KMAX) || k < K) {
-                if ((inbits >> k) & 1u) sp = sp + s_in[wv][k];
-                if (neuron_advance<NK>(n, d, t0 + (uint64_t)k, sp, s_noise[wv][k], s)) mask |= 1u << k;
-                sp = 0.0f + acc[k];
-            }
-        }
-        neuron_store<NK>(n, d, s);
-        n.sp[d] = sp;
-        n.mask[sel ^ 1][d] = mask;
-        if (mask) n.fired_step[d] = t0 + (uint64_t)(31 - __clz(mask));
-    }
-    mask = (uint32_t)__shfl((int)mask, 0);
-    const uint64_t rec_new = (rec_old >> K) | ((uint64_t)mask << (64 - K));
-    if (lane == 0) n.rec[sel ^ 1][d] = rec_new;
-    if (SK == SK_DS && lane < npair) {
-        const int tsub = lane & 1;
-        const int type = (n.ntype[d] == 1 ? 0 : 2) + tsub;
-        const uint32_t delay = (uint32_t)n.tdelay[type];
-        const float U = pick4(tr.U, type), D = pick4(tr.D, type), F = pick4(tr.F, type);
-        // this block's deliveries, as every synapse of the pair saw them
-        const uint32_t kmK = K >= 32 ? 0xFFFFFFFFu : ((1u << K) - 1u);
-        uint32_t hr = (uint32_t)(rec_old >> (64u - delay)) & kmK & since_mask(pE, delay, t0);
-        while (hr) {
-            const int k = __ffs(hr) - 1;
-            hr &= hr - 1;
-            ds_deliver(n, pr, pu, plast, t0 + (uint64_t)k, U, D, F);
-        }
-        n.cst_r[pidx] = pr;
-        n.cst_u[pidx] = pu;
-        n.cst_last[pidx] = plast;
-        // the first delivery of the next block (of up to `look` steps)
-        const uint64_t T = t0 + (uint64_t)K;
-        const uint32_t kmL = n.look >= 32 ? 0xFFFFFFFFu : ((1u << n.look) - 1u);
-        const uint32_t hl = (uint32_t)(rec_new >> (64u - delay)) & kmL & since_mask(pE, delay, T);
-        if (hl) {
-            float r2 = pr, u2 = pu;
-            uint64_t l2 = plast;
-            ds_deliver(n, r2, u2, l2, T + (uint64_t)(__ffs(hl) - 1), U, D, F);
-            n.cfac[(((size_t)(sel ^ 1) * BG_COH_MAX + (size_t)pc) * 2 + (size_t)tsub) * (size_t)n.N + (size_t)d] =
-                make_float2(u2, r2);
-        }
-    }
-}
-
 // The butterfly of wave_sum -- lanes l and l + 32 first, then l and l + 16, ..., l and l + 1, the
 // same pairs and therefore the same bits -- with moves that need no address registers: one
 // permute across the halves, a swizzle within 32 lanes, then row shifts inside the first row.
@@ -354,16 +230,17 @@ __device__ __forceinline__ float wave_sum_lane15(float v)
     return v;
 }
 
-// One workgroup of W = BG_BLOCK / 64 wavefronts advances W destinations through one block: those at
-// positions d0 .. d0 + W - 1 of the network's longest-row-first order (k_lpt_order). Each wavefront walks its destination's row (impl_row) and
-// leaves the K summation-point sums in LDS. While its row's first loads are in flight it stages
-// what the per-neuron work needs into LDS -- its neuron's state, summation point and fire word,
-// the block's noise (lanes k), the neuron's outgoing (cohort, target class) states as a source --
-// so that nothing after the row waits on memory; after the row it applies this block's deliveries
-// to those source states. The work that is serial in the steps -- the stimulus input, the neuron
-// update and the next block's first delivery -- then runs once for all W neurons on wavefront 0,
-// lane w taking neuron d0 + w. Every value is computed by the same operations as in
-// fused_impl_body, so the results are the same bits.
+// One workgroup of W = BG_BLOCK / 64 wavefronts advances W destinations through one block: those
+// at positions d0 .. d0 + W - 1 of the network's longest-row-first order (k_lpt_order). Each
+// wavefront walks its destination's row (impl_row) and leaves the K summation-point sums in LDS.
+// While its row's first loads are in flight it stages what the per-neuron work needs into LDS --
+// its neuron's state, summation point and fire word, the neuron's outgoing (cohort, target class)
+// states as a source, and on wavefront 0 the block's noise of all W neurons (lanes (w, k), one pass
+// of the generator for the workgroup) -- so that nothing after the row waits on memory; after the
+// row it applies this block's deliveries to those source states. The work that is serial in the
+// steps -- the stimulus input, the neuron update and the next block's first delivery -- then runs
+// once for all W neurons on wavefront 0, lane w taking neuron w of the workgroup. Which workgroup and wavefront a destination lands in
+// changes no value: each destination's row, sums and neuron update are its own.
 template <int NK, int SK, int KMAX>
 __device__ __forceinline__ void fused_impl_group(const DevNet &n, int d0, uint64_t t0, int K, int sel)
 {
@@ -399,7 +276,6 @@ __device__ __forceinline__ void fused_impl_group(const DevNet &n, int d0, uint64
         if (lane < BG_COH_MAX) s_coh[wv][lane] = n.coh_step[lane];
         const float2 *fac = n.cfac + (size_t)sel * BG_COH_MAX * 2 * (size_t)n.N;
         const bool young = t0 < n.young_until;
-        const int refr_d = n.refr[d];
         // lane j < npair: outgoing state j of this neuron as a source, (cohort j / 2, class j % 2)
         int pc = 0;
         size_t pidx = 0;
@@ -424,9 +300,16 @@ __device__ __forceinline__ void fused_impl_group(const DevNet &n, int d0, uint64
                 pu = n.cst_u[pidx];
                 pl = n.cst_last[pidx];
             }
-            // the block's noise, unless the neuron stays refractory through the block (then
-            // neuron_advance never reads it)
-            if (refr_d < K && lane < K) s_noise[wv][lane] = bg_noise_normal(n.seed, (uint32_t)d, t0 + (uint64_t)lane);
+            // wavefront 0: the block's noise of the workgroup's W neurons, lane j = (w, k), unless
+            // neuron w stays refractory through the block (then neuron_advance never reads it)
+            if (wv == 0) {
+                for (int j = lane; j < W * K; j += BG_WAVE) {
+                    const int w = j / K, k = j - w * K;
+                    const int dw = dest(d0 + w);
+                    if (dw < n.N && n.refr[dw] < K)
+                        s_noise[w][k] = bg_noise_normal(n.seed, (uint32_t)dw, t0 + (uint64_t)k);
+                }
+            }
             if (lane == 0) {
                 s_nr[wv] = s;
                 s_sp[wv] = spv;
@@ -557,18 +440,19 @@ __global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_IMP
     fused_impl_group<NK, SK, KMAX>(n, (int)(blockIdx.x * (BG_BLOCK / BG_WAVE)), t0, K, sel);
 }
 
+// Many networks: workgroup q advances the four destinations at positions quads[q].y .. + 3 of
+// member quads[q].x (members are padded to whole workgroups, so a workgroup's network is uniform).
 template <int NK, int SK, int KMAX>
 __global__ __launch_bounds__(BG_BLOCK) __attribute__((amdgpu_waves_per_eu(BG_IMPL_WAVES))) void k_fused_impl_batch(
     BatchMap bm, int K, int Kfull, int launch)
 {
-    const uint32_t w = __builtin_amdgcn_readfirstlane(
-        (uint32_t)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE));
-    if (w >= bm.total) return;
-    const int i = __builtin_amdgcn_readfirstlane((int)bm.nof[w]);
-    const BatchNet &b = bm.nets[i];
+    const uint32_t q = blockIdx.x;
+    if (q >= bm.nquads) return;
+    const uint2 e = bm.quads[q];
+    const BatchNet &b = bm.nets[e.x];
     const uint64_t t0 = b.t0 + (uint64_t)launch * (uint64_t)Kfull;
     const int sel = b.sel ^ (launch & 1);
-    fused_impl_body<NK, SK, KMAX>(b.n, (int)(w - bm.base[i]), t0, K, sel);
+    fused_impl_group<NK, SK, KMAX>(b.n, (int)e.y, t0, K, sel);
 }
 
 // The explicit slot state an implicit network stands for at step boundary T (fire words in

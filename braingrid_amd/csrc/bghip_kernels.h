@@ -718,12 +718,13 @@ __global__ __launch_bounds__(BG_BLOCK) void k_synapse_step(DevNet n, uint64_t t)
 // loop, so the divergent transcendental work runs once per synapse and block. With a 32-bit ring
 // the pulled spikes are one rotated copy of fm and the block's deliveries one rotated window of the
 // ring word; wider rings take them bit by bit.
-// Block length of the fused kernels: KMAX 16 and 32 take any K <= KMAX at run time (the step loops
-// are unrolled to KMAX with a guard on k < K, which the compiler turns into selects on every
-// step); any other KMAX is an instantiation for exactly KMAX steps, with no guard at all.
-#define BG_K_EXACT(KMAX) ((KMAX) != 16 && (KMAX) != 32)
+// Block length of the fused kernels: KMAX = BG_K_COMMON is an instantiation for exactly that many
+// steps, with no guard at all; any other KMAX (8, 16, 32) takes any K <= KMAX at run time (the step
+// loops are unrolled to KMAX with a guard on k < K, which the compiler turns into selects on every
+// step). The 8-step one runs the shorter last blocks of a call in the registers of the 9-step one.
 // the exact block length instantiated: the minimum reference delay at dt = 0.1 ms, int(0.8/0.1)+1
 #define BG_K_COMMON 9
+#define BG_K_EXACT(KMAX) ((KMAX) == BG_K_COMMON)
 
 // Occupancy target of the fused kernels (waves per SIMD): bounds their registers.
 #ifndef BG_FUSED_WAVES

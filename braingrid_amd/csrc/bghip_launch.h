@@ -37,6 +37,8 @@ struct BatchMap {
     uint32_t nchunks;
     const uint2 *nwaves;    // thread-per-neuron kernels: {member, first neuron} of each wavefront
     uint32_t nnwaves;
+    const uint2 *quads;     // implicit-ring fused kernels: {member, first position} of each workgroup
+    uint32_t nquads;        // (four destinations of one member per workgroup)
 };
 // Launch shape of a batch group's kernels. Every work item computes the same values whatever
 // block it lands in and however many destinations share its wavefront, so results do not depend

@@ -11,6 +11,11 @@ Families:
                 distributions (configfiles/test-small.xml), DynamicLayout with fraction fE and 10%
                 starters, target rate tR, 1 s epochs, each followed by the growth update and the
                 growth recorder's history step;
+                With `r0` in the specification the network starts pre-grown: every radius is set to
+                r0 and one growth update creates the synapses of overlapping neighbours before the
+                first step (r0 = 2.8 gives about 100 synapses per interior neuron), so the network
+                runs the fused Spiking/DS kernels from its first step; such networks fire near
+                their refractory limit and keep 1000 spikes per neuron and epoch of history;
   izh_stdp   -- the static Izhikevich + DynamicSTDP network of braingrid_amd.networks with
                 min(100, N - 1) outgoing synapses per neuron and a 10 Hz Poisson input of weight 25
                 (configfiles/SInputPoisson/i_10hz_w25.xml) into every neuron; 1 s epochs, each
@@ -34,6 +39,7 @@ EPOCH_S = 1.0
 DT = 1e-4
 LIF_M = 200
 LIF_MAX_RATE = 200
+LIF_PREGROWN_MAX_RATE = 1000
 IZH_MAX_RATE = 400
 POISSON_HZ, POISSON_WEIGHT = 10.0, 25.0
 # batch launch-shape autotuner: timed steps per candidate shape (bghip_batch_autotune)
@@ -46,8 +52,9 @@ def net_seed(global_seed: int, net_id: int) -> int:
 
 
 def sweep_specs(n_nets: int = 256, seed: int = 2024, side_lo: int = 10, side_hi: int = 120, sim_s: float = 5.0,
-                frac_lif: float = 0.5):
-    """Sides log-uniform on [side_lo, side_hi]; families alternate by a seeded draw."""
+                frac_lif: float = 0.5, lif_r0: float | None = None):
+    """Sides log-uniform on [side_lo, side_hi]; families alternate by a seeded draw. With lif_r0
+    the growth networks start pre-grown to that radius."""
     rng = np.random.default_rng(seed)
     specs = []
     for k in range(n_nets):
@@ -55,7 +62,13 @@ def sweep_specs(n_nets: int = 256, seed: int = 2024, side_lo: int = 10, side_hi:
         kind = "lif_growth" if rng.random() < frac_lif else "izh_stdp"
         specs.append(dict(id=k, side=side, kind=kind, tR=float(rng.choice(TR_CHOICES)),
                           fE=float(rng.choice(FE_CHOICES)), seed=net_seed(seed, k), sim_s=float(sim_s)))
+        if lif_r0 and kind == "lif_growth":
+            specs[-1]["r0"] = float(lif_r0)
     return specs
+
+
+def lif_max_rate(spec: dict) -> int:
+    return LIF_PREGROWN_MAX_RATE if spec.get("r0") else LIF_MAX_RATE
 
 
 MAX_SIDE = 512   # N = side^2 stays under the engine's 262,144-neuron limit
@@ -66,7 +79,7 @@ def lif_config(spec: dict) -> dict:
     c.update(width=spec["side"], height=spec["side"], layout="DynamicLayout", frac_exc=spec["fE"], frac_starter=0.1,
              target_rate=spec["tR"], seed=spec["seed"], epoch_duration=EPOCH_S,
              num_epochs=int(round(spec["sim_s"] / EPOCH_S)), max_synapses_per_neuron=LIF_M,
-             max_firing_rate=LIF_MAX_RATE)
+             max_firing_rate=lif_max_rate(spec))
     return c
 
 
@@ -87,7 +100,9 @@ def plan(spec: dict) -> dict:
     if kind == "lif_growth":
         if not (spec.get("tR", 0) > 0 and 0 < spec.get("fE", 0) <= 1):
             raise ValueError("lif_growth needs tR > 0 and 0 < fE <= 1")
-        max_spikes = int(np.float32(EPOCH_S) * np.float32(LIF_MAX_RATE))
+        if spec.get("r0") is not None and not 0 < spec["r0"] <= 4.0:
+            raise ValueError(f"pre-grown radius {spec['r0']!r} outside (0, 4] (the rows must fit {LIF_M} slots)")
+        max_spikes = int(np.float32(EPOCH_S) * np.float32(lif_max_rate(spec)))
         desc = MD.net_desc(N, LIF_M, B.NEURON_LIF, B.SYN_DS, max_spikes, DT, spec["seed"])
         return dict(kind=kind, desc=desc, arrays=None)
     arrays = NW.izh_stdp_draws(N, spec["seed"], k=min(100, N - 1), max_rate=IZH_MAX_RATE, epoch_s=EPOCH_S)
@@ -112,7 +127,10 @@ class SweepNet:
         if self.kind == "lif_growth":
             self.model = host.Model(ctx, lif_config(spec))
             self.net = self.model.net
-            self.max_spikes = int(np.float32(EPOCH_S) * np.float32(LIF_MAX_RATE))
+            self.max_spikes = int(np.float32(EPOCH_S) * np.float32(lif_max_rate(spec)))
+            if spec.get("r0"):
+                self.model.set_radii(spec["r0"])
+                self.model.update_connections(stats=False)
         else:
             arrays = planned["arrays"] if planned else None
             self.net, meta = NW.izh_stdp(ctx, self.N, spec["seed"], k=min(100, self.N - 1), max_rate=IZH_MAX_RATE,

@@ -317,7 +317,8 @@ typedef struct bghip_batch_shape {
     int32_t mode;                  /* out: 0 neurons only (no synapses), 1 fused blocks, 2 per step */
     int32_t neuron_model, synapse_model, ring_words;   /* out: the group's kind */
     uint32_t networks, work_items; /* out: members, and neurons (= destinations) over all members */
-    int32_t block;                 /* threads per block of the neuron, synapse and fused kernels: 64, 128, 256 */
+    int32_t block;                 /* threads per block of the neuron, synapse and fused kernels: 64, 128, 256
+                                      (implicit-ring fused groups: always 256, four destinations each) */
     int32_t dests_per_wave;        /* per-step STDP streaming pass: destinations per wavefront, 1, 2 or 4 */
     int32_t block_stream;          /* threads per block of that pass: 64, 128, 256 */
     int32_t syn_blocks_per_shard;  /* blocks per event-list shard of the learning pass, 1..64 */

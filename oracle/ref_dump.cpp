@@ -12,10 +12,17 @@
 //
 // usage: bgref_dump <config.xml> <dump.bin> [--max-epochs K] [--no-history]
 //                   [--dynamic-layout FRAC_EXC FRAC_STARTER] [--stim STIMULUS.xml]
+//                   [--time-pregrown RADIUS STEPS]
 // --dynamic-layout replaces the configured layout object by a DynamicLayout with the given
 // fractions before setup: the reference's XML visitor for DynamicLayout rejects its own
 // parameters (Layouts/DynamicLayout.cpp:29-77 returns false after parsing them), so this is the
 // only way to run Layouts/DynamicLayout.cpp:98-165.
+// --time-pregrown sets every neurite radius to RADIUS after setup, runs the reference's own growth
+// update once (ConnGrowth::updateConns/updateFrontiers/updateOverlap/updateSynapsesWeights and the
+// synapse index map), then times STEPS calls of IModel::advance (the reference's CPU step) and
+// prints {"steps", "seconds", "synapses"} as one JSON line instead of writing a dump: the CPU
+// baseline of the pre-grown benchmark network, measured on the reference's own code.
+#include <ctime>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -191,6 +198,8 @@ int main(int argc, char **argv)
     bool dyn = false;
     float fexc = 0, fst = 0;
     const char *stim = NULL;
+    float time_radius = 0;
+    int time_steps = 0;
     for (int i = 3; i < argc; i++) {
         if (!strcmp(argv[i], "--max-epochs") && i + 1 < argc) max_epochs = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--no-history")) with_history = false;
@@ -200,6 +209,9 @@ int main(int argc, char **argv)
             fst = (float)atof(argv[++i]);
         } else if (!strcmp(argv[i], "--stim") && i + 1 < argc) {
             stim = argv[++i];
+        } else if (!strcmp(argv[i], "--time-pregrown") && i + 2 < argc) {
+            time_radius = (float)atof(argv[++i]);
+            time_steps = atoi(argv[++i]);
         }
     }
     SimulationInfo *si = new SimulationInfo();
@@ -226,6 +238,26 @@ int main(int argc, char **argv)
     sim.setup(si);
 
     int N = si->totalNeurons;
+    if (time_steps > 0) {
+        ConnGrowth *g = dynamic_cast<ConnGrowth *>(model->m_conns);
+        if (!g) return 1;
+        for (int i = 0; i < N; i++) (*g->radii)[i] = time_radius;
+        si->currentStep = 1;
+        si->model->updateConnections(si);
+        AllSynapses *syn = dynamic_cast<AllSynapses *>(model->m_synapses);
+        struct timespec a, b;
+        clock_gettime(CLOCK_MONOTONIC, &a);
+        for (int k = 0; k < time_steps; k++) {
+            si->model->advance(si);
+            g_simulationStep++;
+        }
+        clock_gettime(CLOCK_MONOTONIC, &b);
+        printf("{\"steps\": %d, \"seconds\": %.6f, \"synapses\": %llu}\n", time_steps,
+               (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec),
+               (unsigned long long)syn->total_synapse_counts);
+        fclose(g_out);
+        return 0;
+    }
     Layout *L = model->m_layout;
     {
         std::vector<float> x(N), y(N);
