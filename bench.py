@@ -240,7 +240,7 @@ def kernel_source_sha() -> str:
 def load_traffic():
     """The counter-measured HBM bytes per launch of the headline kernel, if the profile was taken
     on the kernel sources of this tree (otherwise None: a stale figure is not reported)."""
-    p = os.path.join(ROOT, "profiles", "fused_block_10k_traffic.json")
+    p = os.path.join(ROOT, "profiles", "fused_impl_10k_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -763,7 +763,7 @@ def main():
                      "algorithmic_bytes_per_launch": launch_bytes,
                      "traffic_GBps": (traffic["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9) if traffic and avg_ms > 0 else None,
                      "traffic_frac": (traffic["bytes_per_launch"] / (avg_ms * 1e-3) / HBM_PEAK) if traffic and avg_ms > 0 else None,
-                     "traffic_source": ("profiles/fused_block_10k_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                     "traffic_source": ("profiles/fused_impl_10k_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
                                         "copy-calibrated, same kernel sources)") if traffic else
                                        "no counter profile of the current kernel sources"},
         "device_error_bits": int(err),
