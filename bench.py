@@ -698,8 +698,9 @@ def main():
             model.update_connections()
             model.update_history()
             growth_updates += 1
-    model.net.sync()
+    model.net.wait()   # the stream drained; its error word is read and checked just below, untimed
     wall = time.perf_counter() - t0
+    model.net.sync()   # raises if the timed steps set a device error bit
     err = model.net.error_bits()
     neu = model.net.download_neurons(int(EPOCH * MAX_RATE))
     in_epoch = done % steps_per_epoch or steps_per_epoch

@@ -146,6 +146,7 @@ def load():
         "bghip_clear_spike_counts": (ctypes.c_int, [vp]),
         "bghip_update_history": (ctypes.c_int, [vp]),
         "bghip_sync": (ctypes.c_int, [vp]),
+        "bghip_wait": (ctypes.c_int, [vp]),
         "bghip_net_error_bits": (ctypes.c_uint32, [vp]),
         "bghip_net_step": (ctypes.c_uint64, [vp]),
         "bghip_download_neurons": (ctypes.c_int, [vp, ctypes.POINTER(Neurons)]),
@@ -190,7 +191,7 @@ def load():
 EXPORTED = ("bghip_ctx_create", "bghip_ctx_destroy", "bghip_last_error", "bghip_ctx_mem_info", "bghip_net_create",
             "bghip_net_destroy", "bghip_net_footprint", "bghip_upload_neurons", "bghip_upload_synapses",
             "bghip_upload_layout", "bghip_upload_growth", "bghip_advance", "bghip_update_connections",
-            "bghip_update_history", "bghip_sync", "bghip_net_error_bits", "bghip_net_step",
+            "bghip_update_history", "bghip_sync", "bghip_wait", "bghip_net_error_bits", "bghip_net_step",
             "bghip_download_neurons", "bghip_download_synapses", "bghip_total_synapses",
             "bghip_download_index_map", "bghip_download_growth", "bghip_kernel_timing", "bghip_batch_create",
             "bghip_batch_destroy", "bghip_batch_set", "bghip_batch_advance", "bghip_batch_sync",
@@ -386,6 +387,10 @@ class Net:
 
     def sync(self):
         self._c(load().bghip_sync(self.h), "sync")
+
+    def wait(self):
+        """Waits for the network's queued work; device errors are left for sync / error_bits."""
+        self._c(load().bghip_wait(self.h), "wait")
 
     def error_bits(self) -> int:
         return load().bghip_net_error_bits(self.h)

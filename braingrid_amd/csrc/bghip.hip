@@ -919,6 +919,15 @@ static int read_error_bits(bghip_net *n, uint32_t *e)
     return BGHIP_OK;
 }
 
+extern "C" int bghip_wait(bghip_net *n)
+{
+    if (!n) return BGHIP_EINVAL;
+    net_begin(n);
+    HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
+    HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    return BGHIP_OK;
+}
+
 extern "C" uint32_t bghip_net_error_bits(bghip_net *n)
 {
     uint32_t e = 0;

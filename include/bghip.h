@@ -258,6 +258,9 @@ int bghip_recorder_download(bghip_net *net, float *burstinessHist, float *spikes
                             float *radiiHistory);
 /* waits for the network's stream; returns BGHIP_ESTATE if a device error bit is set */
 int bghip_sync(bghip_net *net);
+/* waits for the network's stream only: the error word stays on the device for the next bghip_sync,
+ * bghip_net_error_bits or download (for callers that check errors later, off their timed path) */
+int bghip_wait(bghip_net *net);
 uint32_t bghip_net_error_bits(bghip_net *net);
 uint64_t bghip_net_step(const bghip_net *net);
 /* 1 when the network's Spiking/DS synapses advance with implicit delay rings (their rings read from
