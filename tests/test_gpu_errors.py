@@ -43,9 +43,11 @@ def test_spike_history_overflow_is_reported_at_the_oracles_step(hip_ctx):
         bad = H.compare_state(ora, net, cfg, "before-overflow", check_maps=False)
         assert not bad, bad[:10]
         net.advance(1)
+        net.wait()   # only waits: the error word is left for sync and error_bits
         with pytest.raises(B.BgHipError, match="device error bits 0x2"):
             net.sync()
         assert net.error_bits() == 2
+        net.wait()
     finally:
         net.close()
         ora.close()
