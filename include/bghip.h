@@ -256,10 +256,13 @@ int bghip_recorder_enable(bghip_net *net, int64_t n_burst, int64_t n_spikes_hist
 /* Copies the recorder's histories to the host (NULL pointers are skipped); blocking. */
 int bghip_recorder_download(bghip_net *net, float *burstinessHist, float *spikesHistory, float *ratesHistory,
                             float *radiiHistory);
-/* waits for the network's stream; returns BGHIP_ESTATE if a device error bit is set */
+/* Waits for the network's stream, as the reference's blocking cudaMemcpy calls wait for its kernels
+ * (e.g. Neurons/AllSpikingNeurons_d.cu:41-42, checked by HANDLE_ERROR), and reports what its device
+ * asserts would have stopped on: BGHIP_ESTATE if a device error bit is set. */
 int bghip_sync(bghip_net *net);
-/* waits for the network's stream only: the error word stays on the device for the next bghip_sync,
- * bghip_net_error_bits or download (for callers that check errors later, off their timed path) */
+/* The same wait without the error report: the error word stays on the device for the next
+ * bghip_sync, bghip_net_error_bits or download (for callers that check errors later, off their
+ * timed path). No counterpart in the reference, whose waits always come with a copy. */
 int bghip_wait(bghip_net *net);
 uint32_t bghip_net_error_bits(bghip_net *net);
 uint64_t bghip_net_step(const bghip_net *net);
