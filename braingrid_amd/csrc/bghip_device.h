@@ -407,6 +407,187 @@ __device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, int typ
     return psr;
 }
 
+// One event synapse of the per-step STDP path, whole: the ring test and clear, the learning walks
+// of stdp_event, changePSR (AllDSSynapses::changePSR for DynamicSTDP, W / decay for STDP) and the
+// decay and store of the PSR (Synapses/AllSTDPSynapses.cpp:289-421), with stdp_event's and
+// stdp_learning's arithmetic, in the same order. The event pass is bound by the latency of one such
+// chain per thread, so the body requests everything it reads before its first store: the slot's
+// state and parameters, the ring word, both neurons' history cursors and then the four newest
+// entries of both histories (the walks rarely go further; deeper entries are read when reached). W,
+// r, u and lastSpike are carried in registers through the walks and stored once at the end.
+// stdp_event reads each field at first use and re-reads W after every store, a chain several
+// times longer.
+template <int SK>
+__device__ __noinline__ void stdp_event_slot(const DevNet &n, int d, uint64_t slot, uint64_t t)
+{
+    // ---- reads: slot and destination first, then what depends on meta, then history entries
+    const int ms = n.max_spikes;
+    const float dt = n.dt;
+    const uint32_t tb = (uint32_t)(t & (uint64_t)(n.R - 1));
+    uint32_t *const wp = &n.ring[(uint64_t)(tb >> 5) * n.S + slot];
+    const uint32_t m = 1u << (tb & 31);
+    float *const Wp = &n.W[slot];
+    float *const psrp = &n.psr[slot];
+    const uint64_t *const hist = n.hist;
+    const uint32_t word = *wp;
+    const uint64_t fs_post = n.fired_step[d];
+    const uint32_t meta = n.meta[slot];
+    float psr = *psrp;
+    float W = *Wp;
+    const float tauspre = n.tauspre[slot], tauspost = n.tauspost[slot], taupos = n.taupos[slot];
+    const float tauneg = n.tauneg[slot];
+    const bool useFD = n.useFD[slot] != 0;
+    const float gap = n.gap[slot], muneg = n.muneg[slot], mupos = n.mupos[slot], Aneg = n.Aneg[slot];
+    const float Apos = n.Apos[slot], Wex = n.Wex[slot];
+    const int base_post = n.cnt[d] + n.off[d] + ms;
+    float r = 0.0f, u = 0.0f, Us = 0.0f, Ds = 0.0f, Fs = 0.0f;
+    uint64_t last = BG_NO_SPIKE;
+    const bool typed = n.ds_typed != 0;
+    float *const rp = SK == SK_DYNSTDP ? &n.r[slot] : nullptr;
+    float *const up = SK == SK_DYNSTDP ? &n.us[slot] : nullptr;
+    uint64_t *const lastp = SK == SK_DYNSTDP ? &n.last[slot] : nullptr;
+    if (SK == SK_DYNSTDP) {
+        r = *rp;
+        u = *up;
+        last = *lastp;
+        if (!typed) {
+            Ds = n.Dd[slot];
+            Fs = n.F[slot];
+            Us = n.U[slot];
+        }
+    }
+    const int src = (int)(meta & BG_META_SRC_MASK);
+    const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
+    const int total_delay = (int)((meta >> BG_META_DELAY_SHIFT) & 0xFFu);
+    const float decay = n.decay[type];
+    if (SK == SK_DYNSTDP && typed) {
+        Ds = n.tD[type];
+        Fs = n.tF[type];
+        Us = n.tU[type];
+    }
+    const int base_pre = n.cnt[src] + n.off[src] + ms;
+    // entries o = -1 .. -4 of both histories; an entry is at (cnt + off + ms + o) mod ms, and
+    // cnt + off + ms - 1 >= 0 (count and offset are not negative)
+    const int kpost = (base_post - 1) % ms, kpre = (base_pre - 1) % ms;
+    auto at = [&](int k1, int j) {
+        int k = k1 - j;
+        while (k < 0) k += ms;
+        return k;
+    };
+    const uint64_t *const hpo = hist + (uint64_t)d * ms, *const hpr = hist + (uint64_t)src * ms;
+    const uint64_t po1 = hpo[kpost], po2 = hpo[at(kpost, 1)], po3 = hpo[at(kpost, 2)], po4 = hpo[at(kpost, 3)];
+    const uint64_t pr1 = hpr[kpre], pr2 = hpr[at(kpre, 1)], pr3 = hpr[at(kpre, 2)], pr4 = hpr[at(kpre, 3)];
+    // getSpikeHistory(idx, o): a prefetched entry where the reference's index agrees with it, else
+    // the reference's own index expression
+    auto hget = [&](bool post_side, int o) -> uint64_t {
+        const int base = post_side ? base_post : base_pre;
+        if (o >= -4 && base + o >= 0) {
+            const uint64_t a = post_side ? po1 : pr1, b = post_side ? po2 : pr2;
+            const uint64_t c = post_side ? po3 : pr3, e = post_side ? po4 : pr4;
+            return o == -1 ? a : (o == -2 ? b : (o == -3 ? c : e));
+        }
+        const int idx = post_side ? d : src;
+        return hist[(uint64_t)idx * ms + (base + o) % ms];
+    };
+    // AllSTDPSynapses::stdpLearning on the W in registers
+    bool learned = false;
+    auto learn = [&](double delta, double epost, double epre) {
+        float dw;
+        if (delta < -(double)gap) {
+            dw = (float)((double)(bg_powf(W, muneg) * Aneg) * bg_exp(delta / (double)tauneg));
+        } else if (delta > (double)gap) {
+            dw = (float)((double)(bg_powf(Wex - W, mupos) * Apos) * bg_exp(-delta / (double)taupos));
+        } else {
+            return;
+        }
+        float Wn = (float)((double)W + epost * epre * (double)dw);
+        if ((Wex < 0 && Wn > 0) || (Wex > 0 && Wn < 0)) Wn = 0;
+        if (fabsf(Wn) > fabsf(Wex)) Wn = Wex;
+        W = Wn;
+        learned = true;
+    };
+
+    // ---- the event
+    const bool pre = (word & m) != 0;
+    const bool post = fs_post == t;
+    bool ru_changed = false;
+    uint64_t sh;
+    float delta, epre, epost;
+    if (pre) {
+        sh = hget(false, -2);
+        if (sh != BG_NO_SPIKE && useFD) {
+            delta = (float)(uint64_t)(t - sh) * dt;
+            epre = (float)(1.0 - (double)bg_expf(-delta / tauspre));
+        } else {
+            epre = 1.0f;
+        }
+        for (int o = -1; o >= -ms; --o) {
+            sh = hget(true, o);
+            if (sh == BG_NO_SPIKE) break;
+            delta = (float)(uint64_t)(sh - t) * dt;
+            if ((double)delta <= -3.0 * (double)tauneg) break;
+            if (useFD) {
+                const uint64_t sh2 = hget(true, o - 1);
+                if (sh2 == BG_NO_SPIKE) break;
+                epost = (float)(1.0 - (double)bg_expf(-((float)(uint64_t)(sh - sh2) * dt) / tauspost));
+            } else {
+                epost = 1.0f;
+            }
+            const uint32_t w0 = __float_as_uint(W);
+            learn((double)delta, (double)epost, (double)epre);
+            // an older entry that left W bit-identical: so do all older ones (see stdp_event)
+            if (sh < t && t < (1ull << 39) && __float_as_uint(W) == w0 && W == W) break;
+        }
+        if (SK == SK_DYNSTDP) {
+            if (last != BG_NO_SPIKE) {
+                const float isi = (float)(t - last) * dt;
+                r = 1.0f + (r * (1.0f - u) - 1.0f) * bg_expf(-isi / Ds);
+                u = Us + u * (1.0f - Us) * bg_expf(-isi / Fs);
+                ru_changed = true;
+            }
+            last = t;
+            psr = psr + ((W / decay) * u) * r;
+        } else {
+            psr = psr + W / decay;
+        }
+    }
+    if (post) {
+        sh = hget(true, -2);
+        if (sh != BG_NO_SPIKE && useFD) {
+            delta = (float)(uint64_t)(t - sh) * dt;
+            epost = (float)(1.0 - (double)bg_expf(-delta / tauspost));
+        } else {
+            epost = 1.0f;
+        }
+        for (int o = -1; o >= -ms; --o) {
+            sh = hget(false, o);
+            if (sh == BG_NO_SPIKE) break;
+            delta = (float)(uint64_t)(t - sh - (uint64_t)(int64_t)total_delay) * dt;
+            if (delta <= 0 || (double)delta >= 3.0 * (double)taupos) break;
+            if (useFD) {
+                const uint64_t sh2 = hget(false, o - 1);
+                if (sh2 == BG_NO_SPIKE) break;
+                epre = (float)(1.0 - (double)bg_expf(-((float)(uint64_t)(sh - sh2) * dt) / tauspre));
+            } else {
+                epre = 1.0f;
+            }
+            learn((double)delta, (double)epost, (double)epre);
+        }
+    }
+
+    // ---- stores
+    if (pre) *wp = word & ~m;
+    if (learned) *Wp = W;
+    if (SK == SK_DYNSTDP && pre) {
+        if (ru_changed) {
+            *rp = r;
+            *up = u;
+        }
+        *lastp = t;
+    }
+    *psrp = psr * decay;
+}
+
 // Wavefront sum in the engine's canonical order: lane l holds the lane-strided sequential partial
 // sum; butterfly over 32, 16, ..., 1 leaves the same value in every lane (for lane 0 this is the
 // tree acc[l] += acc[l + off] of the oracle's wave64 mode).
