@@ -219,21 +219,21 @@ struct Carve {
     }
 };
 
-// Segment offsets of the sharded event lists for destinations (work items) [base_i, base_i + N_i)
-// with M_i slots each: shard s holds the work items w = s (mod BG_EV_SHARDS). Returns false when
-// the synapse list would not be addressable with 32-bit offsets.
-// Segment sizes of the sharded event lists. The streaming pass works on runs ("chunks") of
-// BG_STREAM_DESTS destinations and lists a chunk's events in shard (chunk index mod BG_EV_SHARDS);
-// a part is one network whose chunks are numbered from cbase. Each shard gets room for the worst
-// case: every destination of its chunks listed, with all M of its slots.
-struct EvPart { uint64_t cbase, N, M; };
-static bool ev_offsets(const std::vector<EvPart> &parts, std::vector<uint32_t> &dest_off, std::vector<uint32_t> &syn_off)
+// Segment sizes of the sharded event lists (NS shards). The streaming pass works on runs ("chunks")
+// of BG_STREAM_DESTS destinations; a part is one network, whose chunk j lists its events in shard
+// (sbase + j mod m) mod NS. Each shard gets room for the worst case: every destination of its
+// chunks listed, with all M of its slots. Returns false when the synapse list would not be
+// addressable with 32-bit offsets.
+struct EvPart { uint64_t N, M, sbase, m; };
+static uint32_t ev_shard(const EvPart &p, uint64_t chunk, uint64_t NS) { return (uint32_t)((p.sbase + chunk % p.m) % NS); }
+static bool ev_offsets(const std::vector<EvPart> &parts, uint64_t NS, std::vector<uint32_t> &dest_off,
+                       std::vector<uint32_t> &syn_off)
 {
-    const uint64_t NS = BG_EV_SHARDS, D = BG_STREAM_DESTS;
+    const uint64_t D = BG_STREAM_DESTS;
     std::vector<uint64_t> dc(NS, 0), sc(NS, 0);
     for (const EvPart &p : parts)
         for (uint64_t k = 0; k * D < p.N; k++) {
-            const uint64_t r = (p.cbase + k) % NS, nd = std::min<uint64_t>(D, p.N - k * D);
+            const uint64_t r = ev_shard(p, k, NS), nd = std::min<uint64_t>(D, p.N - k * D);
             dc[r] += nd;
             sc[r] += nd * p.M;
         }
@@ -249,6 +249,30 @@ static bool ev_offsets(const std::vector<EvPart> &parts, std::vector<uint32_t> &
     dest_off[NS] = (uint32_t)ad;
     syn_off[NS] = (uint32_t)as;
     return as < (1ull << 32);
+}
+
+// The batch's shard ranges: member k gets m_k = 1 + (NS - n) * chunks_k / chunks shards from
+// sbase_k on (so sum m_k <= NS, and a larger member spreads its list appends over more counters);
+// with more members than shards every member's chunks go round all NS shards, continuing where the
+// previous member's ended.
+static void ev_batch_parts(std::vector<EvPart> &parts, uint64_t NS)
+{
+    const uint64_t D = BG_STREAM_DESTS, n = parts.size();
+    uint64_t chunks = 0;
+    for (const EvPart &p : parts) chunks += (p.N + D - 1) / D;
+    uint64_t base = 0;
+    for (EvPart &p : parts) {
+        const uint64_t ck = (p.N + D - 1) / D;
+        if (n <= NS) {
+            p.m = std::max<uint64_t>(1, std::min<uint64_t>(ck, 1 + (NS - n) * ck / std::max<uint64_t>(chunks, 1)));
+            p.sbase = base;
+            base += p.m;
+        } else {
+            p.m = NS;
+            p.sbase = base % NS;
+            base += ck;
+        }
+    }
 }
 
 // Networks whose synapses can run with implicit delay rings: Spiking or DS synapses (STDP needs
@@ -403,7 +427,7 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
     HIP_TRY(ctx, hipMemsetAsync(dn.hist, 0xFF, sizeof(uint64_t) * (size_t)dn.N * dn.max_spikes, n->stream));
     std::vector<uint32_t> doff, soff;
     if (n->d_ev_off) {
-        if (!ev_offsets({{0, (uint64_t)dn.N, (uint64_t)dn.M}}, doff, soff))
+        if (!ev_offsets({{(uint64_t)dn.N, (uint64_t)dn.M, 0, BG_EV_SHARDS}}, BG_EV_SHARDS, doff, soff))
             return set_err(ctx, BGHIP_EUNSUPPORTED, "network too large for the event lists");
         doff.insert(doff.end(), soff.begin(), soff.end());
         HIP_TRY(ctx, hipMemcpyAsync(n->d_ev_off, doff.data(), sizeof(uint32_t) * doff.size(), hipMemcpyHostToDevice,
@@ -1814,10 +1838,12 @@ extern "C" void bghip_batch_destroy(bghip_batch *b)
 }
 
 // The untuned launch shape: 256-thread blocks, four destinations per streaming wavefront, event
-// grids growing with the batch (up to 4 and 8 blocks per shard).
+// grids growing with the batch (up to 4 and 8 blocks per shard, one per 64 destinations a shard
+// holds on average).
 static BatchShape default_shape(uint32_t total)
 {
-    const int g = (int)std::max<uint64_t>(1, ((uint64_t)total + 16 * BG_EV_SHARDS - 1) / (16 * BG_EV_SHARDS));
+    const uint64_t per = 64 * BG_EV_SHARDS_BATCH;
+    const int g = (int)std::max<uint64_t>(1, ((uint64_t)total + per - 1) / per);
     return BatchShape{BG_BLOCK, BG_STREAM_DESTS, BG_BLOCK, std::min(4, g), std::min(8, g)};
 }
 
@@ -1921,35 +1947,35 @@ static int regroup(bghip_batch *b)
         HIP_TRY(b->ctx, hipHostMalloc((void **)&g.h_sel, sizeof(int) * nn, hipHostMallocDefault));
         HIP_TRY(b->ctx, hipEventCreateWithFlags(&g.staged, hipEventDisableTiming));
         if (g.mode == BG_MODE_STEP && (g.sk == BGHIP_SYN_STDP || g.sk == BGHIP_SYN_DYNSTDP)) {
+            const uint64_t NS = BG_EV_SHARDS_BATCH;
             std::vector<EvPart> parts(nn);
-            uint64_t cbase = 0;
             for (int k = 0; k < nn; k++) {
                 const DevNet &dn = b->nets[g.members[k]]->dn;
-                parts[k] = {cbase, (uint64_t)dn.N, (uint64_t)dn.M};
-                cbase += ((uint64_t)dn.N + BG_STREAM_DESTS - 1) / BG_STREAM_DESTS;
+                parts[k] = {(uint64_t)dn.N, (uint64_t)dn.M, 0, 1};
             }
+            ev_batch_parts(parts, NS);
             std::vector<uint32_t> doff, soff;
-            if (!ev_offsets(parts, doff, soff))
+            if (!ev_offsets(parts, NS, doff, soff))
                 return set_err(b->ctx, BGHIP_EUNSUPPORTED, "batch too large for the event lists");
-            const size_t nc = 2 * BG_EV_SHARDS * BG_EV_PAD, no = 2 * (BG_EV_SHARDS + 1);
+            const size_t nc = 2 * NS * BG_EV_PAD, no = 2 * (NS + 1);
             HIP_TRY(b->ctx, hipMalloc((void **)&g.d_ev, sizeof(uint32_t) * (nc + no + (size_t)g.total + 1)));
-            HIP_TRY(b->ctx, hipMalloc((void **)&g.d_evs, sizeof(uint2) * (size_t)std::max<uint32_t>(soff[BG_EV_SHARDS], 1)));
+            HIP_TRY(b->ctx, hipMalloc((void **)&g.d_evs, sizeof(uint2) * (size_t)std::max<uint32_t>(soff[NS], 1)));
             doff.insert(doff.end(), soff.begin(), soff.end());
             HIP_TRY(b->ctx, hipMemset(g.d_ev, 0, sizeof(uint32_t) * nc));
             HIP_TRY(b->ctx, hipMemcpy(g.d_ev + nc, doff.data(), sizeof(uint32_t) * no, hipMemcpyHostToDevice));
             g.ev.count = g.d_ev;
             g.ev.dest_off = g.d_ev + nc;
-            g.ev.syn_off = g.d_ev + nc + BG_EV_SHARDS + 1;
+            g.ev.syn_off = g.d_ev + nc + NS + 1;
             g.ev.dest = g.d_ev + nc + no;
             g.ev.syn = g.d_evs;
             // runs of D = 1, 2, 4 destinations; a run's shard is that of its BG_STREAM_DESTS-run
-            // (parts[k].cbase + d / BG_STREAM_DESTS), the assignment the capacities were sized for
+            // (ev_shard of chunk d / BG_STREAM_DESTS), the assignment the capacities were sized for
             for (int v = 0; v < 3; v++) {
                 const uint32_t D = 1u << v;
                 std::vector<uint2> chunks;
                 for (int k = 0; k < nn; k++)
                     for (uint32_t d = 0; d < base[k + 1] - base[k]; d += D) {
-                        const uint32_t shard = (uint32_t)((parts[k].cbase + d / BG_STREAM_DESTS) % BG_EV_SHARDS);
+                        const uint32_t shard = ev_shard(parts[k], d / BG_STREAM_DESTS, NS);
                         chunks.push_back(make_uint2((uint32_t)k | (shard << 16), d));
                     }
                 g.nchunks[v] = (uint32_t)chunks.size();

@@ -26,8 +26,9 @@ static void stdp(const BatchMap &bm, hipStream_t s, const BatchShape &sh, uint64
     case 2: stream<SK, W32, 2>(bm, s, sh.blk_stream, step, ev); break;
     default: stream<SK, W32, BG_STREAM_DESTS>(bm, s, sh.blk_stream, step, ev); break;
     }
-    hipLaunchKernelGGL(k_stdp_event_syn_batch<SK>, dim3(BG_EV_SHARDS * sh.g_syn), dim3(BG_BLOCK), 0, s, bm, step, ev);
-    hipLaunchKernelGGL(k_stdp_event_sum_batch, dim3(BG_EV_SHARDS * sh.g_sum), dim3(BG_BLOCK), 0, s, bm, ev);
+    hipLaunchKernelGGL(k_stdp_event_syn_batch<SK>, dim3(BG_EV_SHARDS_BATCH * sh.g_syn), dim3(BG_BLOCK), 0, s, bm,
+                       step, ev);
+    hipLaunchKernelGGL(k_stdp_event_sum_batch, dim3(BG_EV_SHARDS_BATCH * sh.g_sum), dim3(BG_BLOCK), 0, s, bm, ev);
 }
 template <int SK>
 static void stdp_w(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int w32, uint64_t step, const EvLists &ev)
@@ -43,8 +44,9 @@ static void stdp_w(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int 
 void launch_stdp_step_batch(const BatchMap &bm, hipStream_t s, const BatchShape &sh, int nk, int sk, int w32,
                             uint64_t step, const EvLists &ev)
 {
-    // the neuron pass clears the 2 * BG_EV_SHARDS event counters, one per thread
-    const unsigned nb = std::max(wave_blocks(bm.nnwaves, sh.blk), (unsigned)((2 * BG_EV_SHARDS + sh.blk - 1) / sh.blk));
+    // the neuron pass clears the 2 * BG_EV_SHARDS_BATCH event counters, one per thread
+    const unsigned nb =
+        std::max(wave_blocks(bm.nnwaves, sh.blk), (unsigned)((2 * BG_EV_SHARDS_BATCH + sh.blk - 1) / sh.blk));
     if (nk == NK_LIF)
         hipLaunchKernelGGL(k_stdp_neurons_batch<NK_LIF>, dim3(nb), dim3(sh.blk), 0, s, bm, step, ev);
     else

@@ -28,11 +28,16 @@
 enum { NK_LIF = 0, NK_IZH = 1 };
 enum { SK_SPIKING = 0, SK_DS = 1, SK_STDP = 2, SK_DYNSTDP = 3 };
 
-// Lists of the spike events of one per-step STDP step. Destination (work item) w belongs to shard
-// w mod BG_EV_SHARDS; each shard has its own counters, on separate 64-byte lines, and its own
-// segments of the two lists, sized for the worst case (every synapse of every destination of the
-// shard), so appends never overflow and concurrent waves rarely meet on one counter.
+// Lists of the spike events of one per-step STDP step, in shards: each shard has its own
+// counters, on separate 64-byte lines, and its own segments of the two lists, sized for the worst
+// case (every synapse of every destination of the shard), so appends never overflow and concurrent
+// waves rarely meet on one counter. A network's runs of BG_STREAM_DESTS destinations go to its
+// BG_EV_SHARDS shards in turn. A batch has BG_EV_SHARDS_BATCH shards and gives each member a range
+// of its own, wide in proportion to the member's destinations: a shard then lists one network's
+// events, so the lanes of an event-pass wavefront share one network descriptor (with more members
+// than shards the members' runs go round all shards in turn).
 #define BG_EV_SHARDS 128
+#define BG_EV_SHARDS_BATCH 1024
 #define BG_EV_PAD 16
 struct EvLists {
     uint32_t *count;            // destinations of shard s at [s * PAD], synapses at [(SHARDS + s) * PAD]

@@ -251,7 +251,7 @@ __device__ __forceinline__ void stdp_stream_classify(const DevNet &n, const floa
 // listed with one reservation per list. Each destination keeps its own lane-strided accumulation
 // and wave tree, so its sum is the same as with one destination per wave; rows longer than 128
 // ranks continue in a loop of their own.
-template <int SK, int W32, int D>
+template <int SK, int W32, int D, int NS>
 __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int nd, int lane, uint64_t t, uint32_t g0,
                                                   uint32_t shard, const EvLists &ev)
 {
@@ -343,7 +343,7 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
                     more = true;
                     const uint32_t n1 = (uint32_t)__popcll(b1);
                     uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&ev.count[(BG_EV_SHARDS + shard) * BG_EV_PAD], n1 + (uint32_t)__popcll(b2));
+                    if (lane == 0) base = atomicAdd(&ev.count[(NS + shard) * BG_EV_PAD], n1 + (uint32_t)__popcll(b2));
                     base = __shfl(base, 0) + ev.syn_off[shard];
                     const uint64_t below = (1ull << lane) - 1ull;
                     if (e1) ev.syn[base + (uint32_t)__popcll(b1 & below)] = make_uint2(w, (uint32_t)s1);
@@ -360,7 +360,7 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
     if (event_dests) {
         uint32_t bs = 0, bd = 0;
         if (lane == 0) {
-            if (nsyn) bs = atomicAdd(&ev.count[(BG_EV_SHARDS + shard) * BG_EV_PAD], nsyn);
+            if (nsyn) bs = atomicAdd(&ev.count[(NS + shard) * BG_EV_PAD], nsyn);
             bd = atomicAdd(&ev.count[shard * BG_EV_PAD], (uint32_t)__popc(event_dests));
         }
         bs = (uint32_t)__shfl((int)bs, 0) + ev.syn_off[shard];
@@ -399,16 +399,18 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream(DevNet n, uint32_t off
         (int)((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / BG_WAVE) * BG_STREAM_DESTS);
     if (d0 >= n.N) return;
     const uint32_t chunk = (uint32_t)d0 / BG_STREAM_DESTS;
-    stdp_stream_dests<SK, W32, BG_STREAM_DESTS>(n, d0, min(BG_STREAM_DESTS, n.N - d0), lane, t, (uint32_t)d0,
-                                                chunk % BG_EV_SHARDS, n.ev);
+    stdp_stream_dests<SK, W32, BG_STREAM_DESTS, BG_EV_SHARDS>(n, d0, min(BG_STREAM_DESTS, n.N - d0), lane, t,
+                                                              (uint32_t)d0, chunk % BG_EV_SHARDS, n.ev);
 }
 
 // One event synapse (destination d, rank q): the body of the reference's advanceSynapse for a
 // synapse with fPre or fPost. A single network's event pass runs stdp_event_slot, which requests
 // everything before its first store (12.9 -> 9.1 us per step on a 4,624-neuron Izhikevich +
 // DynamicSTDP network). The batch's event pass keeps this form, which reads descriptor fields where
-// they are first needed: there each lane's descriptor belongs to another network, every field read
-// is a divergent gather, and loading all of them up front made the sweep's pass slower (56 -> 59 us).
+// they are first needed: on the 256-network sweep the up-front form was slower there, 59 against
+// 56 us per step with shards shared by several networks (every descriptor field read a divergent
+// gather) and still 52.6 against 48.1 us with a shard range per member; its 182 VGPRs leave two
+// wavefronts per SIMD for the batch's larger event grid.
 template <int SK>
 __device__ __forceinline__ void stdp_event_syn_lazy(const DevNet &n, int d, uint64_t slot, uint64_t t)
 {
@@ -1057,7 +1059,7 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_neurons_batch(BatchMap bm, ui
 {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (BG_WAVE - 1);
-    if (w < 2 * BG_EV_SHARDS) ev.count[w * BG_EV_PAD] = 0;   // launches have at least 2 * BG_EV_SHARDS threads
+    if (w < 2 * BG_EV_SHARDS_BATCH) ev.count[w * BG_EV_PAD] = 0;   // launches have at least that many threads
     const NeuronLane q = neuron_lane(bm);
     const BatchNet &b = bm.nets[q.member];
     bool fired = false;
@@ -1083,16 +1085,16 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_stream_batch(BatchMap bm, uin
     const int i = __builtin_amdgcn_readfirstlane((int)(e.x & 0xFFFFu)), d0 = __builtin_amdgcn_readfirstlane((int)e.y);
     const uint32_t shard = __builtin_amdgcn_readfirstlane(e.x >> 16);
     const BatchNet &b = bm.nets[i];
-    stdp_stream_dests<SK, W32, D>(b.n, d0, min(D, b.n.N - d0), threadIdx.x & (BG_WAVE - 1), b.t0 + step,
-                                  bm.base[i] + (uint32_t)d0, shard, ev);
+    stdp_stream_dests<SK, W32, D, BG_EV_SHARDS_BATCH>(b.n, d0, min(D, b.n.N - d0), threadIdx.x & (BG_WAVE - 1),
+                                                      b.t0 + step, bm.base[i] + (uint32_t)d0, shard, ev);
 }
 
 template <int SK>
 __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn_batch(BatchMap bm, uint64_t step, EvLists ev)
 {
-    const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
-    const uint32_t G = gridDim.x / BG_EV_SHARDS;
-    const uint32_t c = ev.count[(BG_EV_SHARDS + s) * BG_EV_PAD];
+    const uint32_t s = blockIdx.x % BG_EV_SHARDS_BATCH, g = blockIdx.x / BG_EV_SHARDS_BATCH;
+    const uint32_t G = gridDim.x / BG_EV_SHARDS_BATCH;
+    const uint32_t c = ev.count[(BG_EV_SHARDS_BATCH + s) * BG_EV_PAD];
     const uint2 *list = ev.syn + ev.syn_off[s];
     for (uint32_t k = g * blockDim.x + threadIdx.x; k < c; k += G * blockDim.x) {
         const uint2 e = list[k];
@@ -1105,8 +1107,8 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn_batch(BatchMap bm, 
 static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum_batch(BatchMap bm, EvLists ev)
 {
     const int lane = threadIdx.x & (BG_WAVE - 1);
-    const uint32_t s = blockIdx.x % BG_EV_SHARDS, g = blockIdx.x / BG_EV_SHARDS;
-    const uint32_t G = gridDim.x / BG_EV_SHARDS, wpb = blockDim.x / BG_WAVE;
+    const uint32_t s = blockIdx.x % BG_EV_SHARDS_BATCH, g = blockIdx.x / BG_EV_SHARDS_BATCH;
+    const uint32_t G = gridDim.x / BG_EV_SHARDS_BATCH, wpb = blockDim.x / BG_WAVE;
     const uint32_t c = ev.count[s * BG_EV_PAD];
     const uint32_t *list = ev.dest + ev.dest_off[s];
     for (uint32_t k = g * wpb + threadIdx.x / BG_WAVE; k < c; k += G * wpb) {
