@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 from braingrid_amd import bghip as B
+from braingrid_amd import sweep as SW
 from oracle import oracle_py as O
 from tests import helpers as H
 
@@ -311,3 +312,43 @@ def test_destroyed_member_leaves_the_batch(hip_ctx):
         nets[k].close()
     nnew.close()
     assert not bad, bad[:10]
+
+
+def test_more_stdp_members_than_event_shards(hip_ctx):
+    """1,040 Izhikevich + DynamicSTDP + Poisson networks of 36 neurons in one batch: more members
+    than the batch's 1,024 event-list shards, so the members' runs go round all shards and a shard
+    lists several networks' events (with fewer members each has shards of its own). After 2,000
+    batched steps every sampled network equals its alone run bit for bit (full-state digest), no
+    member has a device error, and the networks spiked (so the event lists were used)."""
+    n, steps = 1040, 2000
+    specs = [dict(id=k, side=6, kind="izh_stdp", tR=0.5, fE=0.9, seed=SW.net_seed(77, k), sim_s=1.0)
+             for k in range(n)]
+    sample = list(range(0, n, 97)) + [n - 1]
+    print(f"\n[batch] {n} STDP members, {steps} steps; alone sample {sample}", flush=True)
+    alone = {}
+    for k in sample:
+        x = SW.SweepNet(hip_ctx, specs[k])
+        x.net.advance(steps)
+        x.net.sync()
+        alone[k] = x.fingerprint()
+        x.close()
+    nets = SW.build(hip_ctx, specs)
+    batch = B.Batch(hip_ctx)
+    try:
+        batch.set([x.net for x in nets])
+        batch.advance(steps)
+        batch.sync()
+        groups = batch.shapes()
+        print(f"[batch] groups {[(g['mode'], g['networks']) for g in groups]}", flush=True)
+        assert any(g["networks"] == n for g in groups)
+        errs = [x.spec["id"] for x in nets if x.net.error_bits()]
+        assert not errs, errs[:10]
+        spikes = sum(int(x.net.download_spikes()["spikeCount"].sum()) for x in nets[::13])
+        print(f"[batch] spikes in every 13th network: {spikes}", flush=True)
+        assert spikes > 0
+        diff = [k for k in sample if nets[k].fingerprint() != alone[k]]
+        assert not diff, diff
+    finally:
+        batch.close()
+        for x in nets:
+            x.close()
