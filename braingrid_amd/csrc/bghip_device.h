@@ -69,6 +69,8 @@ struct DevNet {
     uint64_t *last;
     float *Apos, *Aneg, *gap, *tauspost, *tauspre, *taupos, *tauneg, *Wex, *mupos, *muneg;
     uint8_t *useFD;
+    uint8_t *stype;        // STDP: meta's type bits of every slot, for the streaming pass (rebuilt with the
+                           //       outgoing map whenever the synapse table changed)
     uint32_t *syn_count, *slot_hi, *in_map, *in_begin;
     // growth
     float *xloc, *yloc, *radii, *rates;

@@ -210,25 +210,25 @@ __device__ __forceinline__ bool post_walk_learns(const DevNet &n, uint64_t slot,
     return !(delta <= 0 || (double)delta >= 3.0 * (double)n.taupos[slot]);
 }
 
-// The per-type decay of a synapse from the four held in registers (no dependent load after meta).
-__device__ __forceinline__ float decay_of(const float (&dec)[4], uint32_t meta)
+// The decay of a synapse of type k from the four held in registers (no dependent load).
+__device__ __forceinline__ float decay_of(const float (&dec)[4], uint32_t k)
 {
-    const uint32_t k = (meta >> BG_META_TYPE_SHIFT) & 3u;
     const float d0 = dec[0], d1 = dec[1], d2 = dec[2], d3 = dec[3];
     const float lo = (k & 1u) ? d1 : d0, hi = (k & 1u) ? d3 : d2;
     return (k & 2u) ? hi : lo;
 }
 
-// Classifies the synapses of ranks qa and qb = qa + 64 whose ring word, meta and psr are given:
-// ea / eb mark event synapses (a pre spike arrives, or the destination fired and the post-spike
-// walk can change W); event-free ones are decayed, stored and added to acc.
+// Classifies the synapses of ranks qa and qb = qa + 64 whose ring word, type (stype) and psr are
+// given: ea / eb mark event synapses (a pre spike arrives, or the destination fired and the
+// post-spike walk can change W; only then is meta read); event-free ones are decayed, stored and
+// added to acc.
 __device__ __forceinline__ void stdp_stream_classify(const DevNet &n, const float (&dec)[4], uint64_t t, uint32_t m,
                                                      bool post, bool va, uint64_t sa, uint32_t ra, uint32_t ma,
                                                      float pa, bool vb, uint64_t sb, uint32_t rb, uint32_t mb,
                                                      float pb, float &acc, bool &ea, bool &eb)
 {
-    ea = va && ((ra & m) != 0 || (post && post_walk_learns(n, sa, ma, t)));
-    eb = vb && ((rb & m) != 0 || (post && post_walk_learns(n, sb, mb, t)));
+    ea = va && ((ra & m) != 0 || (post && post_walk_learns(n, sa, n.meta[sa], t)));
+    eb = vb && ((rb & m) != 0 || (post && post_walk_learns(n, sb, n.meta[sb], t)));
     if (va && !ea) {
         pa = pa * decay_of(dec, ma);
         n.psr[sa] = pa;
@@ -283,8 +283,8 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
         sa[j] = row + qa; sb[j] = row + qb;
         ra[j] = rb[j] = ma[j] = mb[j] = 0u;
         pa[j] = pb[j] = 0.0f;
-        if (j < nd && qa < M) { ra[j] = n.ring[wbase + sa[j]]; ma[j] = n.meta[sa[j]]; pa[j] = n.psr[sa[j]]; }
-        if (j < nd && qb < M) { rb[j] = n.ring[wbase + sb[j]]; mb[j] = n.meta[sb[j]]; pb[j] = n.psr[sb[j]]; }
+        if (j < nd && qa < M) { ra[j] = n.ring[wbase + sa[j]]; ma[j] = n.stype[sa[j]]; pa[j] = n.psr[sa[j]]; }
+        if (j < nd && qb < M) { rb[j] = n.ring[wbase + sb[j]]; mb[j] = n.stype[sb[j]]; pb[j] = n.psr[sb[j]]; }
     }
     const uint64_t dense_mask = __ballot(h_cnt == h_hi), post_mask = __ballot(lane < nd && h_fs == t);
 #pragma unroll
@@ -295,11 +295,11 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
         if (!dense) {
             if (qa < cnt[j]) {
                 sa[j] = (uint64_t)imap[j][qa];
-                ra[j] = n.ring[wbase + sa[j]]; ma[j] = n.meta[sa[j]]; pa[j] = n.psr[sa[j]];
+                ra[j] = n.ring[wbase + sa[j]]; ma[j] = n.stype[sa[j]]; pa[j] = n.psr[sa[j]];
             }
             if (qb < cnt[j]) {
                 sb[j] = (uint64_t)imap[j][qb];
-                rb[j] = n.ring[wbase + sb[j]]; mb[j] = n.meta[sb[j]]; pb[j] = n.psr[sb[j]];
+                rb[j] = n.ring[wbase + sb[j]]; mb[j] = n.stype[sb[j]]; pb[j] = n.psr[sb[j]];
             }
         }
     }
@@ -330,11 +330,11 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
                 float p1 = 0.0f, p2 = 0.0f;
                 if (va) {
                     s1 = dense ? row + xa : (uint64_t)imap[j][xa];
-                    r1 = n.ring[wbase + s1]; m1 = n.meta[s1]; p1 = n.psr[s1];
+                    r1 = n.ring[wbase + s1]; m1 = n.stype[s1]; p1 = n.psr[s1];
                 }
                 if (vb) {
                     s2 = dense ? row + xb : (uint64_t)imap[j][xb];
-                    r2 = n.ring[wbase + s2]; m2 = n.meta[s2]; p2 = n.psr[s2];
+                    r2 = n.ring[wbase + s2]; m2 = n.stype[s2]; p2 = n.psr[s2];
                 }
                 bool e1, e2;
                 stdp_stream_classify(n, dec, t, m, post, va, s1, r1, m1, p1, vb, s2, r2, m2, p2, acc[j], e1, e2);

@@ -586,6 +586,14 @@ __global__ __launch_bounds__(1024) void k_lpt_order(const uint32_t *counts, uint
     for (int i = tid; i < N; i += 1024) order[atomicAdd(&s_hist[1023u - min(counts[i], 1023u)], 1u)] = (uint32_t)i;
 }
 
+// The per-step STDP streaming pass reads a synapse's type from this byte array instead of its 4-byte
+// meta (meta is re-read only for rows whose destination fired): one thread per slot.
+__global__ void k_slot_types(DevNet n)
+{
+    const uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (s < n.S) n.stype[s] = (uint8_t)((n.meta[s] >> BG_META_TYPE_SHIFT) & 3u);
+}
+
 // incoming map: in-use slots of each destination in ascending slot order (wave per dest).
 __global__ __launch_bounds__(BG_BLOCK) void k_fill_incoming(DevNet n)
 {
