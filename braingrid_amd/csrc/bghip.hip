@@ -621,6 +621,25 @@ static int build_outgoing(bghip_net *n)
 
 static inline uint32_t ring_get(const uint32_t *w, int b) { return (w[b >> 5] >> (b & 31)) & 1u; }
 
+// Between the host's view of a device ring array (32-bit words, word w of slot s at w * S + s) and
+// the byte planes STDP networks keep on the device (byte k of slot s at k * S + s; sring_word).
+static void ring_words_to_planes(std::vector<uint32_t> &ring, size_t S, int W32)
+{
+    std::vector<uint32_t> out(ring.size(), 0u);
+    uint8_t *b = (uint8_t *)out.data();
+    for (int k = 0; k < 4 * W32; k++)
+        for (size_t s = 0; s < S; s++) b[(size_t)k * S + s] = (uint8_t)(ring[(size_t)(k >> 2) * S + s] >> ((k & 3) * 8));
+    ring.swap(out);
+}
+static void ring_planes_to_words(std::vector<uint32_t> &ring, size_t S, int W32)
+{
+    std::vector<uint32_t> out(ring.size(), 0u);
+    const uint8_t *b = (const uint8_t *)ring.data();
+    for (int k = 0; k < 4 * W32; k++)
+        for (size_t s = 0; s < S; s++) out[(size_t)(k >> 2) * S + s] |= (uint32_t)b[(size_t)k * S + s] << ((k & 3) * 8);
+    ring.swap(out);
+}
+
 extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint64_t step)
 {
     if (!n || !h) return BGHIP_EINVAL;
@@ -684,6 +703,7 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
     bool any = false;
     for (int d = 0; d < N && !any; d++) any = cnt[d] != 0;
     if (any) dn.impl = 0;
+    if (stdp) ring_words_to_planes(ring, S, W32);
     CHK(h2d(n, dn.meta, meta.data(), S)); CHK(h2d(n, dn.phase, phase.data(), S));
     CHK(h2d(n, dn.ring, ring.data(), (size_t)W32 * S)); CHK(h2d(n, dn.psr, h->psr, S)); CHK(h2d(n, dn.W, h->W, S));
     CHK(h2d(n, dn.syn_count, cnt.data(), N)); CHK(h2d(n, dn.slot_hi, hi.data(), N));
@@ -1357,6 +1377,7 @@ extern "C" int bghip_download_synapses(bghip_net *n, bghip_synapses *h)
         CHK(d2h(n, h->useFroemkeDanSTDP, dn.useFD, S));
     }
     CHK(bghip_sync(n));
+    if (stdp) ring_planes_to_words(ring, S, W32);
     const uint64_t step = n->step;
     for (size_t s = 0; s < S; s++) {
         uint32_t m = meta[s];
@@ -1560,7 +1581,7 @@ extern "C" int bghip_download_input(bghip_net *n, bghip_input_state *st)
 // Philox noise and Poisson draws are functions of (seed, neuron, step), so the step counter is the
 // whole random-number state.
 static const char kCkptMagic[8] = {'B', 'G', 'H', 'I', 'P', 'N', 'E', 'T'};
-static const uint32_t kCkptVersion = 3;
+static const uint32_t kCkptVersion = 4;
 
 struct CkptHost {   // fixed-size record of the host-side state
     uint64_t step, last_t0, total, hist_step0, in_cyc_off;

@@ -4,7 +4,8 @@
 //   neurons   SoA float/int arrays of length N (Vm, Vthresh, ... exactly the reference's fields)
 //   synapses  slot arrays of length S = N*M, destination-major (iSyn = M*dest + k), the layout of
 //             Synapses/AllSynapses.cpp:366-371; `meta` packs source | type | total_delay | in_use
-//   rings     ring_words x S, word-major (ring[w*S + slot]) so a step touches one coalesced word
+//   rings     ring_words x S, word-major (ring[w*S + slot]) so a step touches one coalesced word;
+//             for STDP networks byte planes (byte k of a slot at k*S + slot, sring_word), one byte
 //   history   uint64 N x max_spikes (Neurons/AllSpikingNeurons.h:185-204)
 // Delay rings are kept in absolute-time orientation: the spike delivered at step T sits at bit
 // T mod R. The reference's per-synapse delayIdx equals (T - phase) mod R, where phase is the
@@ -124,6 +125,18 @@ struct DevNet {
 enum { BG_INPUT_NONE = 0, BG_INPUT_POISSON = 1, BG_INPUT_REGULAR = 2 };
 
 __device__ __forceinline__ void dev_error(const DevNet &n, uint32_t bit) { atomicOr(n.err, bit); }
+
+// The delay rings of STDP networks are byte planes: bit b of slot s's ring is bit (b & 7) of byte
+// (b >> 3) * S + s of the ring array (the same R / 8 bytes per slot as the 32-bit words w * S + s
+// of the Spiking/DS layout). The per-step streaming pass then reads one byte per synapse-step
+// instead of a word. A byte shares its 32-bit word with three other slots, so bit updates are
+// atomic operations on that word.
+__device__ __forceinline__ uint32_t *sring_word(const DevNet &n, uint32_t b, uint64_t slot, uint32_t &bit)
+{
+    const uint64_t byte = (uint64_t)(b >> 3) * n.S + slot;
+    bit = 1u << (((uint32_t)(byte & 3u) << 3) | (b & 7u));
+    return (uint32_t *)((uint8_t *)n.ring + (byte & ~3ull));
+}
 
 // ---------------------------------------------------------------------------------------------
 // Neuron advance. LIF: Neurons/AllLIFNeurons.cpp:20-77; IZH: Neurons/AllIZHNeurons.cpp:570-655;
@@ -431,8 +444,8 @@ __device__ __noinline__ void stdp_event_slot(const DevNet &n, int d, uint64_t sl
     const int ms = n.max_spikes;
     const float dt = n.dt;
     const uint32_t tb = (uint32_t)(t & (uint64_t)(n.R - 1));
-    uint32_t *const wp = &n.ring[(uint64_t)(tb >> 5) * n.S + slot];
-    const uint32_t m = 1u << (tb & 31);
+    uint32_t m;
+    uint32_t *const wp = sring_word(n, tb, slot, m);
     float *const Wp = &n.W[slot];
     float *const psrp = &n.psr[slot];
     const uint64_t *const hist = n.hist;
@@ -583,7 +596,7 @@ __device__ __noinline__ void stdp_event_slot(const DevNet &n, int d, uint64_t sl
     }
 
     // ---- stores
-    if (pre) *wp = word & ~m;
+    if (pre) atomicAnd(wp, ~m);
     if (learned) *Wp = W;
     if (SK == SK_DYNSTDP && pre) {
         if (ru_changed) {

@@ -70,18 +70,33 @@ __device__ __forceinline__ void synapse_step_body(const DevNet &n, int d, int la
         const int src = (int)(meta & BG_META_SRC_MASK);
         const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
         const int delay = (meta >> BG_META_DELAY_SHIFT) & 0xFF;
+        constexpr bool planes = SK == SK_STDP || SK == SK_DYNSTDP;   // STDP rings are byte planes
         if (n.fired_step[src] == t) {   // preSpikeHit at the firing step
-            uint32_t b = (tb + (uint32_t)delay) & (uint32_t)(R - 1);
-            uint32_t *w = &n.ring[(uint64_t)(b >> 5) * n.S + slot];
-            uint32_t m = 1u << (b & 31);
-            if (*w & m) dev_error(n, 1u);
-            *w |= m;
+            const uint32_t b = (tb + (uint32_t)delay) & (uint32_t)(R - 1);
+            if (planes) {
+                uint32_t m;
+                uint32_t *w = sring_word(n, b, slot, m);
+                if (atomicOr(w, m) & m) dev_error(n, 1u);
+            } else {
+                uint32_t *w = &n.ring[(uint64_t)(b >> 5) * n.S + slot];
+                const uint32_t m = 1u << (b & 31);
+                if (*w & m) dev_error(n, 1u);
+                *w |= m;
+            }
         }
-        uint32_t *w = &n.ring[(uint64_t)(tb >> 5) * n.S + slot];
-        const uint32_t m = 1u << (tb & 31);
-        const uint32_t word = *w;
-        const bool pre = (word & m) != 0;
-        if (pre) *w = word & ~m;
+        bool pre;
+        if (planes) {
+            uint32_t m;
+            uint32_t *w = sring_word(n, tb, slot, m);
+            pre = (*w & m) != 0;
+            if (pre) atomicAnd(w, ~m);
+        } else {
+            uint32_t *w = &n.ring[(uint64_t)(tb >> 5) * n.S + slot];
+            const uint32_t m = 1u << (tb & 31);
+            const uint32_t word = *w;
+            pre = (word & m) != 0;
+            if (pre) *w = word & ~m;
+        }
         float psr = n.psr[slot];
         const float decay = n.decay[type];
         if (SK == SK_STDP || SK == SK_DYNSTDP) {
@@ -160,11 +175,9 @@ __device__ __forceinline__ void push_spike(const DevNet &n, int src, uint64_t t,
     for (uint32_t q = lane; q < c; q += BG_WAVE) {
         const uint32_t slot = n.out_map[beg + q];
         const uint32_t b = (tb + ((n.meta[slot] >> BG_META_DELAY_SHIFT) & 0xFFu)) & Rm;
-        uint32_t *w = &n.ring[(uint64_t)(b >> 5) * n.S + slot];
-        const uint32_t m = 1u << (b & 31);
-        const uint32_t v = *w;
-        if (v & m) dev_error(n, 1u);
-        *w = v | m;
+        uint32_t m;
+        uint32_t *w = sring_word(n, b, slot, m);
+        if (atomicOr(w, m) & m) dev_error(n, 1u);
     }
 }
 
@@ -257,8 +270,8 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
 {
     const float dec[4] = {n.decay[0], n.decay[1], n.decay[2], n.decay[3]};
     const uint32_t tb = (uint32_t)(t & (uint64_t)(n.R - 1));
-    const uint64_t wbase = (uint64_t)(tb >> 5) * n.S;
-    const uint32_t m = 1u << (tb & 31);
+    const uint8_t *const rplane = (const uint8_t *)n.ring + (uint64_t)(tb >> 3) * n.S;   // this step's byte plane
+    const uint32_t m = 1u << (tb & 7);
     uint32_t h_cnt = 0, h_hi = 0, h_beg = 0;
     uint64_t h_fs = BG_NO_SPIKE;
     float h_sp = 0.0f;
@@ -283,8 +296,8 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
         sa[j] = row + qa; sb[j] = row + qb;
         ra[j] = rb[j] = ma[j] = mb[j] = 0u;
         pa[j] = pb[j] = 0.0f;
-        if (j < nd && qa < M) { ra[j] = n.ring[wbase + sa[j]]; ma[j] = n.stype[sa[j]]; pa[j] = n.psr[sa[j]]; }
-        if (j < nd && qb < M) { rb[j] = n.ring[wbase + sb[j]]; mb[j] = n.stype[sb[j]]; pb[j] = n.psr[sb[j]]; }
+        if (j < nd && qa < M) { ra[j] = rplane[sa[j]]; ma[j] = n.stype[sa[j]]; pa[j] = n.psr[sa[j]]; }
+        if (j < nd && qb < M) { rb[j] = rplane[sb[j]]; mb[j] = n.stype[sb[j]]; pb[j] = n.psr[sb[j]]; }
     }
     const uint64_t dense_mask = __ballot(h_cnt == h_hi), post_mask = __ballot(lane < nd && h_fs == t);
 #pragma unroll
@@ -295,11 +308,11 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
         if (!dense) {
             if (qa < cnt[j]) {
                 sa[j] = (uint64_t)imap[j][qa];
-                ra[j] = n.ring[wbase + sa[j]]; ma[j] = n.stype[sa[j]]; pa[j] = n.psr[sa[j]];
+                ra[j] = rplane[sa[j]]; ma[j] = n.stype[sa[j]]; pa[j] = n.psr[sa[j]];
             }
             if (qb < cnt[j]) {
                 sb[j] = (uint64_t)imap[j][qb];
-                rb[j] = n.ring[wbase + sb[j]]; mb[j] = n.stype[sb[j]]; pb[j] = n.psr[sb[j]];
+                rb[j] = rplane[sb[j]]; mb[j] = n.stype[sb[j]]; pb[j] = n.psr[sb[j]];
             }
         }
     }
@@ -330,11 +343,11 @@ __device__ __forceinline__ void stdp_stream_dests(const DevNet &n, int d0, int n
                 float p1 = 0.0f, p2 = 0.0f;
                 if (va) {
                     s1 = dense ? row + xa : (uint64_t)imap[j][xa];
-                    r1 = n.ring[wbase + s1]; m1 = n.stype[s1]; p1 = n.psr[s1];
+                    r1 = rplane[s1]; m1 = n.stype[s1]; p1 = n.psr[s1];
                 }
                 if (vb) {
                     s2 = dense ? row + xb : (uint64_t)imap[j][xb];
-                    r2 = n.ring[wbase + s2]; m2 = n.stype[s2]; p2 = n.psr[s2];
+                    r2 = rplane[s2]; m2 = n.stype[s2]; p2 = n.psr[s2];
                 }
                 bool e1, e2;
                 stdp_stream_classify(n, dec, t, m, post, va, s1, r1, m1, p1, vb, s2, r2, m2, p2, acc[j], e1, e2);
@@ -415,11 +428,10 @@ template <int SK>
 __device__ __forceinline__ void stdp_event_syn_lazy(const DevNet &n, int d, uint64_t slot, uint64_t t)
 {
     const uint32_t tb = (uint32_t)(t & (uint64_t)(n.R - 1));
-    uint32_t *wp = &n.ring[(uint64_t)(tb >> 5) * n.S + slot];
-    const uint32_t m = 1u << (tb & 31);
-    const uint32_t word = *wp;
-    const bool pre = (word & m) != 0;
-    if (pre) *wp = word & ~m;
+    uint32_t m;
+    uint32_t *wp = sring_word(n, tb, slot, m);
+    const bool pre = (*wp & m) != 0;
+    if (pre) atomicAnd(wp, ~m);
     const bool post = n.fired_step[d] == t;
     const uint32_t meta = n.meta[slot];
     const int type = (meta >> BG_META_TYPE_SHIFT) & 3;

@@ -73,7 +73,11 @@ __device__ void create_synapse(const DevNet &n, const CreateCtx &c, uint64_t slo
     n.psr[slot] = 0.0f;
     n.W[slot] = w;
     if (n.Wd) n.Wd[slot] = w / n.decay[type];
-    for (int q = 0; q < n.W32; q++) n.ring[(uint64_t)q * n.S + slot] = 0u;
+    if (c.syn_kind == SK_STDP || c.syn_kind == SK_DYNSTDP) {   // byte planes (sring_word)
+        for (int q = 0; q < 4 * n.W32; q++) ((uint8_t *)n.ring)[(uint64_t)q * n.S + slot] = 0u;
+    } else {
+        for (int q = 0; q < n.W32; q++) n.ring[(uint64_t)q * n.S + slot] = 0u;
+    }
     if (c.syn_kind == SK_DS || c.syn_kind == SK_DYNSTDP) {
         n.r[slot] = c.r0;
         n.us[slot] = c.u0;
