@@ -302,7 +302,7 @@ static void plan_net(bghip_net *n, Carve &c)
     if (stdp) {
         c.add(&dn.Apos, S); c.add(&dn.Aneg, S); c.add(&dn.gap, S); c.add(&dn.tauspost, S); c.add(&dn.tauspre, S);
         c.add(&dn.taupos, S); c.add(&dn.tauneg, S); c.add(&dn.Wex, S); c.add(&dn.mupos, S); c.add(&dn.muneg, S);
-        c.add(&dn.useFD, S); c.add(&dn.stype, S);
+        c.add(&dn.useFD, S); c.add(&dn.stype, S); c.add(&dn.out_delay, S);
     }
     c.add(&dn.syn_count, N); c.add(&dn.slot_hi, N); c.add(&dn.in_map, S); c.add(&dn.in_begin, N);
     c.add(&dn.xloc, N); c.add(&dn.yloc, N); c.add(&dn.radii, N); c.add(&dn.rates, N); c.add(&dn.ntype, N);
@@ -613,6 +613,10 @@ static int build_outgoing(bghip_net *n)
     if (dn.stype && dn.S) {
         const unsigned nb = (unsigned)((dn.S + BG_BLOCK - 1) / BG_BLOCK);
         hipLaunchKernelGGL(k_slot_types, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn);
+    }
+    if (dn.out_delay && T > 0) {
+        const unsigned nb = (unsigned)((T + BG_BLOCK - 1) / BG_BLOCK);
+        hipLaunchKernelGGL(k_out_delays, dim3(nb), dim3(BG_BLOCK), 0, n->stream, dn, (uint64_t)T);
     }
     HIP_TRY(n->ctx, hipGetLastError());
     n->out_valid = true;
@@ -1581,7 +1585,7 @@ extern "C" int bghip_download_input(bghip_net *n, bghip_input_state *st)
 // Philox noise and Poisson draws are functions of (seed, neuron, step), so the step counter is the
 // whole random-number state.
 static const char kCkptMagic[8] = {'B', 'G', 'H', 'I', 'P', 'N', 'E', 'T'};
-static const uint32_t kCkptVersion = 4;
+static const uint32_t kCkptVersion = 5;
 
 struct CkptHost {   // fixed-size record of the host-side state
     uint64_t step, last_t0, total, hist_step0, in_cyc_off;

@@ -97,6 +97,7 @@ struct DevNet {
     // per-step STDP path: the step's spike events, in sharded lists (see EvLists)
     EvLists ev;
     const uint32_t *out_begin, *out_count, *out_map;   // outgoing map (synapses of each source)
+    uint8_t *out_delay;          // STDP: total delay of each outgoing-map entry's synapse (built with the map)
     const DevNet *self;          // this descriptor in device memory (for kernels that call out of line)
     uint64_t *tbase;             // per-step STDP launches: step = *tbase + the launch's offset (graph replays)
     // Implicit delay rings (Spiking/DS networks with 32-bit rings, fused blocks). A synapse's ring

@@ -174,7 +174,7 @@ __device__ __forceinline__ void push_spike(const DevNet &n, int src, uint64_t t,
     const uint32_t tb = (uint32_t)t & Rm;
     for (uint32_t q = lane; q < c; q += BG_WAVE) {
         const uint32_t slot = n.out_map[beg + q];
-        const uint32_t b = (tb + ((n.meta[slot] >> BG_META_DELAY_SHIFT) & 0xFFu)) & Rm;
+        const uint32_t b = (tb + (uint32_t)n.out_delay[beg + q]) & Rm;
         uint32_t m;
         uint32_t *w = sring_word(n, b, slot, m);
         if (atomicOr(w, m) & m) dev_error(n, 1u);

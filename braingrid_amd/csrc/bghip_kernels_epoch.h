@@ -598,6 +598,15 @@ __global__ void k_slot_types(DevNet n)
     if (s < n.S) n.stype[s] = (uint8_t)((n.meta[s] >> BG_META_TYPE_SHIFT) & 3u);
 }
 
+// The per-step STDP neuron pass pushes a spike to each synapse of the outgoing map at its delay;
+// keeping that delay beside the map entry saves the push a dependent read of meta. One thread per
+// entry, after the map is sorted.
+__global__ void k_out_delays(DevNet n, uint64_t total)
+{
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < total) n.out_delay[i] = (uint8_t)((n.meta[n.out_map[i]] >> BG_META_DELAY_SHIFT) & 0xFFu);
+}
+
 // incoming map: in-use slots of each destination in ascending slot order (wave per dest).
 __global__ __launch_bounds__(BG_BLOCK) void k_fill_incoming(DevNet n)
 {
