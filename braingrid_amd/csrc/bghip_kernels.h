@@ -483,6 +483,48 @@ __device__ __forceinline__ void stdp_sum_body(const DevNet &n, int d, int lane)
     if (lane == 0) n.sp[d] = sp0 + acc;
 }
 
+// stdp_sum_body for destinations d_0 .. d_{nd-1} (nd <= D) of one network on one wavefront (lane j
+// holds d_j): the D rows' first 128 slots and headers are requested together, so the chain is paid
+// once for D destinations; each destination's sum has stdp_sum_body's order and tree.
+template <int D>
+__device__ __forceinline__ void stdp_sum_dests(const DevNet &n, int dl, int nd, int lane)
+{
+    const uint32_t M = (uint32_t)n.M, qa = (uint32_t)lane, qb = qa + BG_WAVE;
+    uint32_t h_cnt = 0, h_hi = 0, h_beg = 0;
+    float h_sp = 0.0f;
+    if (lane < nd) {
+        h_cnt = n.syn_count[dl];
+        h_hi = n.slot_hi[dl];
+        h_beg = n.in_begin[dl];
+        h_sp = n.sp[dl];
+    }
+    float sa[D], sb[D];
+    uint64_t row[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        row[j] = (uint64_t)__builtin_amdgcn_readlane(dl, j) * M;
+        sa[j] = j < nd && qa < M ? n.psr[row[j] + qa] : 0.0f;
+        sb[j] = j < nd && qb < M ? n.psr[row[j] + qb] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        if (j >= nd) break;
+        const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)h_cnt, j);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)h_hi, j);
+        float acc = 0.0f;
+        if (cnt == hi) {
+            if (qa < cnt) acc += sa[j];
+            if (qb < cnt) acc += sb[j];
+            for (uint32_t q = qb + BG_WAVE; q < cnt; q += BG_WAVE) acc += n.psr[row[j] + q];
+        } else {
+            const uint32_t *imap = n.in_map + (uint32_t)__builtin_amdgcn_readlane((int)h_beg, j);
+            for (uint32_t q = qa; q < cnt; q += BG_WAVE) acc += n.psr[imap[q]];
+        }
+        acc = wave_sum(acc);
+        if (lane == j) n.sp[dl] = h_sp + acc;
+    }
+}
+
 static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum(DevNet n)
 {
     const int lane = threadIdx.x & (BG_WAVE - 1);
@@ -1116,17 +1158,30 @@ __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_syn_batch(BatchMap bm, 
     }
 }
 
+// Four listed destinations per wavefront: a shard lists one member's events unless the batch has
+// more members than shards, so the four are normally one network's (else one at a time).
 static __global__ __launch_bounds__(BG_BLOCK) void k_stdp_event_sum_batch(BatchMap bm, EvLists ev)
 {
+    constexpr int D = 4;
     const int lane = threadIdx.x & (BG_WAVE - 1);
     const uint32_t s = blockIdx.x % BG_EV_SHARDS_BATCH, g = blockIdx.x / BG_EV_SHARDS_BATCH;
     const uint32_t G = gridDim.x / BG_EV_SHARDS_BATCH, wpb = blockDim.x / BG_WAVE;
     const uint32_t c = ev.count[s * BG_EV_PAD];
     const uint32_t *list = ev.dest + ev.dest_off[s];
-    for (uint32_t k = g * wpb + threadIdx.x / BG_WAVE; k < c; k += G * wpb) {
-        const uint32_t w = __builtin_amdgcn_readfirstlane(list[k]);
-        const int i = __builtin_amdgcn_readfirstlane((int)bm.nof[w]);
-        stdp_sum_body(bm.nets[i].n, (int)(w - bm.base[i]), lane);
+    for (uint32_t k0 = (g * wpb + threadIdx.x / BG_WAVE) * D; k0 < c; k0 += G * wpb * D) {
+        const int nd = (int)min((uint32_t)D, c - k0);
+        const uint32_t wl = lane < nd ? list[k0 + lane] : 0u;
+        const int il = lane < nd ? (int)bm.nof[wl] : 0;
+        const int i0 = __builtin_amdgcn_readfirstlane(il);
+        if (__ballot(lane < nd && il != i0) == 0ull) {
+            stdp_sum_dests<D>(bm.nets[i0].n, (int)(wl - bm.base[i0]), nd, lane);
+        } else {
+            for (int j = 0; j < nd; j++) {
+                const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)wl, j);
+                const int i = __builtin_amdgcn_readlane(il, j);
+                stdp_sum_body(bm.nets[i].n, (int)(w - bm.base[i]), lane);
+            }
+        }
     }
 }
 
