@@ -83,6 +83,8 @@ struct bghip_net {
     bool pending;          // most recent block's spikes not yet inserted into the rings
     int min_delay_uploaded;
     bool has_layout, has_growth;
+    bool tp_set[4];                // STDP: the entry of each type in dn.tp has been set
+    std::vector<int32_t> ntype_host;   // the uploaded layout's neuron types (synapse types of added synapses)
     bghip_growth growth;
     uint32_t *d_out_map, *d_out_begin, *d_out_count, *d_cursor, *d_stats;
     uint64_t *d_total;
@@ -389,6 +391,7 @@ extern "C" int bghip_net_create(bghip_ctx *ctx, const bghip_net_desc *desc, bghi
         dn.tF[t] = desc->types.F[t];
     }
     dn.ds_typed = 1;
+    dn.stdp_typed = 1;
     for (int t = 0; t < 4; t++) dn.tdelay[t] = desc->types.total_delay[t];
     dn.self = n->d_self;
     dn.out_begin = n->d_out_begin;
@@ -644,6 +647,28 @@ static void ring_planes_to_words(std::vector<uint32_t> &ring, size_t S, int W32)
     ring.swap(out);
 }
 
+// The per-type STDP table (DevNet::tp): a synapse of type t with parameters p either sets the
+// entry of t or, if it differs from it in any bit, clears stdp_typed (the slot arrays stay the
+// reference's and are what the kernels read from then on).
+static void tp_merge(bghip_net *n, int t, const StdpParams &p)
+{
+    if (!n->tp_set[t]) {
+        n->dn.tp[t] = p;
+        n->tp_set[t] = true;
+    } else if (memcmp(&n->dn.tp[t], &p, sizeof(StdpParams)) != 0) {
+        n->dn.stdp_typed = 0;
+    }
+}
+static StdpParams tp_of_ctx(const CreateCtx &c)
+{
+    StdpParams p;
+    memset(&p, 0, sizeof p);
+    p.Apos = c.Apos; p.Aneg = c.Aneg; p.gap = c.gap; p.tauspost = c.tauspost; p.tauspre = c.tauspre;
+    p.taupos = c.taupos; p.tauneg = c.tauneg; p.Wex = c.Wex; p.mupos = c.mupos; p.muneg = c.muneg;
+    p.useFD = (int32_t)(uint8_t)c.useFD;
+    return p;
+}
+
 extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint64_t step)
 {
     if (!n || !h) return BGHIP_EINVAL;
@@ -667,6 +692,10 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
     std::vector<uint8_t> phase(S, 0);
     int min_delay = 255;
     int typed = 1;
+    if (stdp) {   // the uploaded slots replace every synapse: the table starts over
+        for (int t = 0; t < 4; t++) n->tp_set[t] = false;
+        dn.stdp_typed = 1;
+    }
     for (int d = 0; d < N; d++) {
         for (int k = 0; k < M; k++) {
             size_t s = (size_t)d * M + k;
@@ -683,6 +712,14 @@ extern "C" int bghip_upload_synapses(bghip_net *n, const bghip_synapses *h, uint
             min_delay = std::min(min_delay, del);
             if (ds && (memcmp(&h->U[s], &dn.tU[t], 4) || memcmp(&h->D[s], &dn.tD[t], 4) || memcmp(&h->F[s], &dn.tF[t], 4)))
                 typed = 0;
+            if (stdp) {
+                StdpParams p;
+                memset(&p, 0, sizeof p);
+                p.Apos = h->Apos[s]; p.Aneg = h->Aneg[s]; p.gap = h->STDPgap[s]; p.tauspost = h->tauspost[s];
+                p.tauspre = h->tauspre[s]; p.taupos = h->taupos[s]; p.tauneg = h->tauneg[s]; p.Wex = h->Wex[s];
+                p.mupos = h->mupos[s]; p.muneg = h->muneg[s]; p.useFD = (int32_t)(uint8_t)h->useFroemkeDanSTDP[s];
+                tp_merge(n, t, p);
+            }
             meta[s] = (uint32_t)src | ((uint32_t)t << BG_META_TYPE_SHIFT) | ((uint32_t)del << BG_META_DELAY_SHIFT) |
                       BG_META_IN_USE;
             // reference bit b (relative to delayIdx) -> absolute bit (b + phase) mod R,
@@ -740,6 +777,7 @@ extern "C" int bghip_upload_layout(bghip_net *n, const float *xloc, const float 
     HIP_TRY(n->ctx, hipSetDevice(n->ctx->device));
     CHK(h2d(n, n->dn.xloc, xloc, N)); CHK(h2d(n, n->dn.yloc, yloc, N)); CHK(h2d(n, n->dn.ntype, ntype, N));
     HIP_TRY(n->ctx, hipStreamSynchronize(n->stream));
+    n->ntype_host.assign(ntype, ntype + N);
     n->has_layout = true;
     return BGHIP_OK;
 }
@@ -1071,6 +1109,19 @@ extern "C" int bghip_add_synapses_ex(bghip_net *n, int64_t count, const int32_t 
     if (total_delay) CHK(to_explicit(n));
     CreateCtx cc = create_ctx(n);
     CHK(coh_begin(n, cc));
+    if (cc.syn_kind == SK_STDP || cc.syn_kind == SK_DYNSTDP) {
+        StdpParams p = tp_of_ctx(cc);
+        if (!Wex) {
+            for (int t = 0; t < 4; t++) tp_merge(n, t, p);
+        } else {
+            // createSynapse's values with the caller's Wex, under the type the device gives the synapse
+            for (int64_t i = 0; i < count; i++) {
+                const int ts = n->ntype_host[src[i]], td = n->ntype_host[dest[i]];
+                p.Wex = Wex[i];
+                tp_merge(n, ts == 1 ? (td == 1 ? 0 : 1) : (td == 1 ? 2 : 3), p);
+            }
+        }
+    }
     const int nd = (int)dests.size();
     char *buf = nullptr;
     const size_t o_dest = 0, o_begin = align256(sizeof(int32_t) * nd),
@@ -1133,6 +1184,8 @@ static int apply_weights(bghip_net *n, const float *Wd, bghip_growth_stats *stat
     DevNet &dn = n->dn;
     CreateCtx cc = create_ctx(n);
     CHK(coh_begin(n, cc));
+    if (cc.syn_kind == SK_STDP || cc.syn_kind == SK_DYNSTDP)
+        for (int t = 0; t < 4; t++) tp_merge(n, t, tp_of_ctx(cc));
     HIP_TRY(n->ctx, hipMemsetAsync(n->d_stats, 0, 16, n->stream));
     if (dn.M > 0) {
         unsigned grid = (unsigned)std::min(dn.N, 4096);
@@ -1585,7 +1638,7 @@ extern "C" int bghip_download_input(bghip_net *n, bghip_input_state *st)
 // Philox noise and Poisson draws are functions of (seed, neuron, step), so the step counter is the
 // whole random-number state.
 static const char kCkptMagic[8] = {'B', 'G', 'H', 'I', 'P', 'N', 'E', 'T'};
-static const uint32_t kCkptVersion = 5;
+static const uint32_t kCkptVersion = 6;
 
 struct CkptHost {   // fixed-size record of the host-side state
     uint64_t step, last_t0, total, hist_step0, in_cyc_off;
@@ -1599,6 +1652,9 @@ struct CkptHost {   // fixed-size record of the host-side state
     int32_t impl;            // implicit rings (the fire words and cohort states are in the arena)
     uint32_t coh_alloc;
     uint64_t young_until;
+    int32_t stdp_typed;      // the per-type STDP table
+    uint8_t tp_set[4];
+    StdpParams tp[4];
 };
 
 template <typename T>
@@ -1621,6 +1677,8 @@ extern "C" int bghip_net_save(bghip_net *n, const char *path)
     h.step = n->step; h.last_t0 = n->last_t0; h.total = net_total(n); h.hist_step0 = n->hist_step0;
     h.in_cyc_off = dn.in_cyc_off; h.n_burst = n->n_burst; h.n_shist = n->n_shist;
     h.kmax = n->kmax; h.sel = n->sel; h.min_delay_uploaded = n->min_delay_uploaded; h.ds_typed = dn.ds_typed;
+    h.stdp_typed = dn.stdp_typed;
+    for (int t = 0; t < 4; t++) { h.tp_set[t] = n->tp_set[t] ? 1 : 0; h.tp[t] = dn.tp[t]; }
     h.in_kind = dn.in_kind; h.in_dur = dn.in_dur; h.in_cycle = dn.in_cycle; h.rec_epochs = n->rec_epochs;
     h.rec_epoch = n->rec_epoch; h.in_phase = n->in_phase; h.pending = n->pending; h.has_layout = n->has_layout;
     h.has_growth = n->has_growth; h.rec_on = n->rec_on; h.rec_growth = n->rec_growth;
@@ -1706,6 +1764,8 @@ extern "C" int bghip_net_load(bghip_ctx *ctx, const char *path, bghip_net **out)
     n->pending = h.pending; n->has_layout = h.has_layout; n->has_growth = h.has_growth; n->growth = h.growth;
     n->hist_range_known = h.hist_range_known; n->in_phase = h.in_phase; n->out_valid = false;
     dn.ds_typed = h.ds_typed; dn.in_kind = h.in_kind; dn.in_dur = h.in_dur; dn.in_cycle = h.in_cycle;
+    dn.stdp_typed = h.stdp_typed;
+    for (int t = 0; t < 4; t++) { n->tp_set[t] = h.tp_set[t] != 0; dn.tp[t] = h.tp[t]; }
     dn.in_cyc_off = h.in_cyc_off; dn.in_W = h.in_W; dn.in_lambda = h.in_lambda;
     for (int k = 0; k < 2; k++) {
         dn.in_decay[k] = h.in_decay[k]; dn.in_U[k] = h.in_U[k]; dn.in_D[k] = h.in_D[k]; dn.in_F[k] = h.in_F[k];
@@ -1730,6 +1790,13 @@ extern "C" int bghip_net_load(bghip_ctx *ctx, const char *path, bghip_net **out)
             hipStreamSynchronize(n->stream) != hipSuccess) {
             bghip_net_destroy(n);
             return set_err(ctx, BGHIP_EDEVICE, "net_load: recorder upload failed");
+        }
+    }
+    if (n->has_layout) {   // the layout's neuron types, for synapses added later
+        n->ntype_host.resize((size_t)dn.N);
+        if (d2h(n, n->ntype_host.data(), dn.ntype, (size_t)dn.N) || hipStreamSynchronize(n->stream) != hipSuccess) {
+            bghip_net_destroy(n);
+            return set_err(ctx, BGHIP_EDEVICE, "net_load: layout download failed");
         }
     }
     *out = n;

@@ -48,6 +48,15 @@ struct EvLists {
     uint2 *syn;                 // event synapses as (destination work item, slot)
 };
 
+// The STDP parameters createSynapse gives a synapse (Synapses/AllSTDPSynapses.cpp:252-273), kept per
+// slot like the reference's arrays and, while every synapse of a type has its type's values, once per
+// type: the learning passes then read them from the network descriptor instead of from eleven slot
+// arrays (a cache line each per event synapse).
+struct StdpParams {
+    float Apos, Aneg, gap, tauspost, tauspre, taupos, tauneg, Wex, mupos, muneg;
+    int32_t useFD;
+};
+
 struct DevNet {
     int N, M, R, W32, max_spikes;
     float dt;
@@ -79,6 +88,8 @@ struct DevNet {
     float decay[4];
     int ds_typed;          // 1: every synapse's U, D, F equal its type's creation values
     float tU[4], tD[4], tF[4];
+    int stdp_typed;        // 1: every STDP synapse's parameters equal its type's entry of tp
+    StdpParams tp[4];
     uint32_t *err;
     // stimulus input (Inputs/SInputPoisson.cpp, Inputs/SInputRegular.cpp)
     int in_kind;                 // BG_INPUT_*
@@ -333,11 +344,23 @@ __device__ __forceinline__ uint64_t spike_hist(const DevNet &n, int idx, int off
     return n.hist[(uint64_t)idx * ms + k];
 }
 
-// AllSTDPSynapses::stdpLearning (Synapses/AllSTDPSynapses.cpp:432-472)
-__device__ __forceinline__ void stdp_learning(const DevNet &n, uint64_t slot, double delta, double epost, double epre)
+// A synapse's STDP parameters: its type's when the network is typed, else its slot's.
+__device__ __forceinline__ StdpParams stdp_params(const DevNet &n, uint64_t slot, int type)
 {
-    float STDPgap = n.gap[slot], muneg = n.muneg[slot], mupos = n.mupos[slot], tauneg = n.tauneg[slot];
-    float taupos = n.taupos[slot], Aneg = n.Aneg[slot], Apos = n.Apos[slot], Wex = n.Wex[slot];
+    if (n.stdp_typed) return n.tp[type];
+    StdpParams p;
+    p.Apos = n.Apos[slot]; p.Aneg = n.Aneg[slot]; p.gap = n.gap[slot]; p.tauspost = n.tauspost[slot];
+    p.tauspre = n.tauspre[slot]; p.taupos = n.taupos[slot]; p.tauneg = n.tauneg[slot]; p.Wex = n.Wex[slot];
+    p.mupos = n.mupos[slot]; p.muneg = n.muneg[slot]; p.useFD = n.useFD[slot];
+    return p;
+}
+
+// AllSTDPSynapses::stdpLearning (Synapses/AllSTDPSynapses.cpp:432-472)
+__device__ __forceinline__ void stdp_learning(const DevNet &n, uint64_t slot, const StdpParams &P, double delta,
+                                              double epost, double epre)
+{
+    const float STDPgap = P.gap, muneg = P.muneg, mupos = P.mupos, tauneg = P.tauneg;
+    const float taupos = P.taupos, Aneg = P.Aneg, Apos = P.Apos, Wex = P.Wex;
     float W = n.W[slot];
     float dw;
     if (delta < -(double)STDPgap) {
@@ -363,8 +386,9 @@ __device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, int typ
                                          int idxPre, int idxPost, int total_delay, float psr, float decay)
 {
     const float dt = n.dt;
-    float tauspre = n.tauspre[slot], tauspost = n.tauspost[slot], taupos = n.taupos[slot], tauneg = n.tauneg[slot];
-    bool useFD = n.useFD[slot] != 0;
+    const StdpParams P = stdp_params(n, slot, type);
+    const float tauspre = P.tauspre, tauspost = P.tauspost, taupos = P.taupos, tauneg = P.tauneg;
+    const bool useFD = P.useFD != 0;
     const int walk = n.max_spikes;
     uint64_t sh, sh2;
     float delta, epre, epost;
@@ -389,7 +413,7 @@ __device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, int typ
                 epost = 1.0f;
             }
             const uint32_t w0 = __float_as_uint(n.W[slot]);
-            stdp_learning(n, slot, (double)delta, (double)epost, (double)epre);
+            stdp_learning(n, slot, P, (double)delta, (double)epost, (double)epre);
             // An entry older than step t has an interval that wraps in the unsigned arithmetic to
             // (float)2^64 * dt, and so does every older entry after it (all are earlier steps, and
             // t < 2^39 keeps the float conversion at 2^64). With that delta, dw is P(W) * 0, so a
@@ -422,7 +446,7 @@ __device__ __noinline__ float stdp_event(const DevNet &n, uint64_t slot, int typ
             } else {
                 epre = 1.0f;
             }
-            stdp_learning(n, slot, (double)delta, (double)epost, (double)epre);
+            stdp_learning(n, slot, P, (double)delta, (double)epost, (double)epre);
         }
     }
     return psr;
@@ -455,11 +479,6 @@ __device__ __noinline__ void stdp_event_slot(const DevNet &n, int d, uint64_t sl
     const uint32_t meta = n.meta[slot];
     float psr = *psrp;
     float W = *Wp;
-    const float tauspre = n.tauspre[slot], tauspost = n.tauspost[slot], taupos = n.taupos[slot];
-    const float tauneg = n.tauneg[slot];
-    const bool useFD = n.useFD[slot] != 0;
-    const float gap = n.gap[slot], muneg = n.muneg[slot], mupos = n.mupos[slot], Aneg = n.Aneg[slot];
-    const float Apos = n.Apos[slot], Wex = n.Wex[slot];
     const int base_post = n.cnt[d] + n.off[d] + ms;
     float r = 0.0f, u = 0.0f, Us = 0.0f, Ds = 0.0f, Fs = 0.0f;
     uint64_t last = BG_NO_SPIKE;
@@ -481,6 +500,10 @@ __device__ __noinline__ void stdp_event_slot(const DevNet &n, int d, uint64_t sl
     const int type = (meta >> BG_META_TYPE_SHIFT) & 3;
     const int total_delay = (int)((meta >> BG_META_DELAY_SHIFT) & 0xFFu);
     const float decay = n.decay[type];
+    const StdpParams P = stdp_params(n, slot, type);
+    const float tauspre = P.tauspre, tauspost = P.tauspost, taupos = P.taupos, tauneg = P.tauneg;
+    const bool useFD = P.useFD != 0;
+    const float gap = P.gap, muneg = P.muneg, mupos = P.mupos, Aneg = P.Aneg, Apos = P.Apos, Wex = P.Wex;
     if (SK == SK_DYNSTDP && typed) {
         Ds = n.tD[type];
         Fs = n.tF[type];

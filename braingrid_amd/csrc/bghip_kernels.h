@@ -220,7 +220,8 @@ __device__ __forceinline__ bool post_walk_learns(const DevNet &n, uint64_t slot,
     if (sh == BG_NO_SPIKE) return false;
     const int total_delay = (int)((meta >> BG_META_DELAY_SHIFT) & 0xFFu);
     const float delta = (float)(uint64_t)(t - sh - (uint64_t)(int64_t)total_delay) * n.dt;
-    return !(delta <= 0 || (double)delta >= 3.0 * (double)n.taupos[slot]);
+    const float taupos = n.stdp_typed ? n.tp[(meta >> BG_META_TYPE_SHIFT) & 3u].taupos : n.taupos[slot];
+    return !(delta <= 0 || (double)delta >= 3.0 * (double)taupos);
 }
 
 // The decay of a synapse of type k from the four held in registers (no dependent load).

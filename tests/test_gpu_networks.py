@@ -95,3 +95,38 @@ def test_device_built_network_equals_uploaded_slots(hip_ctx):
     finally:
         a.close()
         b.close()
+
+
+def test_stdp_parameters_differing_within_a_type(hip_ctx):
+    """Synapses of one type with different STDP parameters (taupos, Apos and Wex of every 7th
+    synapse scaled): no per-type table describes them, so the device must read each synapse's own
+    values, and every state array equals the oracle's over 2,000 steps. The same network with the
+    builder's values (one set per type) is covered by test_izh_dynstdp_static_parity."""
+    import numpy as np
+    from braingrid_amd import bghip as B
+    N, k, seed = 300, 30, 23
+    print(f"\n[stdp_params_per_synapse N={N} k={k} seed={seed}]", flush=True)
+    neu, syn, meta = NW.izh_stdp_arrays(N, seed, k=k)
+    slots = meta["slot"][::7]
+    for f, x in (("taupos", 1.25), ("Apos", 0.8), ("Wex", 1.1)):
+        syn[f] = syn[f].copy()
+        syn[f][slots] = (syn[f][slots] * np.float32(x)).astype(np.float32)
+    ora, cfg = H.izh_static_oracle(neu, syn, meta, seed)
+    for f in ("taupos", "Apos"):
+        ora.view(f)[slots] = syn[f][slots]
+    net = B.Net(hip_ctx, NW.izh_stdp_desc(meta, seed))
+    try:
+        net.upload_neurons(neu)
+        net.upload_synapses(syn, 0)
+        bad = H.compare_state(ora, net, cfg, "upload")
+        assert not bad, bad
+        for c in (700, 1300):
+            ora.advance(c)
+            net.advance(c)
+            bad = H.compare_state(ora, net, cfg, f"step{ora.step}")
+            assert not bad, bad
+        assert net.error_bits() == 0
+        assert ora.array("spikeCount").sum() > 0
+    finally:
+        net.close()
+        ora.close()
