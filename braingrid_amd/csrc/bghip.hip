@@ -1115,6 +1115,7 @@ extern "C" int bghip_add_synapses_ex(bghip_net *n, int64_t count, const int32_t 
             for (int t = 0; t < 4; t++) tp_merge(n, t, p);
         } else {
             // createSynapse's values with the caller's Wex, under the type the device gives the synapse
+            // (syn_type_of: II 0, IE 1, EI 2, EE 3 from the two neurons' types, Layouts/Layout.cpp:129-141)
             for (int64_t i = 0; i < count; i++) {
                 const int ts = n->ntype_host[src[i]], td = n->ntype_host[dest[i]];
                 p.Wex = Wex[i];
