@@ -308,6 +308,7 @@ static void plan_net(bghip_net *n, Carve &c)
     }
     c.add(&dn.syn_count, N); c.add(&dn.slot_hi, N); c.add(&dn.in_map, S); c.add(&dn.in_begin, N);
     c.add(&dn.xloc, N); c.add(&dn.yloc, N); c.add(&dn.radii, N); c.add(&dn.rates, N); c.add(&dn.ntype, N);
+    c.add(&dn.gbox, 5 * ((N + BG_GROWTH_TILE - 1) / BG_GROWTH_TILE));
     c.add(&dn.err, 1);
     c.add(&dn.in_mask, N); c.add(&dn.in_nisi, N); c.add(&dn.in_psr, N); c.add(&dn.in_r, N); c.add(&dn.in_u, N);
     c.add(&dn.in_last, N); c.add(&dn.in_ring, N); c.add(&dn.in_values, N); c.add(&dn.in_shift, N);
@@ -1190,6 +1191,9 @@ static int apply_weights(bghip_net *n, const float *Wd, bghip_growth_stats *stat
     HIP_TRY(n->ctx, hipMemsetAsync(n->d_stats, 0, 16, n->stream));
     if (dn.M > 0) {
         unsigned grid = (unsigned)std::min(dn.N, 4096);
+        if (!Wd)
+            hipLaunchKernelGGL(k_growth_tiles, dim3((unsigned)((dn.N + BG_GROWTH_TILE - 1) / BG_GROWTH_TILE)),
+                               dim3(BG_BLOCK), 0, n->stream, dn);
         hipLaunchKernelGGL(k_growth_dest, dim3(grid), dim3(BG_BLOCK), 0, n->stream, dn, cc, n->d_stats, Wd);
     }
     HIP_TRY(n->ctx, hipGetLastError());
@@ -1639,7 +1643,7 @@ extern "C" int bghip_download_input(bghip_net *n, bghip_input_state *st)
 // Philox noise and Poisson draws are functions of (seed, neuron, step), so the step counter is the
 // whole random-number state.
 static const char kCkptMagic[8] = {'B', 'G', 'H', 'I', 'P', 'N', 'E', 'T'};
-static const uint32_t kCkptVersion = 6;
+static const uint32_t kCkptVersion = 7;
 
 struct CkptHost {   // fixed-size record of the host-side state
     uint64_t step, last_t0, total, hist_step0, in_cyc_off;

@@ -84,6 +84,7 @@ struct DevNet {
     uint32_t *syn_count, *slot_hi, *in_map, *in_begin;
     // growth
     float *xloc, *yloc, *radii, *rates;
+    float *gbox;                 // growth: each 1,024-source tile's min x, max x, min y, max y, max radius
     int32_t *ntype;
     float decay[4];
     int ds_typed;          // 1: every synapse's U, D, F equal its type's creation values

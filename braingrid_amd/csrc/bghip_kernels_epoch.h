@@ -2,6 +2,7 @@
 // Included by bghip.hip only.
 #pragma once
 #include "bghip_kernels.h"
+#include <cfloat>
 
 // Inserts the spikes of the last fused block into the delay rings so that between API calls the
 // rings hold every pending spike, as the reference's rings do after each step.
@@ -108,6 +109,49 @@ __device__ __forceinline__ float pair_weight(const DevNet &n, const float *Wt, i
                                              float rs, float xd, float yd, float rd)
 {
     return Wt ? Wt[(uint64_t)d * (uint64_t)n.N + (uint64_t)s] : overlap_area(xs, ys, rs, xd, yd, rd, s == d);
+}
+
+// Bounding box and largest radius of each BG_GROWTH_TILE-source tile, for k_growth_dest to pass over
+// tiles whose sources cannot overlap a destination (one workgroup per tile; min and max are exact
+// in any order).
+__global__ __launch_bounds__(BG_BLOCK) void k_growth_tiles(DevNet n)
+{
+    __shared__ float red[5][BG_BLOCK / BG_WAVE];
+    const int t = blockIdx.x, tid = threadIdx.x, lane = tid & (BG_WAVE - 1), wv = tid / BG_WAVE;
+    const int s0 = t * BG_GROWTH_TILE, len = min(BG_GROWTH_TILE, n.N - s0);
+    float v[5] = {FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int q = tid; q < len; q += BG_BLOCK) {
+        const float x = n.xloc[s0 + q], y = n.yloc[s0 + q], r = n.radii[s0 + q];
+        v[0] = fminf(v[0], x); v[1] = fmaxf(v[1], x);
+        v[2] = fminf(v[2], y); v[3] = fmaxf(v[3], y);
+        v[4] = fmaxf(v[4], r);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        v[0] = fminf(v[0], __shfl_xor(v[0], off, 64)); v[1] = fmaxf(v[1], __shfl_xor(v[1], off, 64));
+        v[2] = fminf(v[2], __shfl_xor(v[2], off, 64)); v[3] = fmaxf(v[3], __shfl_xor(v[3], off, 64));
+        v[4] = fmaxf(v[4], __shfl_xor(v[4], off, 64));
+    }
+    if (lane == 0)
+        for (int k = 0; k < 5; k++) red[k][wv] = v[k];
+    __syncthreads();
+    if (tid < 5) {
+        float a = red[tid][0];
+        for (int w = 1; w < BG_BLOCK / BG_WAVE; w++) a = (tid & 1) || tid == 4 ? fmaxf(a, red[tid][w]) : fminf(a, red[tid][w]);
+        n.gbox[5 * t + tid] = a;
+    }
+}
+
+// True if no source of the tile with box b can overlap destination (xd, yd, rd): overlap_area is
+// zero unless the distance is below the sum of the radii, and every source lies at least the box
+// distance away with at most the tile's largest radius. The margin far exceeds the rounding of
+// overlap_area's float distance and sum, so a passed-over tile holds no pair with a positive area.
+__device__ __forceinline__ bool growth_tile_far(const float *b, float xd, float yd, float rd)
+{
+    const float dx = fmaxf(0.0f, fmaxf(b[0] - xd, xd - b[1]));
+    const float dy = fmaxf(0.0f, fmaxf(b[2] - yd, yd - b[3]));
+    const float reach = (rd + b[4]) * 1.0001f;
+    return dx * dx + dy * dy > reach * reach;
 }
 
 // In-place transpose of a square N x N float matrix, one 32 x 32 tile pair per workgroup
@@ -379,8 +423,10 @@ __global__ __launch_bounds__(BG_BLOCK) void k_growth_dest(DevNet n, CreateCtx c,
             else if (tid == 0) growth_serial(n, c, d, conn, stats, Wd);
             continue;
         }
-        // rows without holes: the j-th creation (ascending source) takes slot hi0 + j
+        // rows without holes: the j-th creation (ascending source) takes slot hi0 + j; with overlap
+        // weights, tiles no source of which reaches d create nothing and are passed over
         for (int s0 = 0; s0 < N; s0 += BG_GROWTH_TILE) {
+            if (!Wd && growth_tile_far(n.gbox + 5 * (s0 / BG_GROWTH_TILE), xd, yd, rd)) continue;
             const int len = min(BG_GROWTH_TILE, N - s0);
             __syncthreads();
             for (int q = tid; q < len; q += BG_BLOCK) {
